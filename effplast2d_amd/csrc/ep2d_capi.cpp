@@ -1,0 +1,275 @@
+// ep2d_capi.cpp -- extern "C" entry points of libep2d.so (include/ep2d.h).
+// Every call converts C++ exceptions into ep2d_status codes and records the
+// message for ep2d_last_error(); nothing terminates the process.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "ep2d_solver.h"
+
+struct ep2d_handle {
+  ep2d::Solver* solver;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return EP2D_OK;
+  } catch (const ep2d::StatusError& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const ep2d::ArgError& e) {
+    g_last_error = e.what();
+    return EP2D_ERR_ARG;
+  } catch (const ep2d::HipError& e) {
+    g_last_error = e.what();
+    return EP2D_ERR_HIP;
+  } catch (const std::invalid_argument& e) {
+    g_last_error = e.what();
+    return EP2D_ERR_ARG;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return EP2D_ERR_MEMORY;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return EP2D_ERR_STATE;
+  }
+}
+
+ep2d::Solver* S(ep2d_handle* h) {
+  if (!h || !h->solver) throw ep2d::ArgError("null handle");
+  return h->solver;
+}
+}  // namespace
+
+extern "C" {
+
+const char* ep2d_last_error(void) { return g_last_error.c_str(); }
+
+const char* ep2d_version(void) { return "ep2d 0.1 gfx950"; }
+
+int ep2d_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return n;
+}
+
+int ep2d_derive_params(const ep2d_physics* phys, double pa[EP2D_NPARS], int* nX) {
+  return guarded([&] {
+    if (!phys || !pa) throw ep2d::ArgError("null argument");
+    ep2d::derive_params(*phys, pa, nX);
+  });
+}
+
+int ep2d_read_pa(const char* path, double pa[EP2D_NPARS]) {
+  return guarded([&] {
+    if (!path || !pa) throw ep2d::ArgError("null argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) throw ep2d::StatusError(EP2D_ERR_IO, std::string("ERROR:  Cannot open file ") + path + "!");
+    const size_t n = std::fread(pa, sizeof(double), EP2D_NPARS, f);
+    std::fclose(f);
+    if (n != EP2D_NPARS)
+      throw ep2d::StatusError(EP2D_ERR_IO, std::string("short parameter file ") + path);
+  });
+}
+
+int ep2d_write_pa(const char* path, const double pa[EP2D_NPARS]) {
+  return guarded([&] {
+    if (!path || !pa) throw ep2d::ArgError("null argument");
+    FILE* f = std::fopen(path, "wb");
+    if (!f) throw ep2d::StatusError(EP2D_ERR_IO, std::string("cannot create ") + path);
+    const size_t n = std::fwrite(pa, sizeof(double), EP2D_NPARS, f);
+    std::fclose(f);
+    if (n != EP2D_NPARS) throw ep2d::StatusError(EP2D_ERR_IO, std::string("write failed ") + path);
+  });
+}
+
+void ep2d_default_options(ep2d_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->nTasks = 3;            // NL, EffPlast2D.h:25-27
+  o->nIter = 100000;        // NITER, EffPlast2D.h:29-31
+  o->eIter = 1.0e-10;       // EITER, EffPlast2D.h:33-35
+  o->checkInterval = 10000; // output_step, EffPlast2D.cu:1608
+  o->mode = EP2D_MODE_FP64;
+  o->device = 0;            // DEVICE_IDX, EffPlast2D.h:37-39
+  o->nTimeSteps = 1;        // EffPlast2D(const unsigned int nTimeSteps = 1)
+  o->verbose = 0;
+  o->logPath = nullptr;
+}
+
+int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options* opt,
+                ep2d_handle** out) {
+  return guarded([&] {
+    if (!pa || !out) throw ep2d::ArgError("null argument");
+    *out = nullptr;
+    ep2d_options o;
+    if (opt) o = *opt; else ep2d_default_options(&o);
+    auto* h = new ep2d_handle{nullptr};
+    try {
+      h->solver = new ep2d::Solver(pa, nX, nY, o);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ep2d_create_from_physics(const ep2d_physics* phys, const ep2d_options* opt, ep2d_handle** out) {
+  double pa[EP2D_NPARS];
+  int nX = 0;
+  const int rc = ep2d_derive_params(phys, pa, &nX);
+  if (rc) return rc;
+  return ep2d_create(pa, nX, nX, opt, out);
+}
+
+void ep2d_destroy(ep2d_handle* h) {
+  if (!h) return;
+  try {
+    delete h->solver;
+  } catch (...) {
+  }
+  delete h;
+}
+
+int ep2d_compute_eff_moduli(ep2d_handle* h, double initLoadValue, const double loadType[3],
+                            double loadValue, ep2d_result* out) {
+  (void)loadValue;  // deprecated in the reference (EffPlast2D.h:57)
+  return guarded([&] {
+    if (!loadType) throw ep2d::ArgError("null loadType");
+    S(h)->computeEffModuli(initLoadValue, loadType, out);
+  });
+}
+
+int ep2d_compute_eff_damping(ep2d_handle* h, double initLoadValue, const double loadType[3],
+                             double loadValue, ep2d_result* out) {
+  return guarded([&] {
+    if (!loadType) throw ep2d::ArgError("null loadType");
+    S(h)->computeEffDamping(initLoadValue, loadType, loadValue, out);
+  });
+}
+
+int ep2d_get_records(ep2d_handle* h, ep2d_record* out, int capacity, int* count) {
+  return guarded([&] {
+    const auto& rec = S(h)->records();
+    if (count) *count = (int)rec.size();
+    if (out) {
+      const int n = std::min<int>(capacity, (int)rec.size());
+      for (int k = 0; k < n; k++) out[k] = rec[k];
+    }
+  });
+}
+
+int ep2d_field_shape(ep2d_handle* h, int field, int* m, int* n) {
+  return guarded([&] {
+    int a = 0, b = 0;
+    S(h)->fieldShape(field, a, b);
+    if (m) *m = a;
+    if (n) *n = b;
+  });
+}
+
+int ep2d_get_field(ep2d_handle* h, int field, double* host, size_t count) {
+  return guarded([&] {
+    if (!host) throw ep2d::ArgError("null buffer");
+    S(h)->getField(field, host, count);
+  });
+}
+
+int ep2d_set_field(ep2d_handle* h, int field, const double* host, size_t count) {
+  return guarded([&] {
+    if (!host) throw ep2d::ArgError("null buffer");
+    S(h)->setField(field, host, count);
+  });
+}
+
+int ep2d_save_field(ep2d_handle* h, int field, const char* path) {
+  return guarded([&] {
+    if (!path) throw ep2d::ArgError("null path");
+    int m, n;
+    S(h)->fieldShape(field, m, n);
+    std::vector<double> buf((size_t)m * n);
+    S(h)->getField(field, buf.data(), buf.size());
+    FILE* f = std::fopen(path, "wb");
+    if (!f) throw ep2d::StatusError(EP2D_ERR_IO, std::string("cannot create ") + path);
+    const size_t w = std::fwrite(buf.data(), sizeof(double), buf.size(), f);
+    std::fclose(f);
+    if (w != buf.size()) throw ep2d::StatusError(EP2D_ERR_IO, std::string("write failed ") + path);
+  });
+}
+
+int ep2d_get_mask(ep2d_handle* h, unsigned char* host, size_t count) {
+  return guarded([&] {
+    if (!host) throw ep2d::ArgError("null buffer");
+    S(h)->getMask(host, count);
+  });
+}
+
+int ep2d_iterate(ep2d_handle* h, long n) {
+  return guarded([&] {
+    if (n < 0) throw ep2d::ArgError("negative iteration count");
+    S(h)->iterate(n);
+  });
+}
+
+int ep2d_apply_affine(ep2d_handle* h, double dUxdx, double dUydy, double dUxdy, double dUydx) {
+  return guarded([&] { S(h)->applyAffine(dUxdx, dUydy, dUxdy, dUydx); });
+}
+
+int ep2d_residual(ep2d_handle* h, const double curEffStrain[3], double* error, long* nonfinite) {
+  return guarded([&] {
+    if (!curEffStrain || !error) throw ep2d::ArgError("null argument");
+    *error = S(h)->residual(curEffStrain, nonfinite);
+  });
+}
+
+int ep2d_averages(ep2d_handle* h, ep2d_record* rec) {
+  return guarded([&] {
+    if (!rec) throw ep2d::ArgError("null record");
+    S(h)->averages(*rec);
+  });
+}
+
+int ep2d_boundary(ep2d_handle* h, double* deltaP, double* deltaPper, double* tauInfty) {
+  return guarded([&] {
+    double a, b, c;
+    S(h)->boundary(a, b, c);
+    if (deltaP) *deltaP = a;
+    if (deltaPper) *deltaPper = b;
+    if (tauInfty) *tauInfty = c;
+  });
+}
+
+int ep2d_hole_area(ep2d_handle* h, ep2d_record* rec) {
+  return guarded([&] {
+    if (!rec) throw ep2d::ArgError("null record");
+    S(h)->holeArea(*rec);
+  });
+}
+
+int ep2d_set_timing(ep2d_handle* h, long sampleEvery) {
+  return guarded([&] { S(h)->setTiming(sampleEvery); });
+}
+
+int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples, const char** kernelName) {
+  return guarded([&] { S(h)->getTiming(avgKernelMs, samples, kernelName); });
+}
+
+int ep2d_sync(ep2d_handle* h) {
+  return guarded([&] { S(h)->sync(); });
+}
+
+size_t ep2d_device_bytes(ep2d_handle* h) {
+  if (!h || !h->solver) return 0;
+  return h->solver->deviceBytes();
+}
+
+}  // extern "C"

@@ -1,0 +1,105 @@
+// ep2d_internal.h -- shared between the HIP kernels (ep2d_kernels.hip) and
+// the host driver (ep2d_solver.cpp).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace ep2d {
+
+// Pore mask bits, one byte per cell (index j*nX + i).
+enum : unsigned char {
+  M_CELL_MAT = 1,   // SetMaterials pore: K = 0.01 K0, G = 0.01 G0 (EffPlast2D.cu:708-728)
+  M_CELL_ZERO = 2,  // ComputeStress cell test: P = tauXX = tauYY = 0 (EffPlast2D.cu:77-87)
+  M_VERT_ZERO = 4   // ComputeStress vertex test at vertex (i, j): tauXY = 0 (EffPlast2D.cu:88-95)
+};
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define EP2D_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      throw ::ep2d::HipError(std::string(#call) + ": " + hipGetErrorString(e_) +    \
+                             " (" __FILE__ ":" + std::to_string(__LINE__) + ")");   \
+  } while (0)
+
+// Everything an iteration kernel needs about one RVE, passed by value.
+struct RveDev {
+  double* Ux[2];            // displacement, ping-pong in the fused schemes
+  double* Uy[2];
+  void* Vx;                 // velocity: double (plain, FP64) or float (split)
+  void* Vy;
+  const unsigned char* mask;
+  int nX, nY;
+  double dX, dY, dT;
+  double rdX, rdY;          // 1/dX, 1/dY (split scheme only)
+  double K0, Kp, G0, Gp;    // material outside / inside pores (Kp = 0.01*K0)
+  double P0;                // initial pressure (SetInitPressure: 0)
+  double coh;               // yield radius pa[8]
+  double vdampX, vdampY;    // 1 - dT*dampX, 1 - dT*dampY
+  double dTrho;             // dT / rho
+};
+
+// Stress fields in HBM (plain scheme; also the finalised fields of the fused
+// schemes).
+struct StressDev {
+  double* P;
+  double* tauXX;
+  double* tauYY;
+  double* tauXY;
+  double* tauXYav;
+  double* J2;
+  double* J2XY;
+};
+
+// Tile of the fused kernels.
+constexpr int FUSED_TX = 64;
+constexpr int FUSED_TY = 8;
+constexpr int FUSED_NT = 256;
+
+// Partial sums of the volume averages: 11 accumulators.
+constexpr int NAVG = 11;
+enum AvgIdx {
+  A_S0 = 0, A_S1, A_S2, A_SP0, A_SP1, A_SP2, A_SP3, A_E0, A_E1, A_E2, A_SXY
+};
+constexpr int AVG_TX = 64;
+constexpr int AVG_TY = 4;
+
+// ---- launchers (ep2d_kernels.hip) ------------------------------------------
+void launch_build_mask(unsigned char* mask, int nX, int nY, double dX, double dY,
+                       double rad, double nPores, hipStream_t s);
+void launch_fill(double* p, size_t n, double v, hipStream_t s);
+void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
+void launch_affine(double* Ux, double* Uy, int nX, int nY, double dX, double dY,
+                   double dUxdx, double dUydy, double dUxdy, double dUydx, hipStream_t s);
+// plain scheme (also used to finalise the fused schemes' stress fields)
+void launch_plain_stress(const RveDev& r, const double* Ux, const double* Uy,
+                         const StressDev& f, hipStream_t s);
+void launch_plain_j2(const RveDev& r, const StressDev& f, hipStream_t s);
+void launch_plain_plasticity(const RveDev& r, const StressDev& f, hipStream_t s);
+void launch_plain_disp(const RveDev& r, double* Ux, double* Uy, const StressDev& f,
+                       hipStream_t s);
+// fused schemes: reads U[cur], writes U[cur ^ 1], updates V in place
+void launch_fused_fp64(const RveDev& r, int cur, bool plastic, hipStream_t s);
+void launch_fused_split(const RveDev& r, int cur, bool plastic, hipStream_t s);
+const char* fused_kernel_name(int mode);
+// residual: out[0] = bits of max|Vx|, out[1] = bits of max|Vy|, out[2] = nonfinite count
+void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s);
+// volume averages: partial[nblocks*NAVG] then result[NAVG]
+int avg_partial_blocks(int nX, int nY);
+void launch_averages(const RveDev& r, const StressDev& f, const double* Ux,
+                     const double* Uy, double nu0, int perOffX, int perOffY,
+                     double* partial, double* result, hipStream_t s);
+// gather: out[k] = bases[f][table[k].y], f = table[k].x (+2*ucur for ids 3, 4)
+void launch_gather(const double* const* bases, const int2* table, int n, int ucur, double* out,
+                   hipStream_t s);
+void launch_f32_to_f64(const float* src, double* dst, size_t n, hipStream_t s);
+void launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
+
+}  // namespace ep2d

@@ -1,0 +1,912 @@
+// ep2d_solver.cpp -- host driver of one RVE on one MI355X.
+//
+// Mirrors EffPlast2D::ComputeEffModuli / ComputeEffParams (EffPlast2D.cu:179-698)
+// with device-resident state: the iteration loop, the residual, the affine
+// load increment and the FP64 volume averages run on the GPU; only scalars and
+// a few hundred sampled values cross PCIe.  The per-step scalar arithmetic
+// (boundary means, hole area, analytic and final moduli) is evaluated on the
+// host in the reference's operation order (compiled with -ffp-contract=off).
+#include "ep2d_solver.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <sstream>
+
+namespace ep2d {
+
+namespace {
+double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max semantics
+}  // namespace
+
+// get_sigma_2D.m:3-32 (MATLAB evaluates left to right).
+void derive_params(const ep2d_physics& ph, double pa[EP2D_NPARS], int* nXout) {
+  const int nX = ph.nX > 0 ? ph.nX : 32 * ph.nGrid;
+  if (nX < 8) throw ArgError("grid too small: nX = 32*nGrid must be >= 8");
+  if (!(ph.Lx > 0) || !(ph.Ly > 0)) throw ArgError("Lx and Ly must be positive");
+  if (ph.nPores < 1) throw ArgError("nPores must be >= 1");
+  if (!(ph.porosity >= 0) || !(ph.porosity < 1)) throw ArgError("porosity must be in [0, 1)");
+  if (!(ph.K0 > 0) || !(ph.G0 > 0) || !(ph.rho0 > 0) || !(ph.CFL > 0))
+    throw ArgError("K0, G0, rho0 and CFL must be positive");
+  const double N = (double)ph.nPores;
+  const double coh = ph.Y * std::sqrt(2.0);
+  const double rad = std::sqrt(ph.porosity * ph.Lx * ph.Ly / (M_PI * N * N));
+  const double dX = ph.Lx / (double)(nX - 1);
+  const double dY = ph.Ly / (double)(nX - 1);
+  const double mn = dX < dY ? dX : dY;
+  const double dt = ph.CFL * mn / std::sqrt((ph.K0 + 4.0 * ph.G0 / 3.0) / ph.rho0);
+  const double dampX = 4.0 / dt / (double)nX;
+  const double dampY = 4.0 / dt / (double)nX;
+  const double v[EP2D_NPARS] = {dX, dY, dt, ph.K0, ph.G0, ph.rho0, dampX, dampY, coh, rad, N};
+  std::memcpy(pa, v, sizeof(v));
+  if (nXout) *nXout = nX;
+}
+
+// ---------------------------------------------------------------------------
+void* Solver::alloc(size_t bytes) {
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw StatusError(EP2D_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                           " bytes failed: " + hipGetErrorString(e));
+  }
+  allocs_.push_back(p);
+  devBytes_ += bytes;
+  return p;
+}
+
+template <typename T>
+void Solver::h2d(T* dst, const T* src, size_t n) {
+  EP2D_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream_));
+  EP2D_HIP(hipStreamSynchronize(stream_));
+}
+
+template <typename T>
+void Solver::d2h(T* dst, const T* src, size_t n) {
+  EP2D_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream_));
+  EP2D_HIP(hipStreamSynchronize(stream_));
+}
+
+// EffPlast2D::EffPlast2D, EffPlast2D.cu:1541-1612.
+Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt)
+    : nX_(nX), nY_(nY), opt_(opt) {
+  std::memcpy(pa_, pa, sizeof(pa_));
+  if (opt.logPath) logPath_ = opt.logPath;
+  opt_.logPath = nullptr;
+  if (opt_.checkInterval <= 0) opt_.checkInterval = 10000;
+  mode_ = opt_.mode;
+  if (nX < 8 || nY < 8) throw ArgError("nX and nY must be >= 8");
+  if (nX != nY) throw ArgError("the RVE grid must be square (nX == nY), as in the reference");
+  if ((long)(nX + 1) * (long)(nY + 1) > (1L << 30)) throw ArgError("grid too large");
+  if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");
+  if (opt_.nTasks < 1) throw ArgError("nTasks must be >= 1");
+  if (opt_.nIter < 1) throw ArgError("nIter must be >= 1");
+  if (opt_.nTimeSteps < 1) throw ArgError("nTimeSteps must be >= 1");
+  dX_ = pa_[EP2D_PA_DX];
+  dY_ = pa_[EP2D_PA_DY];
+  dT_ = pa_[EP2D_PA_DT];
+  K0_ = pa_[EP2D_PA_K0];
+  G0_ = pa_[EP2D_PA_G0];
+  rho_ = pa_[EP2D_PA_RHO];
+  dampX_ = pa_[EP2D_PA_DAMPX];
+  dampY_ = pa_[EP2D_PA_DAMPY];
+  coh_ = pa_[EP2D_PA_COH];
+  rad_ = pa_[EP2D_PA_RAD];
+  nPores_ = pa_[EP2D_PA_NPORES];
+  if (!(dX_ > 0) || !(dY_ > 0) || !(dT_ > 0) || !(rho_ > 0))
+    throw ArgError("pa: dX, dY, dt and rho0 must be positive");
+  if (!(nPores_ >= 1) || nPores_ != std::floor(nPores_) || nPores_ > 64)
+    throw ArgError("pa: nPores must be an integer in [1, 64]");
+  if (!(rad_ >= 0)) throw ArgError("pa: rad must be >= 0");
+  E0_ = 9.0 * K0_ * G0_ / (3.0 * K0_ + G0_);                  // :1568
+  nu0_ = (1.5 * K0_ - G0_) / (3.0 * K0_ + G0_);               // :1569
+  Y_ = pa_[EP2D_PA_COH] / std::sqrt(2.0);                     // :1572
+  const double lX = (nX_ - 1) * dX_, lY = (nY_ - 1) * dY_;    // :1609-1610
+  porosity_ = 3.1415926 * rad_ * rad_ * nPores_ * nPores_ / (lX * lY);  // :1611
+
+  device_ = opt_.device;
+  int ndev = 0;
+  EP2D_HIP(hipGetDeviceCount(&ndev));
+  if (device_ < 0 || device_ >= ndev) throw ArgError("device index out of range");
+  EP2D_HIP(hipSetDevice(device_));
+  EP2D_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+
+  const size_t nC = (size_t)nX * nY, nV = (size_t)(nX - 1) * (nY - 1);
+  const size_t nUx = (size_t)(nX + 1) * nY, nUy = (size_t)nX * (nY + 1);
+  const bool fused = mode_ != EP2D_MODE_PLAIN;
+  const size_t vbytes = mode_ == EP2D_MODE_SPLIT ? sizeof(float) : sizeof(double);
+  r_.Ux[0] = (double*)alloc(nUx * 8);
+  r_.Uy[0] = (double*)alloc(nUy * 8);
+  r_.Ux[1] = fused ? (double*)alloc(nUx * 8) : r_.Ux[0];
+  r_.Uy[1] = fused ? (double*)alloc(nUy * 8) : r_.Uy[0];
+  r_.Vx = alloc(nUx * vbytes);
+  r_.Vy = alloc(nUy * vbytes);
+  r_.mask = (unsigned char*)alloc(nC);
+  f_.P = (double*)alloc(nC * 8);
+  f_.tauXX = (double*)alloc(nC * 8);
+  f_.tauYY = (double*)alloc(nC * 8);
+  f_.tauXYav = (double*)alloc(nC * 8);
+  f_.J2 = (double*)alloc(nC * 8);
+  f_.tauXY = (double*)alloc(nV * 8);
+  f_.J2XY = (double*)alloc(nV * 8);
+  dMax_ = (unsigned long long*)alloc(4 * sizeof(unsigned long long));
+  dAvgPartial_ = (double*)alloc((size_t)avg_partial_blocks(nX, nY) * NAVG * 8);
+  dAvg_ = (double*)alloc(NAVG * 8);
+  EP2D_HIP(hipHostMalloc((void**)&hMax_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+  EP2D_HIP(hipHostMalloc((void**)&hAvg_, NAVG * sizeof(double), hipHostMallocDefault));
+
+  r_.nX = nX;
+  r_.nY = nY;
+  r_.dX = dX_;
+  r_.dY = dY_;
+  r_.dT = dT_;
+  r_.rdX = 1.0 / dX_;
+  r_.rdY = 1.0 / dY_;
+  r_.K0 = K0_;
+  r_.Kp = 0.01 * K0_;                  // :721
+  r_.G0 = G0_;
+  r_.Gp = 0.01 * G0_;                  // :722
+  r_.P0 = 0.0;                         // SetInitPressure, :770-783
+  r_.coh = coh_;
+  r_.vdampX = 1.0 - dT_ * dampX_;      // :19
+  r_.vdampY = 1.0 - dT_ * dampY_;      // :26
+  r_.dTrho = dT_ / rho_;               // :19
+
+  // SetMaterials / SetMatrixZero (:708-794): masks, zero fields.
+  launch_build_mask(const_cast<unsigned char*>(r_.mask), nX, nY, dX_, dY_, rad_, nPores_, stream_);
+  for (int k = 0; k < 2; k++) {
+    launch_fill(r_.Ux[k], nUx, 0.0, stream_);
+    launch_fill(r_.Uy[k], nUy, 0.0, stream_);
+  }
+  if (mode_ == EP2D_MODE_SPLIT) {
+    launch_fill_f32((float*)r_.Vx, nUx, 0.0f, stream_);
+    launch_fill_f32((float*)r_.Vy, nUy, 0.0f, stream_);
+  } else {
+    launch_fill((double*)r_.Vx, nUx, 0.0, stream_);
+    launch_fill((double*)r_.Vy, nUy, 0.0, stream_);
+  }
+  for (double* p : {f_.P, f_.tauXX, f_.tauYY, f_.tauXYav, f_.J2}) launch_fill(p, nC, 0.0, stream_);
+  for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
+  buildGatherTable();
+  EP2D_HIP(hipStreamSynchronize(stream_));
+
+  if (!logPath_.empty()) log_.open(logPath_, std::ios_base::app);   // :1607
+}
+
+Solver::~Solver() {
+  // teardown: errors are ignored, there is no caller to report them to
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto& e : pending_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (auto& e : freeEvents_) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (void* p : allocs_) (void)hipFree(p);
+  if (hMax_) (void)hipHostFree(hMax_);
+  if (hAvg_) (void)hipHostFree(hAvg_);
+  if (hGather_) (void)hipHostFree(hGather_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Gather table of the sampled values the host post-processing needs:
+// boundary rows/columns of P, tauXX, tauYY (EffPlast2D.cu:825-882, 925-942)
+// and the pore-edge displacement windows (:549-594).
+void Solver::buildGatherTable() {
+  const long nX = nX_, nY = nY_;
+  std::vector<int2> tab;
+  auto cellLine = [&](Line& L, const std::vector<long>& flat) {
+    L.offset = (int)tab.size();
+    L.count = (int)flat.size();
+    for (long q : flat)
+      for (int f = 0; f < 3; f++) tab.push_back(make_int2(f, (int)q));  // P, tauXX, tauYY
+  };
+  std::vector<long> v;
+  v.clear(); for (long i = 1; i < nX - 1; i++) v.push_back(0 * nX + i);            cellLine(hx0_, v);
+  v.clear(); for (long i = 1; i < nX - 1; i++) v.push_back((nY - 1) * nX + i);     cellLine(hxN_, v);
+  v.clear(); for (long j = 1; j < nY - 1; j++) v.push_back(j * nX + 0);            cellLine(hy0_, v);
+  v.clear(); for (long j = 1; j < nY - 1; j++) v.push_back(j * nX + nY - 1);       cellLine(hyN_, v);
+  if (nPores_ > 2) {
+    const long nP = (long)nPores_;
+    v.clear(); for (long i = nX / nP; i < nX * (nP - 1) / nP; i++) v.push_back(nY / nP * nX + i);            cellLine(px0_, v);
+    v.clear(); for (long i = nX / nP; i < nX * (nP - 1) / nP; i++) v.push_back(nY * (nP - 1) / nP * nX + i); cellLine(pxN_, v);
+    v.clear(); for (long j = nY / nP; j < nY * (nP - 1) / nPores_; j++) v.push_back(j * nX + nX / nP);       cellLine(py0_, v);
+    v.clear(); for (long j = nY / nP; j < nY * (nP - 1) / nPores_; j++) v.push_back(j * nX + nX * (nP - 2) / nP); cellLine(pyN_, v);
+  }
+  // pore-edge windows, :552-594 (size_t truncation as in the reference)
+  poreOffset_ = (int)tab.size();
+  samplesValid_ = true;
+  sampleInPore_.clear();
+  const size_t nUx = (size_t)(nX + 1) * nY, nUy = (size_t)nX * (nY + 1);
+  const double Lx = dX_ * (nX - 1), Ly = dY_ * (nY - 1);
+  auto inPore = [&](bool isUy, long i, long j) {   // host U zeroing test, :517-547
+    const double x = isUy ? -0.5 * dX_ * (nX - 1) + dX_ * (int)i : -0.5 * dX_ * nX + dX_ * (int)i;
+    const double y = isUy ? -0.5 * dY_ * nY + dY_ * (int)j : -0.5 * dY_ * (nY - 1) + dY_ * (int)j;
+    bool in = false;
+    for (int k = 0; k < nPores_; k++)
+      for (int l = 0; l < nPores_; l++)
+        if (std::sqrt((x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) * (x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) +
+                      (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l) * (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l)) < rad_)
+          in = true;
+    return in;
+  };
+  auto addSample = [&](bool isUy, size_t flat) {
+    const size_t n = isUy ? nUy : nUx;
+    const size_t m = isUy ? (size_t)nX : (size_t)(nX + 1);
+    if (flat >= n) {
+      samplesValid_ = false;
+      flat = 0;
+    }
+    tab.push_back(make_int2(isUy ? 4 : 3, (int)flat));
+    sampleInPore_.push_back(inPore(isUy, (long)(flat % m), (long)(flat / m)) ? 1 : 0);
+  };
+  for (int k = 0; k < nPores_; k++) {
+    for (int l = 0; l < nPores_; l++) {
+      const double cxdX = 0.5 * (nX - 1) * (1.0 - 1.0 / nPores_) - ((double)(nX - 1) / nPores_) * k;
+      const double cydY = 0.5 * (nY - 1) * (1.0 - 1.0 / nPores_) - ((double)(nY - 1) / nPores_) * l;
+      const size_t cyIdx = (size_t)(cydY + 0.5 * (nY - 1));
+      size_t rxIdx = (size_t)(cxdX - rad_ / dX_ + 0.5 * nX);
+      for (int i = 0; i < 5; i++) addSample(false, cyIdx * (nX + 1) + rxIdx - 1 + i);
+      rxIdx = (size_t)(cxdX + rad_ / dX_ + 0.5 * nX);
+      for (int i = 0; i < 5; i++) addSample(false, cyIdx * (nX + 1) + rxIdx - 2 + i);
+      const size_t cxIdx = (size_t)(cxdX + 0.5 * (nX - 1));
+      size_t ryIdx = (size_t)(cydY - rad_ / dY_ + 0.5 * nY);
+      for (int j = 0; j < 5; j++) addSample(true, (ryIdx - 1 + j) * nX + cxIdx);
+      ryIdx = (size_t)(cydY + rad_ / dY_ + 0.5 * nY);
+      for (int j = 0; j < 5; j++) addSample(true, (ryIdx - 2 + j) * nX + cxIdx);
+    }
+  }
+  nGather_ = (int)tab.size();
+  dTable_ = (int2*)alloc(tab.size() * sizeof(int2));
+  dGather_ = (double*)alloc(tab.size() * sizeof(double));
+  dBases_ = (const double**)alloc(8 * sizeof(double*));
+  EP2D_HIP(hipHostMalloc((void**)&hGather_, std::max<size_t>(tab.size(), 1) * sizeof(double),
+                         hipHostMallocDefault));
+  const double* bases[8] = {f_.P, f_.tauXX, f_.tauYY, r_.Ux[0], r_.Uy[0], r_.Ux[1], r_.Uy[1], nullptr};
+  h2d(dTable_, tab.data(), tab.size());
+  h2d(dBases_, bases, 8);
+}
+
+// ---- iteration ------------------------------------------------------------
+void Solver::launchIteration() {
+  const bool plastic = opt_.nTasks > 1;   // EffPlast2D.cu:392
+  const bool timed = sampleEvery_ > 0 && (launchCount_ % sampleEvery_) == 0;
+  std::pair<hipEvent_t, hipEvent_t> ev{};
+  if (timed) {
+    if (freeEvents_.empty()) {
+      hipEvent_t a, b;
+      EP2D_HIP(hipEventCreate(&a));
+      EP2D_HIP(hipEventCreate(&b));
+      freeEvents_.push_back({a, b});
+    }
+    ev = freeEvents_.back();
+    freeEvents_.pop_back();
+  }
+  if (mode_ == EP2D_MODE_PLAIN) {
+    // EffPlast2D.cu:390-399 (the reference synchronises the device after each
+    // launch; stream order gives the same dependencies)
+    launch_plain_stress(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
+    if (plastic) {
+      launch_plain_j2(r_, f_, stream_);
+      launch_plain_plasticity(r_, f_, stream_);
+    }
+    if (timed) EP2D_HIP(hipEventRecord(ev.first, stream_));
+    launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
+    if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
+  } else {
+    if (timed) EP2D_HIP(hipEventRecord(ev.first, stream_));
+    if (mode_ == EP2D_MODE_FP64)
+      launch_fused_fp64(r_, cur_, plastic, stream_);
+    else
+      launch_fused_split(r_, cur_, plastic, stream_);
+    if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
+    cur_ ^= 1;
+    stressStale_ = true;
+  }
+  if (timed) {
+    pending_.push_back(ev);
+    if (pending_.size() >= 256) resolveTiming(false);
+  }
+  launchCount_++;
+}
+
+void Solver::iterate(long n) {
+  EP2D_HIP(hipSetDevice(device_));
+  for (long k = 0; k < n; k++) launchIteration();
+}
+
+void Solver::resolveTiming(bool all) {
+  const size_t keep = all ? 0 : 64;
+  while (pending_.size() > keep) {
+    auto ev = pending_.front();
+    pending_.erase(pending_.begin());
+    EP2D_HIP(hipEventSynchronize(ev.second));
+    float ms = 0.f;
+    EP2D_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+    timedMs_ += ms;
+    timedSamples_++;
+    freeEvents_.push_back(ev);
+  }
+}
+
+void Solver::setTiming(long sampleEvery) {
+  resolveTiming(true);
+  sampleEvery_ = sampleEvery < 0 ? 0 : sampleEvery;
+  timedMs_ = 0.0;
+  timedSamples_ = 0;
+  launchCount_ = 0;
+}
+
+void Solver::getTiming(double* avgMs, long* samples, const char** name) {
+  resolveTiming(true);
+  if (avgMs) *avgMs = timedSamples_ ? timedMs_ / timedSamples_ : 0.0;
+  if (samples) *samples = timedSamples_;
+  if (name) *name = fused_kernel_name(mode_);
+}
+
+void Solver::sync() { EP2D_HIP(hipStreamSynchronize(stream_)); }
+
+// Stress fields as the reference holds them after the loop: the trial stress
+// of U^{n-1} after the return mapping (the fused schemes keep it on chip, so
+// it is recomputed here in FP64 from the previous displacement buffer).
+void Solver::ensureStress() {
+  if (!stressStale_) return;
+  const int prev = cur_ ^ 1;
+  launch_plain_stress(r_, r_.Ux[prev], r_.Uy[prev], f_, stream_);
+  if (opt_.nTasks > 1) {
+    launch_plain_j2(r_, f_, stream_);
+    launch_plain_plasticity(r_, f_, stream_);
+  }
+  stressStale_ = false;
+}
+
+void Solver::applyAffine(double dUxdx, double dUydy, double dUxdy, double dUydx) {
+  launch_affine(r_.Ux[cur_], r_.Uy[cur_], nX_, nY_, dX_, dY_, dUxdx, dUydy, dUxdy, dUydx, stream_);
+}
+
+// EffPlast2D.cu:407-412.
+double Solver::residual(const double e[3], long* nonfinite) {
+  launch_maxabs(r_, mode_ == EP2D_MODE_SPLIT, dMax_, stream_);
+  d2h(hMax_, dMax_, 3);
+  double mx, my;
+  std::memcpy(&mx, &hMax_[0], 8);
+  std::memcpy(&my, &hMax_[1], 8);
+  if (nonfinite) *nonfinite = (long)hMax_[2];
+  return (mx / (dX_ * (nX_ - 1)) + my / (dY_ * (nY_ - 1))) * dT_ /
+         (std_max(std::fabs(e[0]), std_max(e[1], e[2])));
+}
+
+// Volume averages, EffPlast2D.cu:453-503.
+void Solver::averages(ep2d_record& r) {
+  ensureStress();
+  const int pox = nX_ / nPores_;   // :453-454 (double division, truncated)
+  const int poy = nY_ / nPores_;
+  launch_averages(r_, f_, r_.Ux[cur_], r_.Uy[cur_], nu0_, pox, poy, dAvgPartial_, dAvg_, stream_);
+  d2h(hAvg_, dAvg_, NAVG);
+  const long nX = nX_, nY = nY_;
+  r.sigma[0] = hAvg_[A_S0] / (nX * nY);
+  r.sigma[1] = hAvg_[A_S1] / (nX * nY);
+  r.sigma[2] = hAvg_[A_S2] / ((nX - 1) * (nY - 1));
+  r.sigma[3] = 0.0 / (nX * nY);
+  const long perArea = (nX - 2 * pox) * (nY - 2 * poy);
+  r.sigmaPer[0] = hAvg_[A_SP0] / perArea;
+  r.sigmaPer[1] = hAvg_[A_SP1] / perArea;
+  r.sigmaPer[2] = hAvg_[A_SP2] / perArea;
+  r.sigmaPer[3] = hAvg_[A_SP3] / perArea;
+  r.epsilonPer[0] = hAvg_[A_E0] / perArea;
+  r.epsilonPer[1] = hAvg_[A_E1] / perArea;
+  r.epsilonPer[2] = hAvg_[A_E2] / perArea;
+  r.sigmaXYclean = hAvg_[A_SXY] / (double)((nX - 1) * (nY - 1));
+}
+
+// getDeltaP_honest (:825-853), getDeltaP_periodic (:854-882),
+// getTauInfty_honest (:925-942): summed on the host in the reference order
+// from gathered boundary values.
+void Solver::boundary(double& deltaP, double& deltaPper, double& tauInfty) {
+  ensureStress();
+  launch_gather(dBases_, dTable_, nGather_, cur_, dGather_, stream_);
+  d2h(hGather_, dGather_, (size_t)nGather_);
+  const double* g = hGather_;
+  auto P = [&](const Line& L, int q) { return g[(L.offset + 3 * q) + 0]; };
+  auto XX = [&](const Line& L, int q) { return g[(L.offset + 3 * q) + 1]; };
+  auto YY = [&](const Line& L, int q) { return g[(L.offset + 3 * q) + 2]; };
+  {
+    double dPx = 0.0, dPy = 0.0;
+    for (int q = 0; q < hx0_.count; q++) {
+      dPx += XX(hx0_, q) - P(hx0_, q);
+      dPx += YY(hx0_, q) - P(hx0_, q);
+      dPx += XX(hxN_, q) - P(hxN_, q);
+      dPx += YY(hxN_, q) - P(hxN_, q);
+    }
+    dPx /= (nX_ - 2);
+    for (int q = 0; q < hy0_.count; q++) {
+      dPy += XX(hy0_, q) - P(hy0_, q);
+      dPy += YY(hy0_, q) - P(hy0_, q);
+      dPy += XX(hyN_, q) - P(hyN_, q);
+      dPy += YY(hyN_, q) - P(hyN_, q);
+    }
+    dPy /= (nY_ - 2);
+    deltaP = -0.125 * (dPx + dPy);
+  }
+  if (nPores_ <= 2) {
+    deltaPper = 0.0;
+  } else {
+    const long nP = (long)nPores_;
+    double dPx = 0.0, dPy = 0.0;
+    for (int q = 0; q < px0_.count; q++) {
+      dPx += XX(px0_, q) - P(px0_, q);
+      dPx += YY(px0_, q) - P(px0_, q);
+      dPx += XX(pxN_, q) - P(pxN_, q);
+      dPx += YY(pxN_, q) - P(pxN_, q);
+    }
+    dPx /= (long)nX_ * (nP - 2) / nP;
+    for (int q = 0; q < py0_.count; q++) {
+      dPy += XX(py0_, q) - P(py0_, q);
+      dPy += YY(py0_, q) - P(py0_, q);
+      dPy += XX(pyN_, q) - P(pyN_, q);
+      dPy += YY(pyN_, q) - P(pyN_, q);
+    }
+    dPy /= (long)nY_ * (nP - 2) / nP;
+    deltaPper = -0.125 * (dPx + dPy);
+  }
+  {
+    double tx = 0.0, ty = 0.0;
+    for (int q = 0; q < hx0_.count; q++) {
+      tx += XX(hx0_, q) - YY(hx0_, q);
+      tx += XX(hxN_, q) - YY(hxN_, q);
+    }
+    tx /= (nX_ - 2);
+    for (int q = 0; q < hy0_.count; q++) {
+      ty += XX(hy0_, q) - YY(hy0_, q);
+      ty += XX(hyN_, q) - YY(hyN_, q);
+    }
+    ty /= (nY_ - 2);
+    tauInfty = -0.125 * (tx + ty);
+  }
+}
+
+namespace {
+double find_max_abs_signed(const double* v, int n) {   // EffPlast2D.cu:814-822
+  double m = 0.0;
+  for (int k = 0; k < n; k++)
+    if (std::fabs(v[k]) > std::fabs(m)) m = v[k];
+  return m;
+}
+}  // namespace
+
+// Pore-edge hole area, EffPlast2D.cu:514-620.  Uses the values gathered by
+// the last boundary() call (same state).
+void Solver::holeArea(ep2d_record& r) {
+  if (!samplesValid_)
+    throw StatusError(EP2D_ERR_GEOMETRY,
+                      "pore-edge sample window outside the grid (pores too large or too close "
+                      "to the boundary)");
+  launch_gather(dBases_, dTable_, nGather_, cur_, dGather_, stream_);
+  d2h(hGather_, dGather_, (size_t)nGather_);
+  double HoleAreaPi = 0.0, InternalHoleAreaPi = 0.0;
+  int s = 0;
+  auto take = [&](double* w) {
+    for (int q = 0; q < 5; q++, s++)
+      w[q] = sampleInPore_[s] ? 0.0 : hGather_[poreOffset_ + s];
+  };
+  const double nP = nPores_;
+  for (int k = 0; k < nP; k++) {
+    for (int l = 0; l < nP; l++) {
+      double left[5], right[5], bottom[5], top[5];
+      take(left);
+      take(right);
+      take(bottom);
+      take(top);
+      const double dRx = -0.5 * (find_max_abs_signed(left, 5) - find_max_abs_signed(right, 5));
+      const double dRy = -0.5 * (find_max_abs_signed(bottom, 5) - find_max_abs_signed(top, 5));
+      HoleAreaPi += (rad_ + dRx) * (rad_ + dRy);
+      if (k > 0 && l > 0 && k < nP - 1 && l < nP - 1) InternalHoleAreaPi += (rad_ + dRx) * (rad_ + dRy);
+    }
+  }
+  const long nX = nX_, nY = nY_;
+  const double rad = rad_, nPores = nPores_, dX = dX_, dY = dY_;
+  r.holeAreaPi = HoleAreaPi;
+  r.internalHoleAreaPi = InternalHoleAreaPi;
+  r.Phi0 = 3.1415926 * rad * rad * nPores * nPores / (dX * (nX - 1) * dY * (nY - 1));
+  r.Phi = 3.1415926 * HoleAreaPi / (dX * (nX - 1) * dY * (nY - 1));
+  r.PhiPer = nPores > 2 ?
+    3.1415926 * InternalHoleAreaPi / (dX * (nX - 1) * dY * (nY - 1) * (nPores - 2) * (nPores - 2) / nPores / nPores) :
+    0.0;
+  r.dPhi = 3.1415926 * std::fabs(HoleAreaPi - rad * rad * nPores * nPores) / (dX * (nX - 1) * dY * (nY - 1));
+  r.dPhiPer = std::fabs(r.PhiPer - r.Phi0);
+}
+
+// ---- ComputeEffParams -------------------------------------------------------
+void Solver::computeEffParams(int step, double loadStepValue, const double loadType[3], long nT,
+                              ep2d_result* res) {
+  const int NL = opt_.nTasks;
+  printStepInfo(step);
+  if (step == 0) {   // :329-333: the macro strain and U restart, V does not
+    curEffStrain_[0] = curEffStrain_[1] = curEffStrain_[2] = 0.0;
+    launch_fill(r_.Ux[cur_], (size_t)(nX_ + 1) * nY_, 0.0, stream_);
+    launch_fill(r_.Uy[cur_], (size_t)nX_ * (nY_ + 1), 0.0, stream_);
+  }
+  for (long it = 0; it < nT; it++) {
+    ep2d_record r;
+    std::memset(&r, 0, sizeof(r));
+    say("Time step ", it + 1, " from ", nT, "\n");
+    const double dUxdx = loadStepValue * loadType[0] / (double)nT;   // :349-352
+    const double dUydy = loadStepValue * loadType[1] / (double)nT;
+    const double dUxdy = loadStepValue * loadType[2] / (double)nT;
+    const double dUydx = dUxdy;
+    curEffStrain_[0] += dUxdx;
+    curEffStrain_[1] += dUydy;
+    curEffStrain_[2] += dUxdy;
+    for (int k = 0; k < 3; k++) r.epsilon[k] = curEffStrain_[k];
+    say("Macro strain: (", curEffStrain_[0], ", ", curEffStrain_[1], ", ", curEffStrain_[2], ")\n");
+    applyAffine(dUxdx, dUydy, dUxdy, dUydx);
+
+    // iteration loop with a residual check every checkInterval iterations (:389-428)
+    double error = 0.0;
+    long done = 0;
+    bool conv = false;
+    const long K = opt_.checkInterval;
+    while (done < opt_.nIter) {
+      const long chunk = std::min(K - (done % K), opt_.nIter - done);
+      iterate(chunk);
+      done += chunk;
+      if (done % K == 0) {
+        long bad = 0;
+        error = residual(curEffStrain_, &bad);
+        if (bad)
+          throw StatusError(EP2D_ERR_NONFINITE, "non-finite velocity at iteration " +
+                                                    std::to_string(done) + " of load step " +
+                                                    std::to_string(step));
+        say("    Iteration ", done, ": Error is ", error, "\n");
+        if (error < opt_.eIter) {
+          say("Number of iterations is ", done, "\n\n");
+          conv = true;
+          break;
+        } else if (done >= opt_.nIter) {
+          say("WARNING: Maximum number of iterations reached!\nError is ", error, "\n\n");
+        }
+      }
+    }
+    r.error = error;
+    r.iters = (double)done;
+    totalIters_ += done;
+    if (!conv) allConverged_ = false;
+
+    averages(r);
+    boundary(r.deltaP, r.deltaPper, r.tauInfty);
+    holeArea(r);
+    logOnly("    dPhi = ", r.dPhi, "\n");
+    if (nPores_ > 2) logOnly("    dPhiPer = ", r.dPhiPer, "\n");
+    say("    P / Y = ", r.deltaP / Y_, "\n");
+    if (nPores_ > 2) say("    Pper / Y = ", r.deltaPper / Y_, "\n");
+    say("    tau / Y = ", r.tauInfty / Y_, "\n");
+
+    // analytic moduli of this step (:643-696)
+    const long nX = nX_, nY = nY_;
+    const double Phi0 = r.Phi0, Phi = r.Phi;
+    const double phi = 3.1415926 * rad_ * rad_ / (dX_ * (nX - 1) * dY_ * (nY - 1));
+    const double Y = Y_, G0 = G0_, K0 = K0_;
+    if (NL <= 1) {
+      res->KphiExactElast = G0 / Phi0;
+      logOnly("    KphiExactElast = ", res->KphiExactElast, "\n");
+      res->an.Kphi = G0 / Phi0;
+      res->KdExactElast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactElast);
+      logOnly("    KdExactElast = ", res->KdExactElast, "\n");
+      res->an.Kd = G0 / Phi0;
+    } else if (step == 0) {
+      res->KphiExactPlast = G0 / Phi / std::exp(std::fabs(r.deltaP) / Y - 1.0) /
+                            (1.0 + r.tauInfty * r.tauInfty / Y / Y);
+      logOnly("    KphiExactPlast = ", res->KphiExactPlast, "\n");
+      res->an.Kphi = res->KphiExactPlast;
+      res->KdExactPlast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlast);
+      logOnly("    KdExactPlast = ", res->KdExactPlast, "\n");
+      res->an.Kd = res->KdExactPlast;
+      res->KphiExactPlastPer = G0 / Phi0 / std::exp(std::fabs(r.deltaPper) / Y - 1.0) /
+                               (1.0 + r.tauInfty * r.tauInfty / Y / Y);
+      logOnly("    KphiExactPlastPer = ", res->KphiExactPlastPer, "\n");
+      res->an_per.Kphi = res->KphiExactPlastPer;
+      res->KdExactPlastPer = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlastPer);
+      logOnly("    KdExactPlastPer = ", res->KdExactPlastPer, "\n");
+      res->an_per.Kd = res->KdExactPlastPer;
+    }
+    if (NL == 3 && step == 0) {
+      res->GexactElast = G0 / (1.0 + Phi0);
+      logOnly("    GexactElast = ", res->GexactElast, "\n");
+      res->GexactPlast = G0 / (1.0 + Phi * std::exp(std::fabs(r.deltaP) / Y - 1.0));
+      logOnly("    GexactPlast = ", res->GexactPlast, "\n");
+      res->an.G = res->GexactPlast;
+      res->GexactPlastPer = G0 / (1.0 + Phi0 * std::exp(std::fabs(r.deltaPper) / Y - 1.0));
+      logOnly("    GexactPlastPer = ", res->GexactPlastPer, "\n");
+      res->an_per.G = res->GexactPlastPer;
+    }
+    records_.push_back(r);
+    recordStep_.push_back(step);
+  }
+}
+
+// EffPlast2D::ComputeEffModuli (EffPlast2D.cu:179-229) and the moduli getters
+// (:1412-1539).
+void Solver::computeEffModuli(double initLoadValue, const double loadType[3], ep2d_result* out) {
+  const int NL = opt_.nTasks;
+  if (NL > 3) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
+  EP2D_HIP(hipSetDevice(device_));
+  const auto t0 = std::chrono::steady_clock::now();
+  ep2d_result res;
+  std::memset(&res, 0, sizeof(res));
+  records_.clear();
+  recordStep_.clear();
+  totalIters_ = 0;
+  allConverged_ = true;
+  std::memcpy(loadType_, loadType, sizeof(loadType_));
+  const double incPercent = 0.005;   // EffPlast2D.h:105
+  const double sph[3] = {0.5 * (loadType[0] + loadType[1]), 0.5 * (loadType[0] + loadType[1]), 0.0};
+  double dev[3] = {loadType[0] - sph[0], loadType[1] - sph[1], loadType[2]};
+  if (opt_.useShearLoadType) std::memcpy(dev, opt_.shearLoadType, sizeof(dev));
+  const long T = opt_.nTimeSteps;
+
+  printCalculationType();
+  computeEffParams(0, initLoadValue, loadType, T, &res);
+  const ep2d_record r0 = records_[T - 1];
+  if (NL == 1) {   // calcBulkModuli_PureElast, :1412-1481
+    res.num.Kphi = r0.deltaP / r0.dPhi;
+    logOnly("    ==============\n", "    Kphi = ", res.num.Kphi, "\n");
+    res.num_per.Kphi = r0.deltaPper / r0.dPhiPer;
+    logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
+    const double pInc = -0.33333333 * (r0.sigma[0] + r0.sigma[1] + r0.sigma[2]);
+    const double epsInc = r0.epsilon[0] + r0.epsilon[1];
+    res.num.Kd = -pInc / epsInc;
+    logOnly("    Kd = ", res.num.Kd, "\n");
+    const double pIncPer = -0.33333333 * (r0.sigmaPer[0] + r0.sigmaPer[1] + r0.sigmaPer[2]);
+    const double epsIncPer = r0.epsilonPer[0] + r0.epsilonPer[1];
+    res.num_per.Kd = -pIncPer / epsIncPer;
+    logOnly("    KdPer = ", res.num_per.Kd, "\n");
+  } else {         // calcBulkModuli_ElastPlast, :1429-1514
+    computeEffParams(1, initLoadValue * incPercent, sph, 1, &res);
+    const ep2d_record r1 = records_[T];
+    res.num.Kphi = (r1.deltaP - r0.deltaP) / (r1.dPhi - r0.dPhi);
+    logOnly("==============\n", "    Kphi = ", res.num.Kphi, "\n");
+    res.num_per.Kphi = (r1.deltaPper - r0.deltaPper) / (r1.dPhiPer - r0.dPhiPer);
+    logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
+    const double pInc = -0.33333333 * (r1.sigma[0] + r1.sigma[1] + r1.sigma[2] - r0.sigma[0] - r0.sigma[1] - r0.sigma[2]);
+    const double epsInc = r1.epsilon[0] + r1.epsilon[1] - r0.epsilon[0] - r0.epsilon[1];
+    res.num.Kd = -pInc / epsInc;
+    logOnly("    Kd = ", res.num.Kd, "\n");
+    const double pIncPer = -0.33333333 * (r1.sigmaPer[0] + r1.sigmaPer[1] + r1.sigmaPer[2] - r0.sigmaPer[0] - r0.sigmaPer[1] - r0.sigmaPer[2]);
+    const double epsIncPer = r1.epsilonPer[0] + r1.epsilonPer[1] - r0.epsilonPer[0] - r0.epsilonPer[1];
+    res.num_per.Kd = -pIncPer / epsIncPer;
+    logOnly("    KdPer = ", res.num_per.Kd, "\n");
+  }
+  if (NL == 3) {   // calcShearModulus, :1446-1539
+    computeEffParams(2, initLoadValue * incPercent, dev, 1, &res);
+    const ep2d_record r1 = records_[T];
+    const ep2d_record r2 = records_[T + 1];
+    res.num.G = 0.5 * (r2.sigma[0] - r1.sigma[0] - r2.sigma[1] + r1.sigma[1]) /
+                (r2.epsilon[0] - r1.epsilon[0] - r2.epsilon[1] + r1.epsilon[1]);
+    logOnly("==============\n", "    G = ", res.num.G, "\n");
+    res.num_per.G = 0.5 * (r2.sigmaPer[0] - r1.sigmaPer[0] - r2.sigmaPer[1] + r1.sigmaPer[1]) /
+                    (r2.epsilonPer[0] - r1.epsilonPer[0] - r2.epsilonPer[1] + r1.epsilonPer[1]);
+    logOnly("    Gper = ", res.num_per.G, "\n");
+    res.Gxy = 0.5 * (r2.sigmaXYclean - r1.sigmaXYclean) / (r2.epsilon[2] - r1.epsilon[2]);
+    res.GxyPer = 0.5 * (r2.sigmaPer[2] - r1.sigmaPer[2]) / (r2.epsilonPer[2] - r1.epsilonPer[2]);
+  }
+  printEffectiveModuli(res);
+  printWarnings();
+  const auto t1 = std::chrono::steady_clock::now();
+  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  res.nRecords = (int)records_.size();
+  res.totalIters = totalIters_;
+  res.converged = allConverged_ ? 1 : 0;
+  say("\nCalculation time is ", (int)res.seconds, " sec\n");
+  if (out) *out = res;
+}
+
+// EffPlast2D::ComputeEffDamping (EffPlast2D.cu:230-310).
+void Solver::computeEffDamping(double initLoadValue, const double loadType[3], double loadValue,
+                               ep2d_result* out) {
+  const int NL = opt_.nTasks;
+  if (NL < 4) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
+  EP2D_HIP(hipSetDevice(device_));
+  const auto t0 = std::chrono::steady_clock::now();
+  ep2d_result res;
+  std::memset(&res, 0, sizeof(res));
+  records_.clear();
+  recordStep_.clear();
+  totalIters_ = 0;
+  allConverged_ = true;
+  std::memcpy(loadType_, loadType, sizeof(loadType_));
+  printCalculationType();
+  const long T = opt_.nTimeSteps;
+  computeEffParams(0, initLoadValue, loadType, T, &res);
+  for (long i = 1; i < NL; i++) {
+    const double v = loadValue * (std::sin(i * 2.0 * 3.1415926 / (NL - 1)) -
+                                  std::sin((i - 1) * 2.0 * 3.1415926 / (NL - 1)));
+    computeEffParams((int)i, v, loadType, 1, &res);
+  }
+  auto printTensors = [&](const char* title, auto get, int n) {
+    say("\n", title, ":\n", "0) ");
+    for (int j = 0; j < n; j++) say(get(records_[T - 1])[j], " ");
+    say("\n");
+    for (long i = 1; i < NL; i++) {
+      say(i, ") ");
+      for (int j = 0; j < n; j++) say(get(records_[T - 1 + i])[j], " ");
+      say("\n");
+    }
+  };
+  printTensors("Strain tensors", [](const ep2d_record& r) { return r.epsilon; }, 3);
+  printTensors("Strain tensors periodic", [](const ep2d_record& r) { return r.epsilonPer; }, 3);
+  printTensors("Stress tensors", [](const ep2d_record& r) { return r.sigma; }, 4);
+  printTensors("Stress tensors periodic", [](const ep2d_record& r) { return r.sigmaPer; }, 4);
+  const auto t1 = std::chrono::steady_clock::now();
+  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  res.nRecords = (int)records_.size();
+  res.totalIters = totalIters_;
+  res.converged = allConverged_ ? 1 : 0;
+  if (out) *out = res;
+}
+
+// ---- fields -------------------------------------------------------------------
+void Solver::fieldShape(int field, int& m, int& n) const {
+  switch (field) {
+    case EP2D_FIELD_P: case EP2D_FIELD_TAUXX: case EP2D_FIELD_TAUYY: case EP2D_FIELD_TAUXYAV:
+    case EP2D_FIELD_J2: case EP2D_FIELD_K: case EP2D_FIELD_G:
+      m = nX_; n = nY_; return;
+    case EP2D_FIELD_TAUXY: case EP2D_FIELD_J2XY:
+      m = nX_ - 1; n = nY_ - 1; return;
+    case EP2D_FIELD_UX: case EP2D_FIELD_VX:
+      m = nX_ + 1; n = nY_; return;
+    case EP2D_FIELD_UY: case EP2D_FIELD_VY:
+      m = nX_; n = nY_ + 1; return;
+    default:
+      throw ArgError("unknown field id " + std::to_string(field));
+  }
+}
+
+void Solver::getField(int field, double* host, size_t count) {
+  int m, n;
+  fieldShape(field, m, n);
+  const size_t N = (size_t)m * n;
+  if (count < N) throw ArgError("host buffer too small for field");
+  EP2D_HIP(hipSetDevice(device_));
+  switch (field) {
+    case EP2D_FIELD_P: ensureStress(); d2h(host, f_.P, N); return;
+    case EP2D_FIELD_TAUXX: ensureStress(); d2h(host, f_.tauXX, N); return;
+    case EP2D_FIELD_TAUYY: ensureStress(); d2h(host, f_.tauYY, N); return;
+    case EP2D_FIELD_TAUXY: ensureStress(); d2h(host, f_.tauXY, N); return;
+    case EP2D_FIELD_TAUXYAV: ensureStress(); d2h(host, f_.tauXYav, N); return;
+    case EP2D_FIELD_J2: ensureStress(); d2h(host, f_.J2, N); return;
+    case EP2D_FIELD_J2XY: ensureStress(); d2h(host, f_.J2XY, N); return;
+    case EP2D_FIELD_UX: d2h(host, r_.Ux[cur_], N); return;
+    case EP2D_FIELD_UY: d2h(host, r_.Uy[cur_], N); return;
+    case EP2D_FIELD_VX: case EP2D_FIELD_VY: {
+      void* src = field == EP2D_FIELD_VX ? r_.Vx : r_.Vy;
+      if (mode_ == EP2D_MODE_SPLIT) {
+        std::vector<float> tmp(N);
+        d2h(tmp.data(), (const float*)src, N);
+        for (size_t k = 0; k < N; k++) host[k] = tmp[k];
+      } else {
+        d2h(host, (const double*)src, N);
+      }
+      return;
+    }
+    case EP2D_FIELD_K: case EP2D_FIELD_G: {
+      std::vector<unsigned char> msk(N);
+      d2h(msk.data(), r_.mask, N);
+      const double v0 = field == EP2D_FIELD_K ? K0_ : G0_;
+      const double vp = field == EP2D_FIELD_K ? r_.Kp : r_.Gp;
+      for (size_t k = 0; k < N; k++) host[k] = (msk[k] & M_CELL_MAT) ? vp : v0;
+      return;
+    }
+  }
+}
+
+void Solver::setField(int field, const double* host, size_t count) {
+  int m, n;
+  fieldShape(field, m, n);
+  const size_t N = (size_t)m * n;
+  if (count < N) throw ArgError("host buffer too small for field");
+  EP2D_HIP(hipSetDevice(device_));
+  switch (field) {
+    case EP2D_FIELD_UX: h2d(r_.Ux[cur_], host, N); return;
+    case EP2D_FIELD_UY: h2d(r_.Uy[cur_], host, N); return;
+    case EP2D_FIELD_VX: case EP2D_FIELD_VY: {
+      void* dst = field == EP2D_FIELD_VX ? r_.Vx : r_.Vy;
+      if (mode_ == EP2D_MODE_SPLIT) {
+        std::vector<float> tmp(host, host + N);
+        h2d((float*)dst, tmp.data(), N);
+      } else {
+        h2d((double*)dst, host, N);
+      }
+      return;
+    }
+    default:
+      throw ArgError("only Ux, Uy, Vx and Vy can be set");
+  }
+}
+
+void Solver::getMask(unsigned char* host, size_t count) {
+  const size_t N = (size_t)nX_ * nY_;
+  if (count < N) throw ArgError("host buffer too small for mask");
+  d2h(host, r_.mask, N);
+}
+
+// ---- console / log output (EffPlast2D.cu:1295-1375) -----------------------------
+template <typename... A>
+void Solver::say(const A&... a) {
+  if (!opt_.verbose && !log_.is_open()) return;
+  std::ostringstream os;
+  (os << ... << a);
+  if (opt_.verbose) std::cout << os.str();
+  if (log_.is_open()) log_ << os.str();
+}
+
+template <typename... A>
+void Solver::logOnly(const A&... a) {
+  if (!log_.is_open()) return;
+  std::ostringstream os;
+  (os << ... << a);
+  log_ << os.str();
+}
+
+void Solver::printStepInfo(int step) {
+  const int NL = opt_.nTasks;
+  say("\nLOAD STEP ", step + 1, " FROM ", NL, ": ");
+  if (NL <= 3) {
+    switch (step) {
+      case 0: say("PRELOADING\n"); break;
+      case 1: say("SMALL HYDROSTATIC INCREMENT\n"); break;
+      case 2: say("SMALL DEVIATORIC INCREMENT\n"); break;
+      default: throw ArgError("ERROR:  Wrong step index!");
+    }
+  }
+  say("Porosity is ", porosity_ * 100, "%\n");
+  say("Grid resolution is ", nX_, "x", nY_, "\n\n");
+}
+
+void Solver::printCalculationType() {
+  switch (opt_.nTasks) {
+    case 1: say("\nPURE ELASTIC CALCULATION\nESTIMATION OF THE EFFECTIVE BULK MODULI\n"); break;
+    case 2: say("\nELASTOPLASTIC CALCULATION\nESTIMATION OF THE EFFECTIVE BULK MODULI\n"); break;
+    case 3: say("\nELASTOPLASTIC CALCULATION\nESTIMATION OF THE EFFECTIVE BULK MODULI AND THE EFFECTIVE SHEAR MODULUS\n"); break;
+    default: say("\nELASTOPLASTIC CALCULATION\nESTIMATION OF THE EFFECTIVE DAMPING\n");
+  }
+}
+
+void Solver::printEffectiveModuli(const ep2d_result& r) {
+  auto out = [&](const ep2d_moduli& m) {
+    say("        Kphi = ", m.Kphi, "\n");
+    say("        Kd = ", m.Kd, "\n");
+    if (opt_.nTasks > 2) say("        G = ", m.G, "\n");
+  };
+  say("\nEFFECTIVE MODULI ESTIMATED\nNON-PERIODIC\n    Analytical:\n");
+  out(r.an);
+  say("    Numerical:\n");
+  out(r.num);
+  if (nPores_ > 2) {
+    say("PERIODIC\n    Analytical:\n");
+    out(r.an_per);
+    say("    Numerical:\n");
+    out(r.num_per);
+  }
+}
+
+void Solver::printWarnings() {
+  if (opt_.nTasks <= 1 || records_.empty()) return;
+  const ep2d_record& r0 = records_[opt_.nTimeSteps - 1];
+  if (std::fabs(r0.deltaP) / Y_ < 1.0 || std::fabs(r0.deltaP) / Y_ > 5.0 ||
+      std::fabs(r0.tauInfty) / Y_ > 0.42) {
+    say("\nWARNING:  Analytical solution is not applicable for this load!\n");
+    say("TODO: Set the load so that 1 < |P|/Y < 5 and 0 < |tau|/Y < 0.42\n");
+  }
+  if (std::fabs(loadType_[0] - loadType_[1]) > 0.05 * std::max(std::fabs(loadType_[0]), std::fabs(loadType_[1])) ||
+      std::fabs(loadType_[2]) > 0.01 * std::max(std::fabs(loadType_[0]), std::fabs(loadType_[1]))) {
+    say("\nWARNING:  The numerical effective Kphi is not calculated quite accurately for non-hydrostatic load!\n");
+  }
+}
+
+}  // namespace ep2d

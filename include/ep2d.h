@@ -1,0 +1,221 @@
+/*
+ * ep2d.h -- C ABI of the MI355X (gfx950) EffPlast2D solver: libep2d.so.
+ *
+ * The library replaces the reference's CUDA class EffPlast2D
+ * (EffPlast2D.h:53-198, EffPlast2D.cu:179-1658) and its four kernels
+ * (EffPlast2D.cu:3-177).  The reference configures itself from compile-time
+ * macros (NGRID, NL, NITER, EITER, DEVICE_IDX; EffPlast2D.h:17-39) plus the
+ * binary parameter file data/pa.dat (get_sigma_2D.m:32,41-43); here every
+ * one of those is a runtime field of ep2d_options / ep2d_physics.
+ *
+ * Conventions
+ *  - Every entry point returns EP2D_OK (0) or a negative ep2d_status; the
+ *    message of the last failure on the calling thread is ep2d_last_error().
+ *    Nothing calls exit() (the reference's gpuAssert does, EffPlast2D.h:42-49).
+ *  - Arrays use the reference layout A[j*m + i] (x fastest), FP64:
+ *      cell fields   P, tauXX, tauYY, tauXYav, J2, K, G:  m = nX,   n = nY
+ *      vertex fields tauXY, J2XY:                         m = nX-1, n = nY-1
+ *      Ux, Vx: m = nX+1, n = nY;   Uy, Vy: m = nX, n = nY+1
+ *    which is also the byte layout of the reference's *.dat files
+ *    (SaveMatrix, EffPlast2D.cu:795-799; read_data_2D.m:1-12).
+ *  - A handle owns its device buffers and a HIP stream of default priority.
+ *    One handle must not be used from two threads at once; distinct handles
+ *    are independent.  Handles are not copyable (the reference class is
+ *    copyable and would double-free).
+ */
+#ifndef EP2D_H
+#define EP2D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EP2D_NPARS 11   /* NPARS, EffPlast2D.h:21-23 */
+
+/* pa[] indices: [dX, dY, dt, K0, G0, rho0, dampX, dampY, coh, rad, nPores]
+ * (get_sigma_2D.m:32; read at EffPlast2D.cu:1563-1573). */
+enum ep2d_pa_index {
+  EP2D_PA_DX = 0, EP2D_PA_DY, EP2D_PA_DT, EP2D_PA_K0, EP2D_PA_G0, EP2D_PA_RHO,
+  EP2D_PA_DAMPX, EP2D_PA_DAMPY, EP2D_PA_COH, EP2D_PA_RAD, EP2D_PA_NPORES
+};
+
+typedef enum ep2d_status {
+  EP2D_OK = 0,
+  EP2D_ERR_ARG = -1,        /* invalid argument (std::invalid_argument in the reference) */
+  EP2D_ERR_HIP = -2,        /* HIP runtime error (gpuAssert in the reference)            */
+  EP2D_ERR_IO = -3,         /* file error (std::runtime_error in ReadParams)             */
+  EP2D_ERR_GEOMETRY = -4,   /* pore geometry puts a sample outside the grid               */
+  EP2D_ERR_NONFINITE = -5,  /* NaN/Inf in the velocity field at a residual check          */
+  EP2D_ERR_MEMORY = -6,     /* device memory budget exceeded                              */
+  EP2D_ERR_STATE = -7       /* call not valid in the handle's current state               */
+} ep2d_status;
+
+/* Iteration schemes.  All three compute the reference's algorithm. */
+typedef enum ep2d_mode {
+  EP2D_MODE_PLAIN = 0,  /* four kernels per iteration, FP64, stress fields in HBM:
+                           the reference's kernel structure (EffPlast2D.cu:390-399) */
+  EP2D_MODE_FP64 = 1,   /* one fused FP64 kernel per iteration (U ping-pong)          */
+  EP2D_MODE_SPLIT = 2   /* fused kernel: FP32 stress/velocity, FP64 displacement;
+                           residual, averages and moduli in FP64                     */
+} ep2d_mode;
+
+/* Field identifiers for get/set/save. */
+typedef enum ep2d_field {
+  EP2D_FIELD_P = 0, EP2D_FIELD_TAUXX, EP2D_FIELD_TAUYY, EP2D_FIELD_TAUXY,
+  EP2D_FIELD_TAUXYAV, EP2D_FIELD_J2, EP2D_FIELD_J2XY,
+  EP2D_FIELD_UX, EP2D_FIELD_UY, EP2D_FIELD_VX, EP2D_FIELD_VY,
+  EP2D_FIELD_K, EP2D_FIELD_G,
+  EP2D_FIELD_COUNT
+} ep2d_field;
+
+/* Physical inputs from which pa[] is derived (get_sigma_2D.m:1-32 and the
+ * driver scalars of eff_props_2D.m:7-26). */
+typedef struct ep2d_physics {
+  double Lx, Ly;        /* RVE size                                  */
+  int nGrid;            /* nX = nY = 32*nGrid unless nX is given     */
+  int nX;               /* 0: 32*nGrid                               */
+  int nPores;           /* nPores x nPores pores                     */
+  double porosity;      /* -> rad = sqrt(porosity*Lx*Ly/(pi*nPores^2)) */
+  double Y;             /* yield stress; coh = Y*sqrt(2)             */
+  double K0, G0, rho0;  /* moduli and density                        */
+  double CFL;           /* dt = CFL*min(dX,dY)/sqrt((K0+4G0/3)/rho0) */
+} ep2d_physics;
+
+/* Solver options: the reference's compile-time macros made runtime, plus
+ * the scheme selection. */
+typedef struct ep2d_options {
+  int nTasks;           /* NL: 1 elastic, 2 bulk moduli, 3 + shear, >3 damping */
+  long nIter;           /* NITER: iteration cap per time step        */
+  double eIter;         /* EITER: residual target                    */
+  long checkInterval;   /* output_step (reference: 10000, :1608); 0 -> 10000 */
+  int mode;             /* ep2d_mode                                  */
+  int device;           /* DEVICE_IDX                                 */
+  int nTimeSteps;       /* time steps of the preload (EffPlast2D ctor argument) */
+  /* Extension: when useShearLoadType != 0 the third task applies
+   * shearLoadType instead of loadType minus its spherical part
+   * (EffPlast2D.cu:188-189, 203), e.g. {0,0,1} for a pure xy shear. */
+  int useShearLoadType;
+  double shearLoadType[3];
+  int verbose;          /* 1: print the reference's console lines to stdout */
+  const char* logPath;  /* non-NULL: append the reference's log lines here
+                           (EffPlast2D.log, :1607) */
+} ep2d_options;
+
+typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
+
+/* Per load step and time step, the quantities ComputeEffParams stores
+ * (EffPlast2D.h:107-117, EffPlast2D.cu:344-620). */
+typedef struct ep2d_record {
+  double epsilon[3], epsilonPer[3];
+  double sigma[4], sigmaPer[4];
+  double deltaP, deltaPper, tauInfty;
+  double dPhi, dPhiPer;
+  double holeAreaPi, internalHoleAreaPi;
+  double Phi0, Phi, PhiPer;
+  double error;          /* residual at the last check of this time step */
+  double iters;          /* iterations executed                          */
+  double sigmaXYclean;   /* extension: mean tauXY over all vertices      */
+} ep2d_record;
+
+typedef struct ep2d_result {
+  ep2d_moduli an, an_per, num, num_per;      /* EffPlast2D.h:135-138 */
+  double KphiExactElast, KdExactElast, GexactElast;         /* logged only */
+  double KphiExactPlast, KdExactPlast, GexactPlast;
+  double KphiExactPlastPer, KdExactPlastPer, GexactPlastPer;
+  double Gxy, GxyPer;   /* extension: G = d(sigma_xy) / (2 d(eps_xy)) between tasks 2 and 3 */
+  int nRecords;         /* records available from ep2d_get_records   */
+  long totalIters;      /* iterations over all load and time steps   */
+  double seconds;       /* wall time of the call                     */
+  int converged;        /* 1 when every time step reached eIter      */
+} ep2d_result;
+
+typedef struct ep2d_handle ep2d_handle;
+
+/* ---- library ------------------------------------------------------------ */
+const char* ep2d_last_error(void);
+const char* ep2d_version(void);
+/* Number of visible HIP devices (0 when none; never an error). */
+int ep2d_device_count(void);
+
+/* ---- parameters (host only; no device needed) ----------------------------
+ * get_sigma_2D.m:3-32: derive pa[] and nX from the physical inputs. */
+int ep2d_derive_params(const ep2d_physics* phys, double pa[EP2D_NPARS], int* nX);
+/* ReadParams (EffPlast2D.cu:700-707) / the fwrite of get_sigma_2D.m:41-43:
+ * 11 little-endian float64 values, no header. */
+int ep2d_read_pa(const char* path, double pa[EP2D_NPARS]);
+int ep2d_write_pa(const char* path, const double pa[EP2D_NPARS]);
+/* Default options: the reference macro defaults NL=3, NITER=100000,
+ * EITER=1e-10, output_step=10000, DEVICE_IDX=0, nTimeSteps=1. */
+void ep2d_default_options(ep2d_options* opt);
+
+/* ---- solver handle -------------------------------------------------------
+ * EffPlast2D::EffPlast2D (EffPlast2D.cu:1541-1612): allocate and set up the
+ * fields for an nX x nY grid described by pa[]. */
+int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY,
+                const ep2d_options* opt, ep2d_handle** out);
+/* Convenience: derive pa[] from phys, then ep2d_create. */
+int ep2d_create_from_physics(const ep2d_physics* phys, const ep2d_options* opt,
+                             ep2d_handle** out);
+/* EffPlast2D::~EffPlast2D (EffPlast2D.cu:1613-1658). */
+void ep2d_destroy(ep2d_handle* h);
+
+/* EffPlast2D::ComputeEffModuli (EffPlast2D.cu:179-229): preload, then the
+ * small hydrostatic (and deviatoric) increments, then the moduli getters
+ * (:1412-1539).  loadValue is accepted and ignored, as in the reference
+ * (deprecated parameter).  Results go to *out (the reference returns 0.0
+ * and prints them). */
+int ep2d_compute_eff_moduli(ep2d_handle* h, double initLoadValue,
+                            const double loadType[3], double loadValue,
+                            ep2d_result* out);
+/* EffPlast2D::ComputeEffDamping (EffPlast2D.cu:230-310), nTasks > 3. */
+int ep2d_compute_eff_damping(ep2d_handle* h, double initLoadValue,
+                             const double loadType[3], double loadValue,
+                             ep2d_result* out);
+/* Records of the last compute call, [step][timeStep] row-major with
+ * nTimeSteps entries for step 0 and one for each later step. */
+int ep2d_get_records(ep2d_handle* h, ep2d_record* out, int capacity, int* count);
+
+/* ---- fields ---------------------------------------------------------------*/
+int ep2d_field_shape(ep2d_handle* h, int field, int* m, int* n);
+int ep2d_get_field(ep2d_handle* h, int field, double* host, size_t count);
+int ep2d_set_field(ep2d_handle* h, int field, const double* host, size_t count);
+/* SaveMatrix (EffPlast2D.cu:795-799): raw FP64 dump of one field. */
+int ep2d_save_field(ep2d_handle* h, int field, const char* path);
+/* Pore masks: bit0 SetMaterials pore (K, G reduced, :708-728), bit1
+ * ComputeStress cell zeroing (:77-87), bit2 vertex zeroing (:88-95). */
+int ep2d_get_mask(ep2d_handle* h, unsigned char* host, size_t count);
+
+/* ---- step-level entry points (the pieces ComputeEffParams is made of) ----*/
+/* n iterations of the handle's scheme, no residual checks (:389-399). */
+int ep2d_iterate(ep2d_handle* h, long n);
+/* U += affine field of the macro strain increment (:370-381), on device. */
+int ep2d_apply_affine(ep2d_handle* h, double dUxdx, double dUydy,
+                      double dUxdy, double dUydx);
+/* Residual (:407-412): (max|Vx|/(dX(nX-1)) + max|Vy|/(dY(nY-1)))*dt /
+ * max(|e0|, max(e1, e2)); nonfinite receives the count of NaN/Inf values. */
+int ep2d_residual(ep2d_handle* h, const double curEffStrain[3], double* error,
+                  long* nonfinite);
+/* Volume averages (:453-503) on the device in FP64, plus sigmaXYclean. */
+int ep2d_averages(ep2d_handle* h, ep2d_record* rec);
+/* getDeltaP_honest, getDeltaP_periodic, getTauInfty_honest (:825-882, :925-942). */
+int ep2d_boundary(ep2d_handle* h, double* deltaP, double* deltaPper, double* tauInfty);
+/* Pore-edge hole area, Phi0/Phi/PhiPer, dPhi/dPhiPer (:514-620). */
+int ep2d_hole_area(ep2d_handle* h, ep2d_record* rec);
+
+/* ---- timing ----------------------------------------------------------------
+ * Sample the duration of the dominant iteration kernel with HIP events on
+ * the handle's stream: every sampleEvery-th launch is bracketed (0: off). */
+int ep2d_set_timing(ep2d_handle* h, long sampleEvery);
+int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples,
+                    const char** kernelName);
+/* Synchronise the handle's stream. */
+int ep2d_sync(ep2d_handle* h);
+/* Device bytes held by the handle. */
+size_t ep2d_device_bytes(ep2d_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,202 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle on
+identical seeded inputs.
+
+Tolerances (written per test):
+  * plain and fp64 schemes evaluate the reference expressions in the same
+    order with IEEE division/sqrt and no contraction: fields after a fixed
+    number of iterations agree to <= 1e-12 relative (max-norm) -- in practice
+    bit for bit -- and the masks, which are integer, exactly.
+  * split scheme (FP32 stress/velocity, FP64 displacement): fields after a
+    fixed number of iterations within 1e-4 relative (max-norm); the converged
+    moduli within the north-star bound 1e-4 relative where the moduli are not
+    a small difference quotient (Kd, G), field relative L2 <= 1e-3, iteration
+    counts within 10%.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from effplast2d_amd import EffPlast2D
+from oracle import ora
+from tests.pairs import (PLASTIC_FIELDS, STATE_FIELDS, STRESS_FIELDS, l2_rel, make_pa,
+                         oracle_state, random_state, rel_err)
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("nX,nPores,porosity", [(32, 1, 0.01), (64, 2, 0.04), (100, 3, 0.05),
+                                                (128, 4, 0.05), (128, 1, 0.005)])
+def test_mask_bit_exact(nX, nPores, porosity):
+    _note(f"mask nX={nX} nPores={nPores}")
+    pa = make_pa(nX, nPores=nPores, porosity=porosity)
+    s = oracle_state(nX, pa, nl=3)
+    with EffPlast2D(pa, nX, mode="fp64") as g:
+        m = g.mask()
+        K = g.get("K")
+        G = g.get("G")
+    assert np.array_equal((m & 1).astype(bool), s["K"] < pa[ora.PA_K0])
+    assert np.array_equal(K, s["K"])
+    assert np.array_equal(G, s["G"])
+    assert np.array_equal(((m >> 1) & 1).astype(np.uint8), s["cellPore"])
+    vert = ((m >> 2) & 1).reshape(nX, nX)[: nX - 1, : nX - 1].ravel()
+    assert np.array_equal(vert.astype(np.uint8), s["vertPore"])
+
+
+@pytest.mark.parametrize("mode", ["plain", "fp64"])
+@pytest.mark.parametrize("nX,nPores,nl,iters,seed", [
+    (32, 1, 3, 1, 0), (64, 2, 3, 7, 1), (72, 1, 1, 5, 2), (130, 3, 3, 50, 3),
+    (128, 4, 3, 20, 4), (96, 1, 2, 3, 5)])
+def test_iterations_match_oracle(mode, nX, nPores, nl, iters, seed):
+    _note(f"iterate mode={mode} nX={nX} nPores={nPores} nl={nl} iters={iters} seed={seed}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.02)
+    init = random_state(nX, seed, pa=pa, boundary_v=(seed % 2 == 1))
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode=mode) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        fields = STATE_FIELDS + STRESS_FIELDS + (PLASTIC_FIELDS if nl > 1 else [])
+        worst = 0.0
+        for k in fields:
+            got = g.get(k)
+            e = rel_err(got, s[k])
+            worst = max(worst, e)
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
+        _note(f"  worst rel err {worst:.3e}")
+
+
+@pytest.mark.parametrize("nX,nPores,nl,iters,seed", [(64, 1, 3, 10, 11), (130, 3, 3, 40, 12),
+                                                     (96, 2, 1, 25, 13)])
+def test_split_iterations_close_to_oracle(nX, nPores, nl, iters, seed):
+    _note(f"split iterate nX={nX} nPores={nPores} nl={nl} iters={iters}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.02)
+    init = random_state(nX, seed, pa=pa)
+    # the split scheme stores V in FP32: give the oracle the same rounded start
+    init["Vx"] = init["Vx"].astype(np.float32).astype(np.float64)
+    init["Vy"] = init["Vy"].astype(np.float32).astype(np.float64)
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode="split") as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in ["Ux", "Uy"]:
+            assert rel_err(g.get(k), s[k]) <= 1e-6, k
+        for k in ["Vx", "Vy"] + STRESS_FIELDS:
+            e = l2_rel(g.get(k), s[k])
+            assert e <= 1e-4, f"{k}: L2 rel err {e:.3e}"
+
+
+@pytest.mark.parametrize("mode", ["plain", "fp64"])
+def test_affine_residual_averages_boundary(mode):
+    nX, nl = 96, 3
+    _note(f"pieces mode={mode}")
+    pa = make_pa(nX, nPores=3, porosity=0.05)
+    init = random_state(nX, 21, pa=pa)
+    s = oracle_state(nX, pa, nl, init)
+    with EffPlast2D(pa, nX, nTasks=nl, mode=mode) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        # affine increment (EffPlast2D.cu:370-381): bit for bit
+        g.apply_affine(-1e-3, -2e-3, 5e-4, 5e-4)
+        s.apply_affine(-1e-3, -2e-3, 5e-4, 5e-4)
+        assert np.array_equal(g.get("Ux"), s["Ux"])
+        assert np.array_equal(g.get("Uy"), s["Uy"])
+        g.iterate(9)
+        s.iterate(9)
+        cur = [-3e-3, -2e-3, 1e-3]
+        err, bad = g.residual(cur)
+        assert bad == 0
+        assert err == s.residual(cur)     # max is exact, the formula is host FP64
+        a_g, a_o = g.averages(), s.averages()
+        for k in ["sigma", "sigmaPer", "epsilonPer"]:
+            np.testing.assert_allclose(a_g[k], a_o[k], rtol=1e-11, atol=1e-16, err_msg=k)
+        np.testing.assert_allclose(a_g["sigmaXYclean"], a_o["sigmaXYclean"], rtol=1e-11)
+        # boundary means and hole area: host sums in the reference order over
+        # gathered values, so they are exact whenever the fields are
+        b_g, b_o = g.boundary(), s.boundary()
+        for k in b_o:
+            assert b_g[k] == pytest.approx(b_o[k], rel=1e-12, abs=1e-300), k
+            _note(f"  {k} exact={b_g[k] == b_o[k]}")
+        h_g, h_o = g.hole_area(), s.hole_area()
+        for k in ["holeAreaPi", "internalHoleAreaPi", "Phi0", "Phi", "PhiPer", "dPhi", "dPhiPer"]:
+            assert h_g[k] == pytest.approx(h_o[k], rel=1e-12, abs=1e-300), k
+
+
+def test_nonfinite_velocity_reported():
+    nX = 64
+    pa = make_pa(nX)
+    init = random_state(nX, 3, pa=pa)
+    init["Vx"][5 * (nX + 1) + 7] = np.nan
+    with EffPlast2D(pa, nX, nTasks=3, mode="fp64") as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        err, bad = g.residual([-1e-3, -1e-3, 0.0])
+        assert bad == 1
+
+
+MODULI_CASES = [
+    # nX, nPores, porosity, Y, nl, load, loadType, checkInterval
+    (64, 1, 0.01, 2.5e-3, 3, -3e-3, [1.0, 1.0, 0.0], 100),
+    (64, 1, 0.005, 2.5e-3, 1, -1e-5, [1.0, 1.0, 0.0], 100),
+    (96, 3, 0.05, 2.5e-3, 3, -3e-3, [1.0, 0.6, 0.2], 200),
+    (64, 2, 0.03, 3e-3, 2, -2e-3, [0.8, 1.0, 0.0], 100),
+]
+
+
+def _moduli_pair(nX, nPores, porosity, Y, nl, load, lt, K, mode, shear=None):
+    pa = make_pa(nX, nPores=nPores, porosity=porosity, Y=Y)
+    s = oracle_state(nX, pa, nl, None, nIter=200_000, eIter=1e-8, outputStep=K,
+                     shearLoadType=shear)
+    res_o, recs_o = s.compute_eff_moduli(load, lt, 1)
+    with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=K, mode=mode,
+                    shearLoadType=shear) as g:
+        res_g = g.ComputeEffModuli(load, lt)
+        recs_g = g.records()
+        fields = {k: g.get(k) for k in ["P", "tauXX", "Ux", "Uy"]}
+    return res_o, recs_o, res_g, recs_g, fields, s
+
+
+@pytest.mark.parametrize("mode", ["plain", "fp64"])
+@pytest.mark.parametrize("case", MODULI_CASES)
+def test_moduli_match_oracle(mode, case):
+    _note(f"moduli mode={mode} case={case}")
+    res_o, recs_o, res_g, recs_g, fields, s = _moduli_pair(*case, mode)
+    flat_o = [r for st in recs_o for r in st]
+    assert len(recs_g) == len(flat_o)
+    for rg, ro in zip(recs_g, flat_o):
+        assert rg["iters"] == ro["iters"]
+        for k in ["deltaP", "deltaPper", "tauInfty", "dPhi", "holeAreaPi"]:
+            assert rg[k] == pytest.approx(ro[k], rel=1e-10, abs=1e-300), k
+        np.testing.assert_allclose(rg["sigma"], ro["sigma"], rtol=1e-10, atol=1e-18)
+    for grp in ["num", "an"]:
+        for k in ["Kphi", "Kd", "G"]:
+            a, b = res_g[grp][k], res_o[grp][k]
+            if np.isfinite(b) and b != 0:
+                assert a == pytest.approx(b, rel=1e-8), (grp, k)
+    for k in ["P", "tauXX", "Ux", "Uy"]:
+        assert rel_err(fields[k], s[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("case", MODULI_CASES[:3])
+def test_split_moduli_close_to_fp64_oracle(case):
+    _note(f"split moduli case={case}")
+    res_o, recs_o, res_g, recs_g, fields, s = _moduli_pair(*case, "split")
+    flat_o = [r for st in recs_o for r in st]
+    for rg, ro in zip(recs_g, flat_o):
+        assert abs(rg["iters"] - ro["iters"]) <= 0.1 * ro["iters"] + case[-1]
+    for k in ["Kd", "G"]:
+        b = res_o["num"][k]
+        if np.isfinite(b) and b != 0:
+            assert res_g["num"][k] == pytest.approx(b, rel=1e-4), k
+    b = res_o["num"]["Kphi"]
+    assert res_g["num"]["Kphi"] == pytest.approx(b, rel=1e-3)
+    for k in ["P", "tauXX", "Ux", "Uy"]:
+        assert l2_rel(fields[k], s[k]) <= 1e-3, k
