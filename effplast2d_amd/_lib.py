@@ -124,6 +124,7 @@ SIGNATURES = {
     "ep2d_hole_area": ([C.c_void_p, C.POINTER(Record)], C.c_int),
     "ep2d_set_timing": ([C.c_void_p, C.c_long], C.c_int),
     "ep2d_get_timing": ([C.c_void_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_char_p)], C.c_int),
+    "ep2d_diag_copy": ([C.c_int, C.c_size_t, C.c_int, _dp], C.c_int),
     "ep2d_sync": ([C.c_void_p], C.c_int),
     "ep2d_device_bytes": ([C.c_void_p], C.c_size_t),
 }

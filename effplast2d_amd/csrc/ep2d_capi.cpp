@@ -263,6 +263,44 @@ int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples, const ch
   return guarded([&] { S(h)->getTiming(avgKernelMs, samples, kernelName); });
 }
 
+int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs) {
+  return guarded([&] {
+    if (!avgMs || reps < 1 || bytes < 16) throw ep2d::ArgError("bad argument");
+    EP2D_HIP(hipSetDevice(device));
+    const size_t n = bytes / 16 * 2;
+    double *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (s) (void)hipStreamDestroy(s);
+    };
+    try {
+      EP2D_HIP(hipMalloc(&a, n * 8));
+      EP2D_HIP(hipMalloc(&b, n * 8));
+      EP2D_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      EP2D_HIP(hipEventCreate(&e0));
+      EP2D_HIP(hipEventCreate(&e1));
+      ep2d::launch_fill(a, n, 1.0, s);
+      ep2d::launch_copy(a, b, n, s);   // warm-up
+      EP2D_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; r++) ep2d::launch_copy(a, b, n, s);
+      EP2D_HIP(hipEventRecord(e1, s));
+      EP2D_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      EP2D_HIP(hipEventElapsedTime(&ms, e0, e1));
+      *avgMs = ms / reps;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
 int ep2d_sync(ep2d_handle* h) {
   return guarded([&] { S(h)->sync(); });
 }

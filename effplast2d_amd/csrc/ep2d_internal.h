@@ -100,6 +100,7 @@ void launch_averages(const RveDev& r, const StressDev& f, const double* Ux,
 void launch_gather(const double* const* bases, const int2* table, int n, int ucur, double* out,
                    hipStream_t s);
 void launch_f32_to_f64(const float* src, double* dst, size_t n, hipStream_t s);
+void launch_copy(const double* src, double* dst, size_t n, hipStream_t s);
 void launch_f64_to_f32(const double* src, float* dst, size_t n, hipStream_t s);
 
 }  // namespace ep2d

@@ -573,6 +573,14 @@ __global__ void k_gather(const double* const* __restrict__ bases, const int2* __
   }
 }
 
+// Plain streaming copy (bandwidth calibration).
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
+                                              double2* __restrict__ dst, size_t n) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
 inline dim3 grid2(int nx, int ny, dim3 blk) {
   return dim3((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, 1);
 }
@@ -711,6 +719,13 @@ void launch_gather(const double* const* bases, const int2* table, int n, int ucu
                    hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, bases, table, n, ucur, out);
+  EP2D_HIP(hipGetLastError());
+}
+
+void launch_copy(const double* src, double* dst, size_t n, hipStream_t s) {
+  const size_t n2 = n / 2;
+  const int blocks = (int)std::min<size_t>((n2 + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const double2*)src, (double2*)dst, n2);
   EP2D_HIP(hipGetLastError());
 }
 

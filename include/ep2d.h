@@ -210,6 +210,11 @@ int ep2d_hole_area(ep2d_handle* h, ep2d_record* rec);
 int ep2d_set_timing(ep2d_handle* h, long sampleEvery);
 int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples,
                     const char** kernelName);
+/* Measurement aid, no counterpart in the reference: runs a plain
+ * double-precision device-to-device copy kernel over `bytes` bytes `reps`
+ * times on `device` and reports the mean kernel time.  Used to calibrate HBM
+ * traffic counters and as the achievable-bandwidth reference. */
+int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs);
 /* Synchronise the handle's stream. */
 int ep2d_sync(ep2d_handle* h);
 /* Device bytes held by the handle. */
