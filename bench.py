@@ -168,6 +168,32 @@ def probe(mode, nx, iters, copy_mib):
         f"({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
 
 
+def sweep(sizes, modes, tys, iters):
+    """Average fused-kernel duration per (scheme, tile height, grid size) on the
+    preloaded RVE, every launch timed with dispatch events.  This is the data
+    the default tile-height table is chosen from."""
+    from effplast2d_amd import EffPlast2D
+    rows = []
+    for nx in sizes:
+        pa, _ = make_pa(nx)
+        for mode in modes:
+            for ty in tys:
+                with EffPlast2D(pa, nx, nTasks=3, mode=mode, tileRows=ty) as g:
+                    g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
+                    g.iterate(20)
+                    g.set_timing(1)
+                    g.iterate(iters)
+                    ms, n, name = g.timing()
+                b = BYTES_PER_POINT[mode] * (nx + 1) * nx
+                row = {"nx": nx, "mode": mode, "ty": ty, "kernel_us": ms * 1e3,
+                       "GBs": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "kernel": name}
+                rows.append(row)
+                log(f"[sweep] nx={nx} {mode} TY={ty}: {ms * 1e3:.2f} us, "
+                    f"{row['GBs']:.0f} GB/s ({100 * row['frac']:.1f}%) {name}")
+    print(json.dumps({"sweep": rows}), flush=True)
+
+
 def summarize(outdir, dest):
     """Condense rocprofv3 csv output in outdir into text lines."""
     lines = []
@@ -255,6 +281,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
+    ap.add_argument("--sweep", action="store_true")
+    ap.add_argument("--sweep-sizes", default="256,1024,4096")
+    ap.add_argument("--sweep-iters", type=int, default=300)
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None)
     ap.add_argument("--pmc-kernel", default="k_fused<double")
     ap.add_argument("--pmc-command", default="")
@@ -274,6 +303,10 @@ def main():
         return 2
     if a.probe:
         probe(a.mode, a.nx, a.probe_iters, a.copy_mib)
+        return 0
+    if a.sweep:
+        sweep([int(x) for x in a.sweep_sizes.split(",")], ["fp64", "split"], [6, 8, 14, 30],
+              a.sweep_iters)
         return 0
 
     from effplast2d_amd import device_count

@@ -42,7 +42,7 @@ class Options(C.Structure):
                 ("checkInterval", C.c_long), ("mode", C.c_int), ("device", C.c_int),
                 ("nTimeSteps", C.c_int), ("useShearLoadType", C.c_int),
                 ("shearLoadType", C.c_double * 3), ("verbose", C.c_int),
-                ("logPath", C.c_char_p)]
+                ("logPath", C.c_char_p), ("tileRows", C.c_int)]
 
 
 class Moduli(C.Structure):

@@ -16,6 +16,10 @@ enum : unsigned char {
   M_CELL_ZERO = 2,  // ComputeStress cell test: P = tauXX = tauYY = 0 (EffPlast2D.cu:77-87)
   M_VERT_ZERO = 4   // ComputeStress vertex test at vertex (i, j): tauXY = 0 (EffPlast2D.cu:88-95)
 };
+// Bits 3..6: SetMaterials pore bits of the four cells around vertex (i, j) --
+// (i,j), (i+1,j), (i,j+1), (i+1,j+1) -- which select the vertex shear modulus
+// 0.25*(G + G + G + G) from a 16-entry table (EffPlast2D.cu:72).
+constexpr int M_VCLASS_SHIFT = 3;
 
 struct HipError : std::runtime_error {
   explicit HipError(const std::string& s) : std::runtime_error(s) {}
@@ -44,6 +48,9 @@ struct RveDev {
   double coh;               // yield radius pa[8]
   double vdampX, vdampY;    // 1 - dT*dampX, 1 - dT*dampY
   double dTrho;             // dT / rho
+  double coh2lo;            // below coh^2: a smaller squared J2 cannot exceed coh
+  float coh2lof;            // the same bound for the FP32 arithmetic of the split scheme
+  double gtab[16];          // vertex shear modulus by vertex class (mask bits 3..6)
 };
 
 // Stress fields in HBM (plain scheme; also the finalised fields of the fused
@@ -58,10 +65,18 @@ struct StressDev {
   double* J2XY;
 };
 
-// Tile of the fused kernels.
-constexpr int FUSED_TX = 64;
-constexpr int FUSED_TY = 8;
-constexpr int FUSED_NT = 256;
+// Launch plan of a fused kernel for one grid: tiles are 62 points wide and
+// ty (6, 8, 14 or 30) tall, one workgroup per tile.
+struct FusedPlan {
+  int ty = 8;
+  int ntx = 0;
+  int ntiles = 0;
+  int grid = 0;
+  int blocksPerCU = 0;
+  int smem = 0;
+  int waves = 0;
+};
+bool valid_tile_rows(int ty);
 
 // Partial sums of the volume averages: 11 accumulators.
 constexpr int NAVG = 11;
@@ -84,10 +99,12 @@ void launch_plain_stress(const RveDev& r, const double* Ux, const double* Uy,
 void launch_plain_j2(const RveDev& r, const StressDev& f, hipStream_t s);
 void launch_plain_plasticity(const RveDev& r, const StressDev& f, hipStream_t s);
 void launch_plain_disp(const RveDev& r, double* Ux, double* Uy, const StressDev& f,
-                       hipStream_t s);
-// fused schemes: reads U[cur], writes U[cur ^ 1], updates V in place
-void launch_fused_fp64(const RveDev& r, int cur, bool plastic, hipStream_t s);
-void launch_fused_split(const RveDev& r, int cur, bool plastic, hipStream_t s);
+                       hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// fused schemes (mode 1: FP64, mode 2: split): reads U[cur], writes U[cur ^ 1],
+// updates V in place.  e0/e1 (optional) receive the dispatch's start/end.
+FusedPlan plan_fused(int mode, bool plastic, int ty, int nX, int nY, int device);
+void launch_fused(const RveDev& r, int cur, int mode, bool plastic, const FusedPlan& pl,
+                  hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 const char* fused_kernel_name(int mode);
 // residual: out[0] = bits of max|Vx|, out[1] = bits of max|Vy|, out[2] = nonfinite count
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s);

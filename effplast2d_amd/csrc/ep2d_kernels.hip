@@ -10,15 +10,16 @@
 // vertices (nX-1)*(nY-1), Ux/Vx (nX+1)*nY, Uy/Vy nX*(nY+1).
 #include "ep2d_internal.h"
 
+#include <hip/hip_ext.h>
+
+#include <algorithm>
 #include <cmath>
+#include <utility>
 
 namespace ep2d {
 
 namespace {
 
-constexpr int TX = FUSED_TX;
-constexpr int TY = FUSED_TY;
-constexpr int NT = FUSED_NT;
 
 __device__ __forceinline__ double kmat(unsigned char m, double v0, double vp) {
   return (m & M_CELL_MAT) ? vp : v0;
@@ -28,22 +29,32 @@ __device__ __forceinline__ double kmat(unsigned char m, double v0, double vp) {
 // Pore masks.  SetMaterials (EffPlast2D.cu:708-728) and the two pore tests of
 // ComputeStress (EffPlast2D.cu:77-97) evaluated once, with the reference's
 // expressions, instead of in every iteration.
+__device__ bool mat_pore(int i, int j, int nX, int nY, double dX, double dY, double rad,
+                         double N) {
+  const double x = -0.5 * dX * (nX - 1) + dX * i;
+  const double y = -0.5 * dY * (nY - 1) + dY * j;
+  const double Lx = dX * (nX - 1);
+  const double Ly = dY * (nY - 1);
+  bool in = false;
+  for (int k = 0; k < N; k++) {
+    for (int l = 0; l < N; l++) {
+      const double ex = x - 0.5 * Lx * (1.0 - 1.0 / N) + (Lx / N) * k;
+      const double ey = y - 0.5 * Ly * (1.0 - 1.0 / N) + (Ly / N) * l;
+      if (sqrt(ex * ex + ey * ey) < rad) in = true;
+    }
+  }
+  return in;
+}
+
 __global__ void k_build_mask(unsigned char* __restrict__ mask, int nX, int nY, double dX,
                              double dY, double rad, double nP) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= nX || j >= nY) return;
   const double N = nP;
-  const double x = -0.5 * dX * (nX - 1) + dX * i;
-  const double y = -0.5 * dY * (nY - 1) + dY * j;
-  const double Lx = dX * (nX - 1);
-  const double Ly = dY * (nY - 1);
-  unsigned char m = 0;
+  unsigned char m = mat_pore(i, j, nX, nY, dX, dY, rad, N) ? M_CELL_MAT : 0;
   for (int k = 0; k < N; k++) {
     for (int l = 0; l < N; l++) {
-      const double ex = x - 0.5 * Lx * (1.0 - 1.0 / N) + (Lx / N) * k;
-      const double ey = y - 0.5 * Ly * (1.0 - 1.0 / N) + (Ly / N) * l;
-      if (sqrt(ex * ex + ey * ey) < rad) m |= M_CELL_MAT;
       const double cx = -0.5 * dX * (nX - 1) + dX * i - 0.5 * dX * (nX - 1) * (1.0 - 1.0 / N) + (dX * (nX - 1) / N) * k;
       const double cy = -0.5 * dY * (nY - 1) + dY * j - 0.5 * dY * (nY - 1) * (1.0 - 1.0 / N) + (dY * (nY - 1) / N) * l;
       if (sqrt(cx * cx + cy * cy) < rad) m |= M_CELL_ZERO;
@@ -53,6 +64,12 @@ __global__ void k_build_mask(unsigned char* __restrict__ mask, int nX, int nY, d
         if (sqrt(vx * vx + vy * vy) < rad) m |= M_VERT_ZERO;
       }
     }
+  }
+  if (i < nX - 1 && j < nY - 1) {
+    const int c = (m & M_CELL_MAT ? 1 : 0) | (mat_pore(i + 1, j, nX, nY, dX, dY, rad, N) ? 2 : 0) |
+                  (mat_pore(i, j + 1, nX, nY, dX, dY, rad, N) ? 4 : 0) |
+                  (mat_pore(i + 1, j + 1, nX, nY, dX, dY, rad, N) ? 8 : 0);
+    m |= (unsigned char)(c << M_VCLASS_SHIFT);
   }
   mask[(size_t)j * nX + i] = m;
 }
@@ -210,229 +227,298 @@ __global__ void k_plain_disp(RveDev r, double* __restrict__ Ux, double* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Fused iteration: one workgroup computes one TX x TY tile of points
-// (i0 <= i < i0+TX, j0 <= j < j0+TY) for Vx, Vy, Ux, Uy.
+// Quotient by a uniform divisor d, given rd = RN(1/d): one product and one
+// FMA correction (Markstein's method, the same correction step the AMDGPU and
+// IA-64 division sequences end with).  For the operand ranges of this solver
+// (no overflow, underflow or subnormal quotients) it returns the IEEE quotient
+// a / d, so the fused FP64 kernel keeps the reference's arithmetic.
+__device__ __forceinline__ double qdiv(double a, double d, double rd) {
+  const double q = a * rd;
+  return fma(fma(-q, d, a), rd, q);
+}
+
+// Workgroup barrier that waits for LDS traffic only: global loads issued
+// before it (the next tile's prefetch) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// ---------------------------------------------------------------------------
+// Fused iteration.  A workgroup of NW wave64s computes one tile of TX = 62 x TY
+// points (i0 <= i < i0+62, j0 <= j < j0+TY) for Vx, Vy, Ux, Uy.  Lane a of a
+// wave owns column a of every per-tile array, waves take rows in turn:
 //
-//   phase 0  U^n box [i0-2, i0+TX+2) x [j0-2, j0+TY+2) and mask -> LDS
-//   phase 1  trial stress: cells [i0-1, i0+TX+1)^2, vertices [i0-2, i0+TX)^2
-//   phase 2  return mapping: cells and vertices [i0-1, i0+TX)^2
-//   phase 3  damped velocity and displacement for the tile -> HBM
+//   stage     U^n box [i0-2, i0+64) x [j0-2, j0+TY+2) (66 wide) and the pore
+//             mask -> LDS; this thread's V^n and U^n -> registers
+//   phase 1   trial stress (ComputeStress, EffPlast2D.cu:37-98): cells
+//             i0-1+a and vertices i0-2+a, a = lane (64 wide)
+//   phase 2a  vertex return mapping (ComputeJ2 / ComputePlasticity,
+//             EffPlast2D.cu:138-144, 172-176), 63 wide, over the dead U box
+//   phase 2b  cell return mapping (:109-137, :165-170), 63 wide, in place
+//   phase 3   damped velocity and displacement (ComputeDisp, :3-35) -> HBM
 //
 // U is read from U[cur] and written to U[cur^1] (a neighbour tile may still
-// read U^n); V is updated in place (each point reads only its own V).  The
-// tile whose x range holds nX-1 also copies column nX of Ux, and the tile
-// whose y range holds nY-1 copies row nY of Uy: those nodes are never
-// updated (ComputeDisp runs nX x nY threads), so U^{n+1} = U^n there.
+// read U^n); V is updated in place (a point reads only its own V).  The tile
+// holding i = nX-1 also copies column nX of Ux and the tile holding j = nY-1
+// copies row nY of Uy: ComputeDisp never updates those nodes.
 //
-// CT is the arithmetic type of the stress and velocity update, VT the
-// storage type of V.  With CT = double the expressions are those of the
-// plain kernels, in the same order.
-template <typename CT, typename VT, bool PLASTIC>
-struct FusedTile {
-  static constexpr int BX = TX + 4, BY = TY + 4;  // U box
-  static constexpr int CX = TX + 2, CY = TY + 2;  // trial cells / vertices
-  static constexpr int QX = TX + 1, QY = TY + 1;  // corrected cells / vertices
-  static constexpr size_t SMEM =
-      sizeof(double) * 2 * BX * BY + sizeof(CT) * (4 * CX * CY + 3 * QX * QY) + BX * BY;
+// Tiles whose whole dependency box lies inside the grid run without bounds
+// checks, edge clamps or boundary conditions (EDGE = false).  A return
+// mapping is skipped when the squared invariant is below coh2lo: then
+// sqrt(s) <= coh exactly, as the reference's test would find.
+//
+// CT is the arithmetic type of stress and velocity, VT the storage type of V.
+// With CT = double every expression is the plain kernels' expression, in the
+// same order, so the fused FP64 scheme reproduces the plain scheme bit for bit.
+template <typename CT, typename VT, bool PLASTIC, int TY, int NW>
+struct Fused {
+  static constexpr int S = 64;                 // row stride of the per-tile arrays
+  static constexpr int TX = S - 2;             // output columns
+  static constexpr int NT = 64 * NW;
+  static constexpr int BX = TX + 4, BY = TY + 4, NBOX = BX * BY;  // U box
+  static constexpr int CY = TY + 2;            // trial rows
+  static constexpr int QY = TY + 1;            // corrected rows
+  static constexpr int KP = (TY + NW - 1) / NW;  // output rows per wave
+  static constexpr size_t OFF_BOX = 128;       // after the 16-entry G table
+  static constexpr size_t OFF_C = OFF_BOX + 16 * NBOX;
+  static constexpr size_t OFF_M = OFF_C + 4 * sizeof(CT) * CY * S;
+  static constexpr size_t SMEM = (OFF_M + NBOX + 15) / 16 * 16;
+  static_assert(sizeof(CT) * QY * S <= 16 * NBOX, "corrected vertex stress must fit the U box");
 
-  static __device__ __forceinline__ CT diffx(double a, double b, const RveDev& r) {
-    if constexpr (sizeof(CT) == 8) return (a - b) / r.dX;
-    else return (CT)(a - b) * (CT)r.rdX;
+  static __device__ __forceinline__ CT dx(double a, const RveDev& r) {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, r.dX, r.rdX);
+    else return (CT)a * (CT)r.rdX;
   }
-  static __device__ __forceinline__ CT diffy(double a, double b, const RveDev& r) {
-    if constexpr (sizeof(CT) == 8) return (a - b) / r.dY;
-    else return (CT)(a - b) * (CT)r.rdY;
-  }
-  static __device__ __forceinline__ CT divx(CT a, const RveDev& r) {
-    if constexpr (sizeof(CT) == 8) return a / r.dX;
-    else return a * (CT)r.rdX;
-  }
-  static __device__ __forceinline__ CT divy(CT a, const RveDev& r) {
-    if constexpr (sizeof(CT) == 8) return a / r.dY;
-    else return a * (CT)r.rdY;
+  static __device__ __forceinline__ CT dy(double a, const RveDev& r) {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, r.dY, r.rdY);
+    else return (CT)a * (CT)r.rdY;
   }
   static __device__ __forceinline__ CT third(CT a) {
-    if constexpr (sizeof(CT) == 8) return a / 3.0;
+    if constexpr (sizeof(CT) == 8) return qdiv(a, 3.0, 1.0 / 3.0);
     else return a * (CT)(1.0 / 3.0);
   }
 
-  static __device__ __forceinline__ void run(const RveDev& r, int cur, int i0, int j0,
-                                             char* smem) {
+  template <bool EDGE>
+  static __device__ __forceinline__ void run(const RveDev& r, int cur, int i0, int j0, char* smem) {
     const int nX = r.nX, nY = r.nY;
-    const int tid = threadIdx.x;
-    double* sUx = reinterpret_cast<double*>(smem);
-    double* sUy = sUx + BX * BY;
-    CT* sP = reinterpret_cast<CT*>(sUy + BX * BY);
-    CT* sTxx = sP + CX * CY;
-    CT* sTyy = sTxx + CX * CY;
-    CT* sTxy = sTyy + CX * CY;
-    CT* cTxx = sTxy + CX * CY;
-    CT* cTyy = cTxx + QX * QY;
-    CT* cTxy = cTyy + QX * QY;
-    unsigned char* sM = reinterpret_cast<unsigned char*>(cTxy + QX * QY);
-
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* gt = reinterpret_cast<double*>(smem);
+    double* sUx = reinterpret_cast<double*>(smem + OFF_BOX);
+    double* sUy = sUx + NBOX;
+    CT* sP = reinterpret_cast<CT*>(smem + OFF_C);
+    CT* sTxx = sP + CY * S;
+    CT* sTyy = sTxx + CY * S;
+    CT* sTxy = sTyy + CY * S;
+    CT* cTxy = reinterpret_cast<CT*>(smem + OFF_BOX);  // written after phase 1
+    unsigned char* sM = reinterpret_cast<unsigned char*>(smem + OFF_M);
     const double* __restrict__ Uxi = r.Ux[cur];
     const double* __restrict__ Uyi = r.Uy[cur];
+    const unsigned char* __restrict__ msk = r.mask;
+    const int sx = nX + 1;  // Ux row length
 
-    // phase 0
-    for (int e = tid; e < BX * BY; e += NT) {
-      const int a = e % BX, b = e / BX;
-      const int gi = i0 - 2 + a, gj = j0 - 2 + b;
-      double ux = 0.0, uy = 0.0;
-      unsigned char m = 0;
-      if (gi >= 0 && gj >= 0) {
-        if (gi <= nX && gj < nY) ux = Uxi[(size_t)gj * (nX + 1) + gi];
-        if (gi < nX && gj <= nY) uy = Uyi[(size_t)gj * nX + gi];
-        if (gi < nX && gj < nY) m = r.mask[(size_t)gj * nX + gi];
-      }
-      sUx[e] = ux;
-      sUy[e] = uy;
-      sM[e] = m;
+    // ---- stage ----
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
     }
-    __syncthreads();
+    for (int b = wave; b < BY; b += NW) {
+      const int gj = j0 - 2 + b;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int a = lane + 64 * h;
+        if (h == 1 && a >= BX) break;
+        const int gi = i0 - 2 + a;
+        double ux = 0.0, uy = 0.0;
+        unsigned char m = 0;
+        if (!EDGE) {
+          ux = Uxi[gj * sx + gi];
+          uy = Uyi[gj * nX + gi];
+          m = msk[gj * nX + gi];
+        } else if (gi >= 0 && gj >= 0) {
+          if (gi <= nX && gj < nY) ux = Uxi[gj * sx + gi];
+          if (gi < nX && gj <= nY) uy = Uyi[gj * nX + gi];
+          if (gi < nX && gj < nY) m = msk[gj * nX + gi];
+        }
+        sUx[b * BX + a] = ux;
+        sUy[b * BX + a] = uy;
+        sM[b * BX + a] = m;
+      }
+    }
+    // this thread's velocities (read early: the latency overlaps phases 1-2)
+    const VT* __restrict__ Vxi = static_cast<const VT*>(r.Vx);
+    const VT* __restrict__ Vyi = static_cast<const VT*>(r.Vy);
+    VT vxr[KP], vyr[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      const int b = wave + NW * k, i = i0 + lane, j = j0 + b;
+      vxr[k] = 0;
+      vyr[k] = 0;
+      if (b < TY && lane < TX && (!EDGE || (i < nX && j < nY))) {
+        vxr[k] = Vxi[j * sx + i];
+        vyr[k] = Vyi[j * nX + i];
+      }
+    }
+    lds_barrier();
 
-    // phase 1: trial stress (ComputeStress, EffPlast2D.cu:37-98)
-    for (int e = tid; e < CX * CY; e += NT) {
-      const int a = e % CX, b = e / CX;
+    double uxo[KP], uyo[KP], uxe[KP], uye[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      const int b = wave + NW * k;
+      const int bb = b < TY ? b : 0;
+      const int a = lane < TX ? lane : 0;
+      uxo[k] = sUx[(bb + 2) * BX + a + 2];
+      uyo[k] = sUy[(bb + 2) * BX + a + 2];
+      uxe[k] = EDGE ? sUx[(bb + 2) * BX + a + 3] : 0.0;
+      uye[k] = EDGE ? sUy[(bb + 3) * BX + a + 2] : 0.0;
+    }
+
+    // ---- phase 1: trial stress ----
+    for (int b = wave; b < CY; b += NW) {
+      const int a = lane;
       {  // cell (i0-1+a, j0-1+b) <-> box (a+1, b+1)
         const int ci = i0 - 1 + a, cj = j0 - 1 + b;
-        CT p = 0, txx = 0, tyy = 0;
-        if (ci >= 0 && ci < nX && cj >= 0 && cj < nY) {
+        CT pp = 0, txx = 0, tyy = 0;
+        if (!EDGE || (ci >= 0 && ci < nX && cj >= 0 && cj < nY)) {
           const unsigned char m = sM[(b + 1) * BX + a + 1];
           const CT K = (CT)kmat(m, r.K0, r.Kp);
           const CT G = (CT)kmat(m, r.G0, r.Gp);
-          const CT exx = diffx(sUx[(b + 1) * BX + a + 2], sUx[(b + 1) * BX + a + 1], r);
-          const CT eyy = diffy(sUy[(b + 2) * BX + a + 1], sUy[(b + 1) * BX + a + 1], r);
-          p = (CT)r.P0 - K * (exx + eyy);
+          const CT exx = dx(sUx[(b + 1) * BX + a + 2] - sUx[(b + 1) * BX + a + 1], r);
+          const CT eyy = dy(sUy[(b + 2) * BX + a + 1] - sUy[(b + 1) * BX + a + 1], r);
+          pp = (CT)r.P0 - K * (exx + eyy);
           txx = (CT)2.0 * G * (exx - third(exx + eyy));
           tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
           if (m & M_CELL_ZERO) {
-            p = 0;
+            pp = 0;
             txx = 0;
             tyy = 0;
           }
         }
-        sP[e] = p;
-        sTxx[e] = txx;
-        sTyy[e] = tyy;
+        sP[b * S + a] = pp;
+        sTxx[b * S + a] = txx;
+        sTyy[b * S + a] = tyy;
       }
       {  // vertex (i0-2+a, j0-2+b) <-> box (a, b)
         const int vi = i0 - 2 + a, vj = j0 - 2 + b;
         CT txy = 0;
-        if (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1) {
+        if (!EDGE || (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1)) {
           const unsigned char m00 = sM[b * BX + a];
-          const CT g = (CT)(0.25 * (kmat(m00, r.G0, r.Gp) + kmat(sM[b * BX + a + 1], r.G0, r.Gp) +
-                                    kmat(sM[(b + 1) * BX + a], r.G0, r.Gp) +
-                                    kmat(sM[(b + 1) * BX + a + 1], r.G0, r.Gp)));
-          txy = g * (diffy(sUx[(b + 1) * BX + a + 1], sUx[b * BX + a + 1], r) +
-                     diffx(sUy[(b + 1) * BX + a + 1], sUy[(b + 1) * BX + a], r));
+          const CT g = (CT)gt[(m00 >> M_VCLASS_SHIFT) & 15];
+          txy = g * (dy(sUx[(b + 1) * BX + a + 1] - sUx[b * BX + a + 1], r) +
+                     dx(sUy[(b + 1) * BX + a + 1] - sUy[(b + 1) * BX + a], r));
           if (m00 & M_VERT_ZERO) txy = 0;
         }
-        sTxy[e] = txy;
+        sTxy[b * S + a] = txy;
       }
     }
-    __syncthreads();
+    lds_barrier();
 
-    // phase 2: return mapping (ComputeJ2 + ComputePlasticity, EffPlast2D.cu:100-177)
     if constexpr (PLASTIC) {
       const CT coh = (CT)r.coh;
-      for (int e = tid; e < QX * QY; e += NT) {
-        const int a = e % QX, b = e / QX;
-        {  // cell (i0-1+a, j0-1+b) <-> trial cell (a, b)
-          const int ci = i0 - 1 + a, cj = j0 - 1 + b;
-          CT txx = sTxx[b * CX + a], tyy = sTyy[b * CX + a];
-          if (ci >= 0 && ci < nX && cj >= 0 && cj < nY) {
-            const int va = min(max(ci - 1, 0), nX - 3) - (i0 - 2);
-            const int vb = min(max(cj - 1, 0), nY - 3) - (j0 - 2);
-            const CT av = (CT)0.25 * (sTxy[vb * CX + va] + sTxy[vb * CX + va + 1] +
-                                      sTxy[(vb + 1) * CX + va] + sTxy[(vb + 1) * CX + va + 1]);
-            const CT J2 = sqrt(txx * txx + tyy * tyy + (CT)2.0 * av * av);
-            if (J2 > coh) {
-              const CT s = coh / J2;
-              txx = txx * s;
-              tyy = tyy * s;
+      CT c2lo;
+      if constexpr (sizeof(CT) == 8) c2lo = r.coh2lo;
+      else c2lo = r.coh2lof;
+      // phase 2a: vertex (i0-1+a, j0-1+b) <-> trial vertex (a+1, b+1), trial cells (a..a+1, b..b+1)
+      for (int b = wave; b < QY; b += NW) {
+        const int a = lane;
+        if (a < S - 1) {
+          const int vi = i0 - 1 + a, vj = j0 - 1 + b;
+          CT txy = sTxy[(b + 1) * S + a + 1];
+          if (!EDGE || (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1)) {
+            const CT sxx = sTxx[b * S + a] + sTxx[b * S + a + 1] + sTxx[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1];
+            const CT syy = sTyy[b * S + a] + sTyy[b * S + a + 1] + sTyy[(b + 1) * S + a] + sTyy[(b + 1) * S + a + 1];
+            const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+            if (s2 > c2lo) {
+              const CT J2XY = sqrt(s2);
+              if (J2XY > coh) txy *= coh / J2XY;
             }
           }
-          cTxx[e] = txx;
-          cTyy[e] = tyy;
-        }
-        {  // vertex (i0-1+a, j0-1+b) <-> trial vertex (a+1, b+1), trial cells (a..a+1, b..b+1)
-          const int vi = i0 - 1 + a, vj = j0 - 1 + b;
-          CT txy = sTxy[(b + 1) * CX + a + 1];
-          if (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1) {
-            const CT sxx = sTxx[b * CX + a] + sTxx[b * CX + a + 1] + sTxx[(b + 1) * CX + a] + sTxx[(b + 1) * CX + a + 1];
-            const CT syy = sTyy[b * CX + a] + sTyy[b * CX + a + 1] + sTyy[(b + 1) * CX + a] + sTyy[(b + 1) * CX + a + 1];
-            const CT J2XY = sqrt((CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy);
-            if (J2XY > coh) txy *= coh / J2XY;
-          }
-          cTxy[e] = txy;
+          cTxy[b * S + a] = txy;
         }
       }
-      __syncthreads();
+      lds_barrier();
+      // phase 2b: cell (i0-1+a, j0-1+b) <-> trial cell (a, b), corrected in place
+      for (int b = wave; b < QY; b += NW) {
+        const int a = lane;
+        if (a < S - 1) {
+          const int ci = i0 - 1 + a, cj = j0 - 1 + b;
+          if (!EDGE || (ci >= 0 && ci < nX && cj >= 0 && cj < nY)) {
+            const int va = EDGE ? min(max(ci - 1, 0), nX - 3) - (i0 - 2) : a;
+            const int vb = EDGE ? min(max(cj - 1, 0), nY - 3) - (j0 - 2) : b;
+            const CT av = (CT)0.25 * (sTxy[vb * S + va] + sTxy[vb * S + va + 1] +
+                                      sTxy[(vb + 1) * S + va] + sTxy[(vb + 1) * S + va + 1]);
+            const CT txx = sTxx[b * S + a], tyy = sTyy[b * S + a];
+            const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+            if (s2 > c2lo) {
+              const CT J2 = sqrt(s2);
+              if (J2 > coh) {
+                const CT sc = coh / J2;
+                sTxx[b * S + a] = txx * sc;
+                sTyy[b * S + a] = tyy * sc;
+              }
+            }
+          }
+        }
+      }
+      lds_barrier();
     }
 
-    // phase 3: damped velocity and displacement (ComputeDisp, EffPlast2D.cu:3-35)
+    // ---- phase 3: corrected cell (i, j) <-> (a+1, b+1); corrected vertex (i-1, j-1) <-> (a, b) ----
     VT* __restrict__ Vx = static_cast<VT*>(r.Vx);
     VT* __restrict__ Vy = static_cast<VT*>(r.Vy);
     double* __restrict__ Uxo = r.Ux[cur ^ 1];
     double* __restrict__ Uyo = r.Uy[cur ^ 1];
     const CT vdampX = (CT)r.vdampX, vdampY = (CT)r.vdampY, dTrho = (CT)r.dTrho;
-    for (int e = tid; e < TX * TY; e += NT) {
-      const int a = e % TX, b = e / TX;
+    auto Txy = [&](int x, int y) -> CT {
+      if constexpr (PLASTIC) return cTxy[y * S + x];
+      else return sTxy[(y + 1) * S + x + 1];
+    };
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      const int b = wave + NW * k, a = lane;
       const int i = i0 + a, j = j0 + b;
-      if (i >= nX || j >= nY) continue;
-      // corrected cell (i,j) <-> (a+1, b+1) in both trial (P) and corrected arrays;
-      // corrected vertex (i-1, j-1) <-> (a, b) corrected / (a+1, b+1) trial.
-      auto Pc = [&](int x, int y) -> CT { return sP[y * CX + x]; };
-      auto Txx = [&](int x, int y) -> CT {
-        if constexpr (PLASTIC) return cTxx[y * QX + x];
-        else return sTxx[y * CX + x];
-      };
-      auto Tyy = [&](int x, int y) -> CT {
-        if constexpr (PLASTIC) return cTyy[y * QX + x];
-        else return sTyy[y * CX + x];
-      };
-      auto Txy = [&](int x, int y) -> CT {
-        if constexpr (PLASTIC) return cTxy[y * QX + x];
-        else return sTxy[(y + 1) * CX + x + 1];
-      };
-      const size_t ix = (size_t)j * (nX + 1) + i;
-      const size_t iy = (size_t)j * nX + i;
-      CT vx = (CT)Vx[ix];
-      CT vy = (CT)Vy[iy];
-      if (i > 0 && j > 0 && j < nY - 1) {
-        vx = vx * vdampX + dTrho * (divx(-Pc(a + 1, b + 1) + Pc(a, b + 1) + Txx(a + 1, b + 1) - Txx(a, b + 1), r) +
-                                    divy(Txy(a, b + 1) - Txy(a, b), r));
-        Vx[ix] = (VT)vx;
+      if (b >= TY || a >= TX) continue;
+      if (EDGE && (i >= nX || j >= nY)) continue;
+      CT vx = (CT)vxr[k];
+      CT vy = (CT)vyr[k];
+      if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
+        vx = vx * vdampX + dTrho * (dx(-sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a], r) +
+                                    dy(Txy(a, b + 1) - Txy(a, b), r));
+        Vx[j * sx + i] = (VT)vx;
       }
-      if (i > 0 && i < nX - 1 && j > 0) {
-        vy = vy * vdampY + dTrho * (divy(-Pc(a + 1, b + 1) + Pc(a + 1, b) + Tyy(a + 1, b + 1) - Tyy(a + 1, b), r) +
-                                    divx(Txy(a + 1, b) - Txy(a, b), r));
-        Vy[iy] = (VT)vy;
+      if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
+        vy = vy * vdampY + dTrho * (dy(-sP[(b + 1) * S + a + 1] + sP[b * S + a + 1] + sTyy[(b + 1) * S + a + 1] - sTyy[b * S + a + 1], r) +
+                                    dx(Txy(a + 1, b) - Txy(a, b), r));
+        Vy[j * nX + i] = (VT)vy;
       }
-      Uxo[ix] = sUx[(b + 2) * BX + a + 2] + (double)vx * r.dT;
-      Uyo[iy] = sUy[(b + 2) * BX + a + 2] + (double)vy * r.dT;
-      if (i == nX - 1) Uxo[ix + 1] = sUx[(b + 2) * BX + a + 3];
-      if (j == nY - 1) Uyo[iy + nX] = sUy[(b + 3) * BX + a + 2];
+      Uxo[j * sx + i] = uxo[k] + (double)vx * r.dT;
+      Uyo[j * nX + i] = uyo[k] + (double)vy * r.dT;
+      if (EDGE) {
+        if (i == nX - 1) Uxo[j * sx + i + 1] = uxe[k];
+        if (j == nY - 1) Uyo[(j + 1) * nX + i] = uye[k];
+      }
     }
   }
 };
 
 // Block -> tile map.  Workgroups are dispatched round-robin over the 8 XCDs;
 // when the tile count divides by 8, XCD k receives the contiguous band of
-// tiles [k*nb/8, (k+1)*nb/8), so vertically adjacent tiles (which share halo
-// rows) run on the same XCD and share its L2.
+// tiles [k*nb/8, (k+1)*nb/8), so a tile and the tile above it (which share
+// halo rows) mostly run on the same XCD and share its L2.
 __device__ __forceinline__ int tile_of_block(int b, int nb) {
   if ((nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
   return b;
 }
 
-template <typename CT, typename VT, bool PLASTIC>
-__global__ __launch_bounds__(NT) void k_fused(RveDev r, int cur, int ntx, int nb) {
-  using Tile = FusedTile<CT, VT, PLASTIC>;
-  __shared__ __attribute__((aligned(16))) char smem[Tile::SMEM];
-  const int t = tile_of_block(blockIdx.x, nb);
-  Tile::run(r, cur, (t % ntx) * TX, (t / ntx) * TY, smem);
+template <typename CT, typename VT, bool PLASTIC, int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fused(RveDev r, int cur, int ntx, int ntiles) {
+  using F = Fused<CT, VT, PLASTIC, TY, NW>;
+  __shared__ __attribute__((aligned(16))) char smem[F::SMEM];
+  const int t = tile_of_block(blockIdx.x, ntiles);
+  const int i0 = (t % ntx) * F::TX, j0 = (t / ntx) * TY;
+  if (i0 >= 2 && i0 + F::TX + 2 <= r.nX && j0 >= 2 && j0 + TY + 2 <= r.nY)
+    F::template run<false>(r, cur, i0, j0, smem);
+  else
+    F::template run<true>(r, cur, i0, j0, smem);
 }
 
 // ---------------------------------------------------------------------------
@@ -585,8 +671,66 @@ inline dim3 grid2(int nx, int ny, dim3 blk) {
   return dim3((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, 1);
 }
 
-inline int tiles_x(int nX) { return (nX + TX - 1) / TX; }
-inline int tiles_y(int nY) { return (nY + TY - 1) / TY; }
+// Tile heights and their workgroup sizes (waves per block).
+template <int TY> struct TileNW;
+template <> struct TileNW<6> { static constexpr int value = 8; };
+template <> struct TileNW<8> { static constexpr int value = 4; };
+template <> struct TileNW<14> { static constexpr int value = 8; };
+template <> struct TileNW<30> { static constexpr int value = 16; };
+
+template <typename CT, typename VT, bool P, int TY>
+struct LaunchFn {
+  static constexpr int NW = TileNW<TY>::value;
+  static int call(const RveDev& r, int cur, const FusedPlan& pl, hipStream_t s, hipEvent_t e0,
+                  hipEvent_t e1) {
+    if (e0)
+      hipExtLaunchKernelGGL((k_fused<CT, VT, P, TY, NW>), dim3(pl.grid), dim3(64 * NW), 0, s, e0,
+                            e1, 0, r, cur, pl.ntx, pl.ntiles);
+    else
+      hipLaunchKernelGGL((k_fused<CT, VT, P, TY, NW>), dim3(pl.grid), dim3(64 * NW), 0, s, r,
+                         cur, pl.ntx, pl.ntiles);
+    return 0;
+  }
+};
+
+template <typename CT, typename VT, bool P, int TY>
+struct OccFn {
+  static constexpr int NW = TileNW<TY>::value;
+  static int call() {
+    int nb = 0;
+    EP2D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &nb, (const void*)k_fused<CT, VT, P, TY, NW>, 64 * NW, 0));
+    return nb;
+  }
+};
+
+template <typename CT, typename VT, bool P, int TY>
+struct SmemFn {
+  static int call() { return (int)Fused<CT, VT, P, TY, TileNW<TY>::value>::SMEM; }
+};
+
+template <typename CT, typename VT, bool P, int TY>
+struct NwFn {
+  static int call() { return TileNW<TY>::value; }
+};
+
+// Dispatch over scheme (double / float arithmetic), plasticity and tile height.
+template <template <typename, typename, bool, int> class Fn, typename... A>
+int dispatch_fused(int mode, bool plastic, int ty, A&&... a) {
+#define EP2D_TY_CASES(CT, VT, P)                                    \
+  switch (ty) {                                                     \
+    case 6: return Fn<CT, VT, P, 6>::call(std::forward<A>(a)...);   \
+    case 14: return Fn<CT, VT, P, 14>::call(std::forward<A>(a)...); \
+    case 30: return Fn<CT, VT, P, 30>::call(std::forward<A>(a)...); \
+    default: return Fn<CT, VT, P, 8>::call(std::forward<A>(a)...);  \
+  }
+  if (mode == 2) {
+    if (plastic) { EP2D_TY_CASES(float, float, true) } else { EP2D_TY_CASES(float, float, false) }
+  } else {
+    if (plastic) { EP2D_TY_CASES(double, double, true) } else { EP2D_TY_CASES(double, double, false) }
+  }
+#undef EP2D_TY_CASES
+}
 
 }  // namespace
 
@@ -655,35 +799,42 @@ void launch_plain_plasticity(const RveDev& r, const StressDev& f, hipStream_t s)
 }
 
 void launch_plain_disp(const RveDev& r, double* Ux, double* Uy, const StressDev& f,
-                       hipStream_t s) {
+                       hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const dim3 blk(64, 4);
-  hipLaunchKernelGGL(k_plain_disp, grid2(r.nX, r.nY, blk), blk, 0, s, r, Ux, Uy, f);
+  if (e0)
+    hipExtLaunchKernelGGL(k_plain_disp, grid2(r.nX, r.nY, blk), blk, 0, s, e0, e1, 0, r, Ux, Uy, f);
+  else
+    hipLaunchKernelGGL(k_plain_disp, grid2(r.nX, r.nY, blk), blk, 0, s, r, Ux, Uy, f);
   EP2D_HIP(hipGetLastError());
 }
 
-void launch_fused_fp64(const RveDev& r, int cur, bool plastic, hipStream_t s) {
-  const int ntx = tiles_x(r.nX), nb = ntx * tiles_y(r.nY);
-  if (plastic)
-    hipLaunchKernelGGL((k_fused<double, double, true>), dim3(nb), dim3(NT), 0, s, r, cur, ntx, nb);
-  else
-    hipLaunchKernelGGL((k_fused<double, double, false>), dim3(nb), dim3(NT), 0, s, r, cur, ntx, nb);
-  EP2D_HIP(hipGetLastError());
+bool valid_tile_rows(int ty) { return ty == 6 || ty == 8 || ty == 14 || ty == 30; }
+
+FusedPlan plan_fused(int mode, bool plastic, int ty, int nX, int nY, int device) {
+  (void)device;
+  if (!valid_tile_rows(ty)) ty = 8;
+  FusedPlan pl;
+  pl.ty = ty;
+  pl.ntx = (nX + 61) / 62;
+  pl.ntiles = pl.ntx * ((nY + ty - 1) / ty);
+  pl.grid = pl.ntiles;
+  pl.blocksPerCU = dispatch_fused<OccFn>(mode, plastic, ty);
+  pl.smem = dispatch_fused<SmemFn>(mode, plastic, ty);
+  pl.waves = dispatch_fused<NwFn>(mode, plastic, ty);
+  return pl;
 }
 
-void launch_fused_split(const RveDev& r, int cur, bool plastic, hipStream_t s) {
-  const int ntx = tiles_x(r.nX), nb = ntx * tiles_y(r.nY);
-  if (plastic)
-    hipLaunchKernelGGL((k_fused<float, float, true>), dim3(nb), dim3(NT), 0, s, r, cur, ntx, nb);
-  else
-    hipLaunchKernelGGL((k_fused<float, float, false>), dim3(nb), dim3(NT), 0, s, r, cur, ntx, nb);
+void launch_fused(const RveDev& r, int cur, int mode, bool plastic, const FusedPlan& pl,
+                  hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  dispatch_fused<LaunchFn>(mode, plastic, pl.ty, r, cur, pl, s, e0, e1);
   EP2D_HIP(hipGetLastError());
 }
 
 const char* fused_kernel_name(int mode) {
   switch (mode) {
     case 0: return "k_plain_disp";
-    case 1: return "k_fused<double, double, true>";
-    default: return "k_fused<float, float, true>";
+    case 1: return "k_fused<double, double, true, TY>";
+    default: return "k_fused<float, float, true, TY>";
   }
 }
 

@@ -19,6 +19,13 @@ namespace ep2d {
 
 namespace {
 double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max semantics
+
+// Tile height of the fused kernels when the caller does not choose one.
+int default_tile_rows(int mode, int nX) {
+  (void)mode;
+  (void)nX;
+  return 8;
+}
 }  // namespace
 
 // get_sigma_2D.m:3-32 (MATLAB evaluates left to right).
@@ -154,6 +161,18 @@ Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& 
   r_.vdampX = 1.0 - dT_ * dampX_;      // :19
   r_.vdampY = 1.0 - dT_ * dampY_;      // :26
   r_.dTrho = dT_ / rho_;               // :19
+  // coh2lo <= coh^2 (real): s <= coh2lo implies RN(sqrt(s)) <= coh, i.e. no yield
+  r_.coh2lo = (coh_ * coh_) * (1.0 - 0x1p-50);
+  {
+    const double cf = (double)(float)coh_;
+    r_.coh2lof = (float)((cf * cf) * (1.0 - 0x1p-20));
+  }
+  // vertex shear modulus 0.25*(G[i,j] + G[i+1,j] + G[i,j+1] + G[i+1,j+1]) (:72)
+  // for every in-pore pattern of the four cells, summed in the reference order
+  for (int c = 0; c < 16; c++) {
+    auto g = [&](int bit) { return ((c >> bit) & 1) ? r_.Gp : r_.G0; };
+    r_.gtab[c] = 0.25 * (g(0) + g(1) + g(2) + g(3));
+  }
 
   // SetMaterials / SetMatrixZero (:708-794): masks, zero fields.
   launch_build_mask(const_cast<unsigned char*>(r_.mask), nX, nY, dX_, dY_, rad_, nPores_, stream_);
@@ -171,6 +190,12 @@ Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& 
   for (double* p : {f_.P, f_.tauXX, f_.tauYY, f_.tauXYav, f_.J2}) launch_fill(p, nC, 0.0, stream_);
   for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
   buildGatherTable();
+  if (fused) {
+    int ty = opt_.tileRows;
+    if (ty != 0 && !valid_tile_rows(ty)) throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
+    if (ty == 0) ty = default_tile_rows(mode_, nX);
+    plan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
+  }
   EP2D_HIP(hipStreamSynchronize(stream_));
 
   if (!logPath_.empty()) log_.open(logPath_, std::ios_base::app);   // :1607
@@ -288,6 +313,8 @@ void Solver::launchIteration() {
     ev = freeEvents_.back();
     freeEvents_.pop_back();
   }
+  // timed launches carry their own start/stop events (hipExtLaunchKernelGGL),
+  // which record the dispatch's execution window
   if (mode_ == EP2D_MODE_PLAIN) {
     // EffPlast2D.cu:390-399 (the reference synchronises the device after each
     // launch; stream order gives the same dependencies)
@@ -296,16 +323,11 @@ void Solver::launchIteration() {
       launch_plain_j2(r_, f_, stream_);
       launch_plain_plasticity(r_, f_, stream_);
     }
-    if (timed) EP2D_HIP(hipEventRecord(ev.first, stream_));
-    launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
-    if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
+    launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_, timed ? ev.first : nullptr,
+                      timed ? ev.second : nullptr);
   } else {
-    if (timed) EP2D_HIP(hipEventRecord(ev.first, stream_));
-    if (mode_ == EP2D_MODE_FP64)
-      launch_fused_fp64(r_, cur_, plastic, stream_);
-    else
-      launch_fused_split(r_, cur_, plastic, stream_);
-    if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
+    launch_fused(r_, cur_, mode_, plastic, plan_, stream_, timed ? ev.first : nullptr,
+                 timed ? ev.second : nullptr);
     cur_ ^= 1;
     stressStale_ = true;
   }
@@ -347,7 +369,16 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   resolveTiming(true);
   if (avgMs) *avgMs = timedSamples_ ? timedMs_ / timedSamples_ : 0.0;
   if (samples) *samples = timedSamples_;
-  if (name) *name = fused_kernel_name(mode_);
+  if (name) {
+    static thread_local std::string nm;
+    nm = fused_kernel_name(mode_);
+    if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(plan_.ty) + " grid=" +
+                                        std::to_string(plan_.grid) + " tiles=" +
+                                        std::to_string(plan_.ntiles) + " waves=" +
+                                        std::to_string(plan_.waves) + " blocks/CU=" +
+                                        std::to_string(plan_.blocksPerCU);
+    *name = nm.c_str();
+  }
 }
 
 void Solver::sync() { EP2D_HIP(hipStreamSynchronize(stream_)); }

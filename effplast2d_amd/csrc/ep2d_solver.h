@@ -88,6 +88,7 @@ class Solver {
   int device_ = 0;
   RveDev r_{};
   StressDev f_{};
+  FusedPlan plan_{};
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;

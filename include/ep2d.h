@@ -101,6 +101,7 @@ typedef struct ep2d_options {
   int verbose;          /* 1: print the reference's console lines to stdout */
   const char* logPath;  /* non-NULL: append the reference's log lines here
                            (EffPlast2D.log, :1607) */
+  int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */

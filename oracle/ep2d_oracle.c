@@ -16,6 +16,10 @@
 #include <omp.h>
 #endif
 
+#ifndef ORA_MUTANT
+#define ORA_MUTANT 0   /* >0 only in the pin tests' planted-mistake builds */
+#endif
+
 #define IDX_C(i, j) ((j) * nX + (i))             /* cell           */
 #define IDX_V(i, j) ((j) * (nX - 1) + (i))       /* vertex         */
 #define IDX_UX(i, j) ((j) * (nX + 1) + (i))      /* Ux / Vx        */
@@ -133,17 +137,33 @@ void ora_compute_stress(const ora_config* c, ora_fields* f) {
 #pragma omp parallel for schedule(static) num_threads(nthreads_of(c))
   for (long j = 0; j < nY; j++) {
     for (long i = 0; i < nX; i++) {
+#if ORA_MUTANT == 3   /* planted mistake for the pin tests: dropped eyy term */
+      P[j * nX + i] = P0[j * nX + i] - K[j * nX + i] * (
+        (Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX);
+#else
       P[j * nX + i] = P0[j * nX + i] - K[j * nX + i] * (
         (Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX + (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY);
+#endif
+#if ORA_MUTANT == 4   /* planted mistake: transposed operand (eyy for exx) */
+      tauXX[j * nX + i] = 2.0 * G[j * nX + i] * (
+        (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY -
+        ((Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX + (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY) / 3.0);
+#else
       tauXX[j * nX + i] = 2.0 * G[j * nX + i] * (
         (Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX -
         ((Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX + (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY) / 3.0);
+#endif
       tauYY[j * nX + i] = 2.0 * G[j * nX + i] * (
         (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY -
         ((Ux[j * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i]) / dX + (Uy[(j + 1) * nX + i] - Uy[j * nX + i]) / dY) / 3.0);
       if (i < nX - 1 && j < nY - 1) {
+#if ORA_MUTANT == 1   /* planted mistake: wrong sign of dUy/dx */
+        tauXY[j * (nX - 1) + i] = 0.25 * (G[j * nX + i] + G[j * nX + i + 1] + G[(j + 1) * nX + i] + G[(j + 1) * nX + i + 1]) * (
+          (Ux[(j + 1) * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i + 1]) / dY - (Uy[(j + 1) * nX + i + 1] - Uy[(j + 1) * nX + i]) / dX);
+#else
         tauXY[j * (nX - 1) + i] = 0.25 * (G[j * nX + i] + G[j * nX + i + 1] + G[(j + 1) * nX + i] + G[(j + 1) * nX + i + 1]) * (
           (Ux[(j + 1) * (nX + 1) + i + 1] - Ux[j * (nX + 1) + i + 1]) / dY + (Uy[(j + 1) * nX + i + 1] - Uy[(j + 1) * nX + i]) / dX);
+#endif
       }
       if (f->cellPore[j * nX + i]) {
         P[j * nX + i] = 0.0;
@@ -239,7 +259,11 @@ void ora_compute_disp(const ora_config* c, ora_fields* f) {
         Vx[j * (nX + 1) + i] = Vx[j * (nX + 1) + i] * (1.0 - dT * dampX) + (dT / rho) * ((
           -P[j * nX + i] + P[j * nX + i - 1] + tauXX[j * nX + i] - tauXX[j * nX + i - 1]
           ) / dX + (
+#if ORA_MUTANT == 2   /* planted mistake: shear term one vertex to the right (kept in bounds) */
+          tauXY[j * (nX - 1) + (i < nX - 1 ? i : i - 1)] - tauXY[(j - 1) * (nX - 1) + (i < nX - 1 ? i : i - 1)]
+#else
           tauXY[j * (nX - 1) + i - 1] - tauXY[(j - 1) * (nX - 1) + i - 1]
+#endif
           ) / dY);
       }
       if (i > 0 && i < nX - 1 && j > 0 && j < nY) {

@@ -1,0 +1,141 @@
+"""C ABI checks that need no GPU: the library loads, exports every symbol the
+public header declares, and its host-only entry points (parameter
+derivation, pa.dat I/O, option defaults, error reporting) behave like the
+reference's; the *.dat layout round-trips through read_data_2D's reading."""
+import ctypes as C
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import effplast2d_amd as ep
+from effplast2d_amd import _lib
+from effplast2d_amd.datfile import dat_name, read_data_2D, write_dat
+from oracle import ora
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(ep2d_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for n in sorted(names):
+        assert hasattr(L, n), n
+    # the Python binding types every declared entry point
+    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_device_count():
+    assert b"gfx950" in _lib.lib().ep2d_version()
+    assert ep.device_count() >= 0
+
+
+def test_derive_params_matches_golden_and_oracle():
+    gold = json.load(open(os.path.join(GOLDEN, "pa_golden.json")))
+    for name, g in gold.items():
+        i = g["inputs"]
+        pa, nx = ep.derive_params(Lx=i["Lx"], Ly=i["Ly"], nGrid=i["nGrid"], nPores=i["nPores"],
+                                  porosity=i["porosity"], Y=i["Y"], K0=i["K0"], G0=i["G0"],
+                                  rho0=i["rho0"], CFL=i["CFL"])
+        assert nx == g["nX"]
+        assert list(pa) == [float.fromhex(h) for h in g["pa_hex"]], name
+
+
+def test_pa_file_round_trip(tmp_path):
+    src = os.path.join(GOLDEN, "pa_config2_1024_plastic.dat")
+    pa = ep.read_pa(src)
+    out = tmp_path / "pa.dat"
+    ep.write_pa(out, pa)
+    assert open(out, "rb").read() == open(src, "rb").read()
+
+
+def test_read_pa_errors():
+    with pytest.raises(ep.Ep2dError) as e:
+        ep.read_pa("/nonexistent/data/pa.dat")
+    assert e.value.status == -3 and "Cannot open file" in str(e.value)
+
+
+def test_derive_params_rejects_bad_inputs():
+    with pytest.raises(ep.Ep2dError) as e:
+        ep.derive_params(nGrid=0)
+    assert e.value.status == -1
+    with pytest.raises(ep.Ep2dError):
+        ep.derive_params(porosity=1.5)
+
+
+def test_default_options_are_the_reference_macros():
+    o = _lib.Options()
+    _lib.lib().ep2d_default_options(C.byref(o))
+    assert (o.nTasks, o.nIter, o.eIter, o.checkInterval, o.device, o.nTimeSteps) == \
+        (3, 100000, 1e-10, 10000, 0, 1)
+
+
+def test_create_reports_errors_without_exiting():
+    """Without a GPU (or with bad arguments) creation returns an error code;
+    the reference would exit(-1) from gpuAssert."""
+    pa, nx = ep.derive_params(nGrid=1)
+    h = C.c_void_p()
+    L = _lib.lib()
+    o = _lib.Options()
+    L.ep2d_default_options(C.byref(o))
+    rc = L.ep2d_create(pa.ctypes.data_as(C.POINTER(C.c_double)), nx, nx + 1, C.byref(o), C.byref(h))
+    assert rc == -1 and b"square" in L.ep2d_last_error()
+    o.nIter = 0
+    rc = L.ep2d_create(pa.ctypes.data_as(C.POINTER(C.c_double)), nx, nx, C.byref(o), C.byref(h))
+    assert rc == -1
+    if ep.device_count() == 0:
+        L.ep2d_default_options(C.byref(o))
+        rc = L.ep2d_create(pa.ctypes.data_as(C.POINTER(C.c_double)), nx, nx, C.byref(o),
+                           C.byref(h))
+        assert rc in (-1, -2) and not h.value
+    assert L.ep2d_iterate(None, 1) == -1
+    L.ep2d_destroy(None)
+
+
+def test_dat_file_layout_round_trip(tmp_path, monkeypatch):
+    """SaveMatrix writes A[j*m + i] raw; read_data_2D.m reshapes to (m, n)
+    column-major, so MATLAB's field(i+1, j+1) is A[j*m + i]."""
+    m, n = 5, 3
+    A = np.arange(m * n, dtype=np.float64) * 1.5 - 2.0
+    monkeypatch.chdir(tmp_path)
+    write_dat(dat_name("Pc", 128), A)
+    F = read_data_2D("Pc", 128, m, n)
+    assert F.shape == (m, n)
+    for i in range(m):
+        for j in range(n):
+            assert F[i, j] == A[j * m + i]
+    assert os.path.getsize(dat_name("Pc", 128)) == 8 * m * n
+    Z = read_data_2D("missing", 128, m, n)
+    assert not Z.any()
+
+
+def test_python_mirror_validates_arguments():
+    with pytest.raises(ValueError):
+        ep.EffPlast2D(np.zeros(5), 64)
+
+
+def test_oracle_is_not_imported_by_the_package():
+    import sys
+    pkg_dir = os.path.join(ROOT, "effplast2d_amd")
+    for fn in os.listdir(pkg_dir):
+        if fn.endswith(".py"):
+            assert "oracle" not in open(os.path.join(pkg_dir, fn)).read(), fn
+    for fn in os.listdir(os.path.join(pkg_dir, "csrc")):
+        if fn.endswith((".cpp", ".hip", ".h", "Makefile")):
+            assert "ep2d_oracle" not in open(os.path.join(pkg_dir, "csrc", fn)).read(), fn
+    assert ora.LIB_PATH not in [getattr(m, "__file__", "") for m in sys.modules.values()]

@@ -72,6 +72,34 @@ def test_iterations_match_oracle(mode, nX, nPores, nl, iters, seed):
         _note(f"  worst rel err {worst:.3e}")
 
 
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+@pytest.mark.parametrize("ty", [6, 8, 14, 30])
+def test_tile_heights_match_oracle(mode, ty):
+    """Every fused tile shape, on a grid with interior and edge tiles."""
+    nX, nPores, nl, iters = 200, 2, 3, 15
+    _note(f"tile mode={mode} ty={ty}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.03)
+    init = random_state(nX, 100 + ty, pa=pa, boundary_v=True)
+    if mode == "split":
+        init["Vx"] = init["Vx"].astype(np.float32).astype(np.float64)
+        init["Vy"] = init["Vy"].astype(np.float32).astype(np.float64)
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode=mode, tileRows=ty) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in STATE_FIELDS + STRESS_FIELDS:
+            got = g.get(k)
+            if mode == "fp64":
+                e = rel_err(got, s[k])
+                _note(f"  {k}: rel {e:.2e}, {np.count_nonzero(got != s[k])} differ")
+                assert e <= 1e-12, k
+            else:
+                e = l2_rel(got, s[k])
+                assert e <= (1e-6 if k in ("Ux", "Uy") else 1e-4), f"{k}: {e:.3e}"
+
+
 @pytest.mark.parametrize("nX,nPores,nl,iters,seed", [(64, 1, 3, 10, 11), (130, 3, 3, 40, 12),
                                                      (96, 2, 1, 25, 13)])
 def test_split_iterations_close_to_oracle(nX, nPores, nl, iters, seed):
