@@ -316,35 +316,37 @@ struct Fused {
     const unsigned char* __restrict__ msk = r.mask;
     const int sx = nX + 1;  // Ux row length
 
-    // ---- stage ----
-    if (threadIdx.x == 0) {
+    // ---- stage: issue every load of this thread first, then fill LDS ----
+    constexpr int KB = (BY + NW - 1) / NW;   // box rows per wave
+    double bux[KB][2], buy[KB][2];
+    unsigned int bm[KB][2];
 #pragma unroll
-      for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
-    }
-    for (int b = wave; b < BY; b += NW) {
+    for (int k = 0; k < KB; k++) {
+      const int b = wave + NW * k;
       const int gj = j0 - 2 + b;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         const int a = lane + 64 * h;
-        if (h == 1 && a >= BX) break;
         const int gi = i0 - 2 + a;
         double ux = 0.0, uy = 0.0;
-        unsigned char m = 0;
-        if (!EDGE) {
-          ux = Uxi[gj * sx + gi];
-          uy = Uyi[gj * nX + gi];
-          m = msk[gj * nX + gi];
-        } else if (gi >= 0 && gj >= 0) {
-          if (gi <= nX && gj < nY) ux = Uxi[gj * sx + gi];
-          if (gi < nX && gj <= nY) uy = Uyi[gj * nX + gi];
-          if (gi < nX && gj < nY) m = msk[gj * nX + gi];
+        unsigned int m = 0;
+        if (b < BY && a < BX) {
+          if (!EDGE) {
+            ux = Uxi[gj * sx + gi];
+            uy = Uyi[gj * nX + gi];
+            m = msk[gj * nX + gi];
+          } else if (gi >= 0 && gj >= 0) {
+            if (gi <= nX && gj < nY) ux = Uxi[gj * sx + gi];
+            if (gi < nX && gj <= nY) uy = Uyi[gj * nX + gi];
+            if (gi < nX && gj < nY) m = msk[gj * nX + gi];
+          }
         }
-        sUx[b * BX + a] = ux;
-        sUy[b * BX + a] = uy;
-        sM[b * BX + a] = m;
+        bux[k][h] = ux;
+        buy[k][h] = uy;
+        bm[k][h] = m;
       }
     }
-    // this thread's velocities (read early: the latency overlaps phases 1-2)
+    // this thread's velocities (consumed in phase 3: the latency overlaps phases 1-2)
     const VT* __restrict__ Vxi = static_cast<const VT*>(r.Vx);
     const VT* __restrict__ Vyi = static_cast<const VT*>(r.Vy);
     VT vxr[KP], vyr[KP];
@@ -356,6 +358,23 @@ struct Fused {
       if (b < TY && lane < TX && (!EDGE || (i < nX && j < nY))) {
         vxr[k] = Vxi[j * sx + i];
         vyr[k] = Vyi[j * nX + i];
+      }
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
+    }
+#pragma unroll
+    for (int k = 0; k < KB; k++) {
+      const int b = wave + NW * k;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int a = lane + 64 * h;
+        if (b < BY && a < BX) {
+          sUx[b * BX + a] = bux[k][h];
+          sUy[b * BX + a] = buy[k][h];
+          sM[b * BX + a] = (unsigned char)bm[k][h];
+        }
       }
     }
     lds_barrier();
@@ -373,7 +392,11 @@ struct Fused {
     }
 
     // ---- phase 1: trial stress ----
-    for (int b = wave; b < CY; b += NW) {
+    constexpr int K1 = (CY + NW - 1) / NW;
+#pragma unroll
+    for (int k1 = 0; k1 < K1; k1++) {
+      const int b = wave + NW * k1;
+      if (b >= CY) break;
       const int a = lane;
       {  // cell (i0-1+a, j0-1+b) <-> box (a+1, b+1)
         const int ci = i0 - 1 + a, cj = j0 - 1 + b;
@@ -418,7 +441,11 @@ struct Fused {
       if constexpr (sizeof(CT) == 8) c2lo = r.coh2lo;
       else c2lo = r.coh2lof;
       // phase 2a: vertex (i0-1+a, j0-1+b) <-> trial vertex (a+1, b+1), trial cells (a..a+1, b..b+1)
-      for (int b = wave; b < QY; b += NW) {
+      constexpr int K2 = (QY + NW - 1) / NW;
+#pragma unroll
+      for (int k2 = 0; k2 < K2; k2++) {
+        const int b = wave + NW * k2;
+        if (b >= QY) break;
         const int a = lane;
         if (a < S - 1) {
           const int vi = i0 - 1 + a, vj = j0 - 1 + b;
@@ -437,7 +464,10 @@ struct Fused {
       }
       lds_barrier();
       // phase 2b: cell (i0-1+a, j0-1+b) <-> trial cell (a, b), corrected in place
-      for (int b = wave; b < QY; b += NW) {
+#pragma unroll
+      for (int k2 = 0; k2 < K2; k2++) {
+        const int b = wave + NW * k2;
+        if (b >= QY) break;
         const int a = lane;
         if (a < S - 1) {
           const int ci = i0 - 1 + a, cj = j0 - 1 + b;
