@@ -87,6 +87,43 @@ class Result(C.Structure):
         return d
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [("mode", C.c_int), ("checkInterval", C.c_long), ("tileRows", C.c_int),
+                ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
+                ("useGraphs", C.c_int), ("timingEvery", C.c_long)]
+
+
+class Request(C.Structure):
+    _fields_ = [("pa", C.c_double * NPARS), ("nX", C.c_int), ("nTasks", C.c_int),
+                ("nIter", C.c_long), ("eIter", C.c_double), ("nTimeSteps", C.c_int),
+                ("initLoadValue", C.c_double), ("loadType", C.c_double * 3),
+                ("loadValue", C.c_double), ("useShearLoadType", C.c_int),
+                ("shearLoadType", C.c_double * 3), ("tag", C.c_longlong)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("tag", C.c_longlong), ("status", C.c_int),
+                ("result", Result), ("tSubmit", C.c_double), ("tAdmit", C.c_double),
+                ("tDone", C.c_double), ("message", C.c_char * 160)]
+
+    def as_dict(self):
+        return {"id": self.id, "tag": self.tag, "status": self.status,
+                "result": self.result.as_dict(), "tSubmit": self.tSubmit,
+                "tAdmit": self.tAdmit, "tDone": self.tDone,
+                "message": self.message.decode(errors="replace")}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [(n, C.c_longlong) for n in ("launches", "chunks", "tilesLaunched", "submitted",
+                                            "finished", "failed", "cellIterations")] + \
+               [("bytesAlgorithmic", C.c_double), ("timedMs", C.c_double),
+                ("timedBytes", C.c_double), ("timedLaunches", C.c_longlong),
+                ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Ep2dError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -127,6 +164,16 @@ SIGNATURES = {
     "ep2d_diag_copy": ([C.c_int, C.c_size_t, C.c_int, _dp], C.c_int),
     "ep2d_sync": ([C.c_void_p], C.c_int),
     "ep2d_device_bytes": ([C.c_void_p], C.c_size_t),
+    "ep2d_batch_default_options": ([C.POINTER(BatchOptions)], None),
+    "ep2d_batch_create": ([C.POINTER(BatchOptions), C.POINTER(C.c_void_p)], C.c_int),
+    "ep2d_batch_destroy": ([C.c_void_p], None),
+    "ep2d_batch_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
+    "ep2d_batch_step": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "ep2d_batch_run": ([C.c_void_p], C.c_int),
+    "ep2d_batch_poll": ([C.c_void_p, C.POINTER(BatchResult), C.c_int, C.POINTER(C.c_int)],
+                        C.c_int),
+    "ep2d_batch_clock": ([C.c_void_p], C.c_double),
+    "ep2d_batch_stats_get": ([C.c_void_p, C.POINTER(BatchStats)], C.c_int),
 }
 
 _lib = None

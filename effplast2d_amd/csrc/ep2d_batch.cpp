@@ -1,0 +1,374 @@
+// ep2d_batch.cpp -- batched multi-RVE engine (see ep2d_batch.h).
+//
+// Each admitted request owns a Solver (its fields, masks, reduction scratch)
+// on the engine's stream.  An iteration of all admitted RVEs is one launch of
+// k_fused_batch over the concatenated tile lists; a chunk is the run of
+// launches up to the next residual check (the check interval is common, so
+// all RVEs reach their checks together).  Between chunks the host checks the
+// residuals (one synchronisation for all RVEs), post-processes finished load
+// steps, retires finished jobs and admits queued requests.  An RVE's
+// arithmetic does not depend on its neighbours in the launch, so its result
+// is the same as on the single-RVE path.
+#include "ep2d_batch.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+namespace ep2d {
+
+namespace {
+constexpr int kTileX = 62;  // fused tile width (Fused::TX)
+
+bool valid_mode(int m) { return m == EP2D_MODE_FP64 || m == EP2D_MODE_SPLIT; }
+}  // namespace
+
+Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
+  if (!valid_mode(opt_.mode)) throw ArgError("batch mode must be EP2D_MODE_FP64 or EP2D_MODE_SPLIT");
+  if (opt_.checkInterval <= 0) opt_.checkInterval = 10000;
+  if (opt_.tileRows != 0 && !valid_tile_rows(opt_.tileRows))
+    throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
+  ty_ = opt_.tileRows ? opt_.tileRows : 8;
+  device_ = opt_.device;
+  int ndev = 0;
+  EP2D_HIP(hipGetDeviceCount(&ndev));
+  if (device_ < 0 || device_ >= ndev) throw ArgError("device index out of range");
+  EP2D_HIP(hipSetDevice(device_));
+  EP2D_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  size_t freeB = 0, totalB = 0;
+  EP2D_HIP(hipMemGetInfo(&freeB, &totalB));
+  budget_ = opt_.memoryBudget ? opt_.memoryBudget : (size_t)(0.9 * (double)freeB);
+  t0_ = std::chrono::steady_clock::now();
+}
+
+Batch::~Batch() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  active_.clear();
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  for (auto& e : evPending_) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto& e : evPool_) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  if (dRves_) (void)hipFree(dRves_);
+  if (dTiles_) (void)hipFree(dTiles_);
+  if (hRves_) (void)hipHostFree(hRves_);
+  if (hTiles_) (void)hipHostFree(hTiles_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+double Batch::clock() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+long long Batch::submit(const ep2d_request& rq) {
+  if (rq.nX < 8) throw ArgError("request: nX must be >= 8");
+  if (rq.nTasks < 1 || rq.nIter < 1 || rq.nTimeSteps < 1)
+    throw ArgError("request: nTasks, nIter and nTimeSteps must be >= 1");
+  queue_.push_back({nextId_, rq, clock()});
+  st_.submitted++;
+  return nextId_++;
+}
+
+// Device bytes one RVE holds (Solver allocations), to admit within budget.
+size_t Batch::estimateBytes(const ep2d_request& rq) const {
+  const size_t n = (size_t)rq.nX;
+  const size_t pts = (n + 1) * n;
+  const size_t vb = opt_.mode == EP2D_MODE_SPLIT ? 4 : 8;
+  return pts * 8 * 4 + pts * vb * 2 + n * n * (1 + 5 * 8) + (n - 1) * (n - 1) * 16 +
+         ((n + 63) / 64) * ((n + 3) / 4) * 88 + 64 * n * 8 + (1 << 16);
+}
+
+void Batch::admit() {
+  while (!queue_.empty()) {
+    if (opt_.maxActive > 0 && (int)active_.size() >= opt_.maxActive) return;
+    Job& jb = queue_.front();
+    const size_t need = estimateBytes(jb.rq);
+    if (used_ + need > budget_ && !active_.empty()) return;  // wait for retirements
+    Slot sl;
+    sl.id = jb.id;
+    sl.tag = jb.rq.tag;
+    sl.tSubmit = jb.tSubmit;
+    sl.tAdmit = clock();
+    ep2d_options o;
+    std::memset(&o, 0, sizeof(o));
+    o.nTasks = jb.rq.nTasks;
+    o.nIter = jb.rq.nIter;
+    o.eIter = jb.rq.eIter;
+    o.checkInterval = opt_.checkInterval;
+    o.mode = opt_.mode;
+    o.device = device_;
+    o.nTimeSteps = jb.rq.nTimeSteps;
+    o.useShearLoadType = jb.rq.useShearLoadType;
+    std::memcpy(o.shearLoadType, jb.rq.shearLoadType, sizeof(o.shearLoadType));
+    o.tileRows = ty_;
+    try {
+      sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_);
+      sl.bytes = sl.s->deviceBytes();
+      sl.s->jobBegin(jb.rq.nTasks > 3 ? 1 : 0, jb.rq.initLoadValue, jb.rq.loadType,
+                     jb.rq.loadValue);
+      sl.s->tsBegin();
+    } catch (const StatusError& e) {
+      fail(sl, e.status, e.what());
+      queue_.pop_front();
+      continue;
+    } catch (const ArgError& e) {
+      fail(sl, EP2D_ERR_ARG, e.what());
+      queue_.pop_front();
+      continue;
+    } catch (const HipError& e) {
+      fail(sl, EP2D_ERR_HIP, e.what());
+      queue_.pop_front();
+      continue;
+    }
+    used_ += sl.bytes;
+    active_.push_back(std::move(sl));
+    queue_.pop_front();
+  }
+}
+
+void Batch::fail(Slot& sl, int status, const char* what) {
+  ep2d_batch_result r;
+  std::memset(&r, 0, sizeof(r));
+  r.id = sl.id;
+  r.tag = sl.tag;
+  r.status = status;
+  r.tSubmit = sl.tSubmit;
+  r.tAdmit = sl.tAdmit;
+  r.tDone = clock();
+  std::snprintf(r.message, sizeof(r.message), "%s", what ? what : "");
+  finished_.push_back(r);
+  st_.failed++;
+}
+
+void Batch::finish(Slot& sl) {
+  ep2d_batch_result r;
+  std::memset(&r, 0, sizeof(r));
+  r.id = sl.id;
+  r.tag = sl.tag;
+  r.status = EP2D_OK;
+  sl.s->jobFinish(&r.result);
+  r.tSubmit = sl.tSubmit;
+  r.tAdmit = sl.tAdmit;
+  r.tDone = clock();
+  finished_.push_back(r);
+  st_.finished++;
+}
+
+void Batch::ensureCapacity(size_t nRves, size_t nTiles) {
+  if (nRves > capRves_) {
+    const size_t cap = std::max<size_t>(nRves, 2 * capRves_);
+    if (dRves_) EP2D_HIP(hipFree(dRves_));
+    if (hRves_) EP2D_HIP(hipHostFree(hRves_));
+    dRves_ = nullptr;
+    hRves_ = nullptr;
+    EP2D_HIP(hipMalloc((void**)&dRves_, cap * sizeof(RveDev)));
+    EP2D_HIP(hipHostMalloc((void**)&hRves_, cap * sizeof(RveDev), hipHostMallocDefault));
+    capRves_ = cap;
+    gLen_ = -1;
+  }
+  if (nTiles > capTiles_) {
+    const size_t cap = std::max<size_t>(nTiles, 2 * capTiles_);
+    if (dTiles_) EP2D_HIP(hipFree(dTiles_));
+    if (hTiles_) EP2D_HIP(hipHostFree(hTiles_));
+    dTiles_ = nullptr;
+    hTiles_ = nullptr;
+    EP2D_HIP(hipMalloc((void**)&dTiles_, cap * sizeof(int4)));
+    EP2D_HIP(hipHostMalloc((void**)&hTiles_, cap * sizeof(int4), hipHostMallocDefault));
+    capTiles_ = cap;
+    gLen_ = -1;
+  }
+}
+
+// c launches of the batched kernel over every admitted RVE.  Launch q uses
+// parity q&1; RVE k starts from buffer cur_k, so its table offset is cur_k.
+void Batch::launchChunk(long c) {
+  std::vector<long long> key;
+  key.reserve(2 * active_.size() + 1);
+  size_t nt = 0;
+  for (auto& sl : active_) {
+    key.push_back(sl.id);
+    key.push_back(sl.s->cur());
+    nt += (size_t)sl.s->fusedPlan().ntiles;
+  }
+  key.push_back(c);
+  const bool sameTables = key == gKey_;
+  if (!sameTables) {
+    ensureCapacity(active_.size(), nt);
+    size_t t = 0;
+    for (size_t k = 0; k < active_.size(); k++) {
+      const Solver& s = *active_[k].s;
+      hRves_[k] = s.dev();
+      const FusedPlan& fp = s.fusedPlan();
+      const int nty = fp.ntiles / fp.ntx;
+      for (int ty = 0; ty < nty; ty++)
+        for (int tx = 0; tx < fp.ntx; tx++)
+          hTiles_[t++] = make_int4((int)k, tx * kTileX, ty * fp.ty, s.cur());
+    }
+    nTiles_ = (int)t;
+    EP2D_HIP(hipMemcpyAsync(dRves_, hRves_, active_.size() * sizeof(RveDev),
+                            hipMemcpyHostToDevice, stream_));
+    EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, t * sizeof(int4), hipMemcpyHostToDevice, stream_));
+  }
+  double bytes = 0.0, cells = 0.0;
+  for (auto& sl : active_) {
+    const double n = sl.s->nX();
+    bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
+    cells += n * n;
+  }
+  if (opt_.useGraphs) {
+    if (!sameTables || gLen_ != c) {
+      if (gexec_) EP2D_HIP(hipGraphExecDestroy(gexec_));
+      if (graph_) EP2D_HIP(hipGraphDestroy(graph_));
+      gexec_ = nullptr;
+      graph_ = nullptr;
+      EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      for (long q = 0; q < c; q++)
+        launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_);
+      EP2D_HIP(hipStreamEndCapture(stream_, &graph_));
+      EP2D_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
+      gLen_ = c;
+      gKey_ = key;
+    }
+    // with graphs the chunk is timed whole and counted as c launches
+    const bool timed = opt_.timingEvery > 0;
+    Pending ev{};
+    if (timed) {
+      ev = takeEvents();
+      ev.bytes = c * bytes;
+      ev.launches = c;
+      EP2D_HIP(hipEventRecord(ev.a, stream_));
+    }
+    EP2D_HIP(hipGraphLaunch(gexec_, stream_));
+    if (timed) {
+      EP2D_HIP(hipEventRecord(ev.b, stream_));
+      evPending_.push_back(ev);
+    }
+  } else {
+    for (long q = 0; q < c; q++) {
+      const bool timed = opt_.timingEvery > 0 && (st_.launches + q) % opt_.timingEvery == 0;
+      Pending ev{};
+      if (timed) {
+        ev = takeEvents();
+        ev.bytes = bytes;
+        ev.launches = 1;
+      }
+      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_,
+                         ev.a, ev.b);
+      if (timed) evPending_.push_back(ev);
+    }
+    gKey_ = key;
+    gLen_ = -1;  // tables may be reused, no graph to reuse
+  }
+  st_.launches += c;
+  st_.chunks++;
+  st_.tilesLaunched += (long long)c * nTiles_;
+  st_.cellIterations += (long long)(c * cells);
+  st_.bytesAlgorithmic += c * bytes;
+}
+
+Batch::Pending Batch::takeEvents() {
+  if (evPool_.empty()) {
+    Pending p{};
+    EP2D_HIP(hipEventCreate(&p.a));
+    EP2D_HIP(hipEventCreate(&p.b));
+    return p;
+  }
+  Pending p = evPool_.back();
+  evPool_.pop_back();
+  return p;
+}
+
+void Batch::resolveEvents() {
+  for (auto& e : evPending_) {
+    float ms = 0.f;
+    EP2D_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+    st_.timedMs += ms;
+    st_.timedBytes += e.bytes;
+    st_.timedLaunches += e.launches;
+    evPool_.push_back(e);
+  }
+  evPending_.clear();
+}
+
+void Batch::step(int* nActive, int* nQueued) {
+  EP2D_HIP(hipSetDevice(device_));
+  admit();
+  if (!active_.empty()) {
+    long c = active_.front().s->tsChunk();
+    for (auto& sl : active_) c = std::min(c, sl.s->tsChunk());
+    c = std::max<long>(c, 1);
+    launchChunk(c);
+    for (auto& sl : active_) {
+      sl.s->noteBatchIterations(c);
+      if (sl.s->tsAtCheck()) sl.s->tsCheckLaunch();
+    }
+    EP2D_HIP(hipStreamSynchronize(stream_));
+    resolveEvents();
+    std::vector<char> gone(active_.size(), 0);
+    for (size_t k = 0; k < active_.size(); k++) {
+      Slot& sl = active_[k];
+      try {
+        if (sl.s->tsAtCheck()) sl.s->tsCheckFinish();
+        if (sl.s->tsOver()) {
+          sl.s->tsFinish();
+          if (sl.s->jobDone()) {
+            finish(sl);
+            gone[k] = 1;
+          } else {
+            sl.s->tsBegin();
+          }
+        }
+      } catch (const StatusError& e) {
+        fail(sl, e.status, e.what());
+        gone[k] = 1;
+      } catch (const ArgError& e) {
+        fail(sl, EP2D_ERR_ARG, e.what());
+        gone[k] = 1;
+      }
+    }
+    std::vector<Slot> keep;
+    keep.reserve(active_.size());
+    for (size_t k = 0; k < active_.size(); k++) {
+      if (gone[k]) {
+        used_ -= active_[k].bytes;
+        active_[k].s.reset();
+      } else {
+        keep.push_back(std::move(active_[k]));
+      }
+    }
+    active_.swap(keep);
+    admit();
+  }
+  if (nActive) *nActive = (int)active_.size();
+  if (nQueued) *nQueued = (int)queue_.size();
+}
+
+void Batch::run() {
+  int a = 0, q = 0;
+  do {
+    step(&a, &q);
+  } while (a > 0 || q > 0);
+}
+
+int Batch::poll(ep2d_batch_result* out, int capacity) {
+  const int n = std::min<int>(capacity, (int)finished_.size());
+  for (int k = 0; k < n; k++) out[k] = finished_[k];
+  finished_.erase(finished_.begin(), finished_.begin() + n);
+  return n;
+}
+
+void Batch::stats(ep2d_batch_stats* s) const {
+  *s = st_;
+  s->active = (int)active_.size();
+  s->queued = (int)queue_.size();
+  s->deviceBytes = used_;
+}
+
+}  // namespace ep2d

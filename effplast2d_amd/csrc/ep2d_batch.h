@@ -1,0 +1,87 @@
+// ep2d_batch.h -- many RVEs on one GPU: one launch per iteration over the
+// tiles of every admitted RVE, admission by device-memory budget, retirement
+// on convergence and backfill from a queue.
+#pragma once
+
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <vector>
+
+#include "ep2d_solver.h"
+
+namespace ep2d {
+
+class Batch {
+ public:
+  explicit Batch(const ep2d_batch_options& opt);
+  ~Batch();
+  Batch(const Batch&) = delete;
+  Batch& operator=(const Batch&) = delete;
+
+  long long submit(const ep2d_request& rq);
+  void step(int* nActive, int* nQueued);
+  void run();
+  int poll(ep2d_batch_result* out, int capacity);
+  double clock() const;
+  void stats(ep2d_batch_stats* s) const;
+
+ private:
+  struct Job {
+    long long id;
+    ep2d_request rq;
+    double tSubmit;
+  };
+  struct Slot {
+    std::unique_ptr<Solver> s;
+    long long id = 0, tag = 0;
+    double tSubmit = 0, tAdmit = 0;
+    size_t bytes = 0;
+  };
+
+  size_t estimateBytes(const ep2d_request& rq) const;
+  void admit();
+  void fail(Slot& sl, int status, const char* what);
+  void finish(Slot& sl);
+  void launchChunk(long c);
+  void ensureCapacity(size_t nRves, size_t nTiles);
+
+  ep2d_batch_options opt_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  std::chrono::steady_clock::time_point t0_;
+  std::deque<Job> queue_;
+  std::vector<Slot> active_;
+  std::vector<ep2d_batch_result> finished_;
+  long long nextId_ = 1;
+  size_t budget_ = 0, used_ = 0;
+
+  // device tables of the batched launch (pinned host staging)
+  RveDev* dRves_ = nullptr;
+  int4* dTiles_ = nullptr;
+  RveDev* hRves_ = nullptr;
+  int4* hTiles_ = nullptr;
+  size_t capRves_ = 0, capTiles_ = 0;
+  int nTiles_ = 0;
+  int ty_ = 8;
+
+  // graph of one chunk, reused while the tile table and chunk length repeat
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  long gLen_ = -1;
+  std::vector<long long> gKey_;
+
+  // statistics
+  ep2d_batch_stats st_{};
+  struct Pending {
+    hipEvent_t a = nullptr, b = nullptr;
+    double bytes = 0;  // algorithmic bytes of the timed launches
+    long launches = 0;
+  };
+  std::vector<Pending> evPool_;
+  std::vector<Pending> evPending_;
+  Pending takeEvents();
+  void resolveEvents();
+};
+
+}  // namespace ep2d

@@ -5,10 +5,14 @@
 #include <cstring>
 #include <string>
 
-#include "ep2d_solver.h"
+#include "ep2d_batch.h"
 
 struct ep2d_handle {
   ep2d::Solver* solver;
+};
+
+struct ep2d_batch {
+  ep2d::Batch* engine;
 };
 
 namespace {
@@ -43,6 +47,11 @@ int guarded(F&& f) {
 ep2d::Solver* S(ep2d_handle* h) {
   if (!h || !h->solver) throw ep2d::ArgError("null handle");
   return h->solver;
+}
+
+ep2d::Batch* B(ep2d_batch* b) {
+  if (!b || !b->engine) throw ep2d::ArgError("null batch");
+  return b->engine;
 }
 }  // namespace
 
@@ -308,6 +317,82 @@ int ep2d_sync(ep2d_handle* h) {
 size_t ep2d_device_bytes(ep2d_handle* h) {
   if (!h || !h->solver) return 0;
   return h->solver->deviceBytes();
+}
+
+// ---- batched engine -------------------------------------------------------------
+void ep2d_batch_default_options(ep2d_batch_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->mode = EP2D_MODE_SPLIT;
+  o->checkInterval = 500;
+  o->tileRows = 0;
+  o->device = 0;
+  o->memoryBudget = 0;
+  o->maxActive = 0;
+  o->useGraphs = 1;
+  o->timingEvery = 0;
+}
+
+int ep2d_batch_create(const ep2d_batch_options* opt, ep2d_batch** out) {
+  return guarded([&] {
+    if (!out) throw ep2d::ArgError("null argument");
+    *out = nullptr;
+    ep2d_batch_options o;
+    if (opt) o = *opt; else ep2d_batch_default_options(&o);
+    auto* b = new ep2d_batch{nullptr};
+    try {
+      b->engine = new ep2d::Batch(o);
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+void ep2d_batch_destroy(ep2d_batch* b) {
+  if (!b) return;
+  try {
+    delete b->engine;
+  } catch (...) {
+  }
+  delete b;
+}
+
+int ep2d_batch_submit(ep2d_batch* b, const ep2d_request* rq, long long* id) {
+  return guarded([&] {
+    if (!rq) throw ep2d::ArgError("null request");
+    const long long v = B(b)->submit(*rq);
+    if (id) *id = v;
+  });
+}
+
+int ep2d_batch_step(ep2d_batch* b, int* nActive, int* nQueued) {
+  return guarded([&] { B(b)->step(nActive, nQueued); });
+}
+
+int ep2d_batch_run(ep2d_batch* b) {
+  return guarded([&] { B(b)->run(); });
+}
+
+int ep2d_batch_poll(ep2d_batch* b, ep2d_batch_result* out, int capacity, int* count) {
+  return guarded([&] {
+    if (!out && capacity > 0) throw ep2d::ArgError("null buffer");
+    const int n = B(b)->poll(out, capacity);
+    if (count) *count = n;
+  });
+}
+
+double ep2d_batch_clock(ep2d_batch* b) {
+  if (!b || !b->engine) return 0.0;
+  return b->engine->clock();
+}
+
+int ep2d_batch_stats_get(ep2d_batch* b, ep2d_batch_stats* out) {
+  return guarded([&] {
+    if (!out) throw ep2d::ArgError("null argument");
+    B(b)->stats(out);
+  });
 }
 
 }  // extern "C"

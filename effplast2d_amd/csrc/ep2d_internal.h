@@ -48,6 +48,7 @@ struct RveDev {
   double coh;               // yield radius pa[8]
   double vdampX, vdampY;    // 1 - dT*dampX, 1 - dT*dampY
   double dTrho;             // dT / rho
+  int plastic;              // 1: return mapping on (nTasks > 1, EffPlast2D.cu:392)
   double coh2lo;            // below coh^2: a smaller squared J2 cannot exceed coh
   float coh2lof;            // the same bound for the FP32 arithmetic of the split scheme
   double gtab[16];          // vertex shear modulus by vertex class (mask bits 3..6)
@@ -106,6 +107,11 @@ FusedPlan plan_fused(int mode, bool plastic, int ty, int nX, int nY, int device)
 void launch_fused(const RveDev& r, int cur, int mode, bool plastic, const FusedPlan& pl,
                   hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 const char* fused_kernel_name(int mode);
+// batched fused iteration over many RVEs: tiles[k] = (rve, i0, j0, parity
+// offset); RVE k reads U[parity ^ offset].  All RVEs share mode and tile height.
+void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,
+                        int parity, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 // residual: out[0] = bits of max|Vx|, out[1] = bits of max|Vy|, out[2] = nonfinite count
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s);
 // volume averages: partial[nblocks*NAVG] then result[NAVG]

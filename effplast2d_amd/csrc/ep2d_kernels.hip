@@ -701,6 +701,30 @@ inline dim3 grid2(int nx, int ny, dim3 blk) {
   return dim3((nx + blk.x - 1) / blk.x, (ny + blk.y - 1) / blk.y, 1);
 }
 
+// Many RVEs in one launch: the tile list names the RVE, the tile origin and
+// the RVE's ping-pong offset; the RVE's parameters come from a device table.
+template <typename CT, typename VT, int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fused_batch(const RveDev* __restrict__ rves,
+                                                         const int4* __restrict__ tiles,
+                                                         int ntiles, int parity) {
+  using FP = Fused<CT, VT, true, TY, NW>;
+  using FE = Fused<CT, VT, false, TY, NW>;
+  static_assert(FP::SMEM == FE::SMEM, "same LDS layout");
+  __shared__ __attribute__((aligned(16))) char smem[FP::SMEM];
+  const int4 t = tiles[tile_of_block(blockIdx.x, ntiles)];
+  const RveDev& r = rves[t.x];
+  const int cur = parity ^ t.w;
+  const int i0 = t.y, j0 = t.z;
+  const bool interior = i0 >= 2 && i0 + FP::TX + 2 <= r.nX && j0 >= 2 && j0 + TY + 2 <= r.nY;
+  if (r.plastic) {
+    if (interior) FP::template run<false>(r, cur, i0, j0, smem);
+    else FP::template run<true>(r, cur, i0, j0, smem);
+  } else {
+    if (interior) FE::template run<false>(r, cur, i0, j0, smem);
+    else FE::template run<true>(r, cur, i0, j0, smem);
+  }
+}
+
 // Tile heights and their workgroup sizes (waves per block).
 template <int TY> struct TileNW;
 template <> struct TileNW<6> { static constexpr int value = 8; };
@@ -719,6 +743,21 @@ struct LaunchFn {
     else
       hipLaunchKernelGGL((k_fused<CT, VT, P, TY, NW>), dim3(pl.grid), dim3(64 * NW), 0, s, r,
                          cur, pl.ntx, pl.ntiles);
+    return 0;
+  }
+};
+
+template <typename CT, typename VT, bool P, int TY>
+struct BatchFn {
+  static constexpr int NW = TileNW<TY>::value;
+  static int call(const RveDev* rves, const int4* tiles, int ntiles, int parity, hipStream_t s,
+                  hipEvent_t e0, hipEvent_t e1) {
+    if (e0)
+      hipExtLaunchKernelGGL((k_fused_batch<CT, VT, TY, NW>), dim3(ntiles), dim3(64 * NW), 0, s,
+                            e0, e1, 0, rves, tiles, ntiles, parity);
+    else
+      hipLaunchKernelGGL((k_fused_batch<CT, VT, TY, NW>), dim3(ntiles), dim3(64 * NW), 0, s,
+                         rves, tiles, ntiles, parity);
     return 0;
   }
 };
@@ -857,6 +896,13 @@ FusedPlan plan_fused(int mode, bool plastic, int ty, int nX, int nY, int device)
 void launch_fused(const RveDev& r, int cur, int mode, bool plastic, const FusedPlan& pl,
                   hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   dispatch_fused<LaunchFn>(mode, plastic, pl.ty, r, cur, pl, s, e0, e1);
+  EP2D_HIP(hipGetLastError());
+}
+
+void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,
+                        int parity, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (ntiles <= 0) return;
+  dispatch_fused<BatchFn>(mode, true, ty, rves, tiles, ntiles, parity, s, e0, e1);
   EP2D_HIP(hipGetLastError());
 }
 

@@ -78,8 +78,20 @@ void Solver::d2h(T* dst, const T* src, size_t n) {
 }
 
 // EffPlast2D::EffPlast2D, EffPlast2D.cu:1541-1612.
-Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt)
+Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt,
+               hipStream_t stream)
     : nX_(nX), nY_(nY), opt_(opt) {
+  try {
+    init(pa, stream);
+  } catch (...) {
+    release();
+    throw;
+  }
+}
+
+void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
+  const int nX = nX_, nY = nY_;
+  const ep2d_options opt = opt_;
   std::memcpy(pa_, pa, sizeof(pa_));
   if (opt.logPath) logPath_ = opt.logPath;
   opt_.logPath = nullptr;
@@ -119,7 +131,12 @@ Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& 
   EP2D_HIP(hipGetDeviceCount(&ndev));
   if (device_ < 0 || device_ >= ndev) throw ArgError("device index out of range");
   EP2D_HIP(hipSetDevice(device_));
-  EP2D_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (stream) {
+    stream_ = stream;
+    ownStream_ = false;
+  } else {
+    EP2D_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  }
 
   const size_t nC = (size_t)nX * nY, nV = (size_t)(nX - 1) * (nY - 1);
   const size_t nUx = (size_t)(nX + 1) * nY, nUy = (size_t)nX * (nY + 1);
@@ -158,6 +175,7 @@ Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& 
   r_.Gp = 0.01 * G0_;                  // :722
   r_.P0 = 0.0;                         // SetInitPressure, :770-783
   r_.coh = coh_;
+  r_.plastic = opt_.nTasks > 1 ? 1 : 0;
   r_.vdampX = 1.0 - dT_ * dampX_;      // :19
   r_.vdampY = 1.0 - dT_ * dampY_;      // :26
   r_.dTrho = dT_ / rho_;               // :19
@@ -194,14 +212,16 @@ Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& 
     int ty = opt_.tileRows;
     if (ty != 0 && !valid_tile_rows(ty)) throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
     if (ty == 0) ty = default_tile_rows(mode_, nX);
-    plan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
+    fplan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
   }
   EP2D_HIP(hipStreamSynchronize(stream_));
 
   if (!logPath_.empty()) log_.open(logPath_, std::ios_base::app);   // :1607
 }
 
-Solver::~Solver() {
+Solver::~Solver() { release(); }
+
+void Solver::release() {
   // teardown: errors are ignored, there is no caller to report them to
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
@@ -213,11 +233,18 @@ Solver::~Solver() {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
   }
+  pending_.clear();
+  freeEvents_.clear();
   for (void* p : allocs_) (void)hipFree(p);
+  allocs_.clear();
   if (hMax_) (void)hipHostFree(hMax_);
   if (hAvg_) (void)hipHostFree(hAvg_);
   if (hGather_) (void)hipHostFree(hGather_);
-  if (stream_) (void)hipStreamDestroy(stream_);
+  hMax_ = nullptr;
+  hAvg_ = nullptr;
+  hGather_ = nullptr;
+  if (stream_ && ownStream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
 }
 
 // Gather table of the sampled values the host post-processing needs:
@@ -326,7 +353,7 @@ void Solver::launchIteration() {
     launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_, timed ? ev.first : nullptr,
                       timed ? ev.second : nullptr);
   } else {
-    launch_fused(r_, cur_, mode_, plastic, plan_, stream_, timed ? ev.first : nullptr,
+    launch_fused(r_, cur_, mode_, plastic, fplan_, stream_, timed ? ev.first : nullptr,
                  timed ? ev.second : nullptr);
     cur_ ^= 1;
     stressStale_ = true;
@@ -372,11 +399,11 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   if (name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
-    if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(plan_.ty) + " grid=" +
-                                        std::to_string(plan_.grid) + " tiles=" +
-                                        std::to_string(plan_.ntiles) + " waves=" +
-                                        std::to_string(plan_.waves) + " blocks/CU=" +
-                                        std::to_string(plan_.blocksPerCU);
+    if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(fplan_.ty) + " grid=" +
+                                        std::to_string(fplan_.grid) + " tiles=" +
+                                        std::to_string(fplan_.ntiles) + " waves=" +
+                                        std::to_string(fplan_.waves) + " blocks/CU=" +
+                                        std::to_string(fplan_.blocksPerCU);
     *name = nm.c_str();
   }
 }
@@ -401,7 +428,7 @@ void Solver::applyAffine(double dUxdx, double dUydy, double dUxdy, double dUydx)
   launch_affine(r_.Ux[cur_], r_.Uy[cur_], nX_, nY_, dX_, dY_, dUxdx, dUydy, dUxdy, dUydx, stream_);
 }
 
-// EffPlast2D.cu:407-412.
+// EffPlast2D.cu:407-412 (step-level entry point).
 double Solver::residual(const double e[3], long* nonfinite) {
   launch_maxabs(r_, mode_ == EP2D_MODE_SPLIT, dMax_, stream_);
   d2h(hMax_, dMax_, 3);
@@ -553,183 +580,249 @@ void Solver::holeArea(ep2d_record& r) {
   r.dPhiPer = std::fabs(r.PhiPer - r.Phi0);
 }
 
-// ---- ComputeEffParams -------------------------------------------------------
-void Solver::computeEffParams(int step, double loadStepValue, const double loadType[3], long nT,
-                              ep2d_result* res) {
+// ---- ComputeEffModuli / ComputeEffDamping as a resumable job -------------------
+// The reference runs a fixed sequence of load steps (EffPlast2D.cu:179-243),
+// each a loop of time steps (ComputeEffParams, :311-698) with an iteration loop
+// inside.  Here the sequence is a plan, and a time step is split into begin /
+// iterate-and-check / finish so that the same code drives one RVE on its own
+// (runJob) and many RVEs in one batched launch (the Batch engine).
+void Solver::jobBegin(int kind, double initLoadValue, const double loadType[3], double loadValue) {
   const int NL = opt_.nTasks;
-  printStepInfo(step);
-  if (step == 0) {   // :329-333: the macro strain and U restart, V does not
-    curEffStrain_[0] = curEffStrain_[1] = curEffStrain_[2] = 0.0;
-    launch_fill(r_.Ux[cur_], (size_t)(nX_ + 1) * nY_, 0.0, stream_);
-    launch_fill(r_.Uy[cur_], (size_t)nX_ * (nY_ + 1), 0.0, stream_);
-  }
-  for (long it = 0; it < nT; it++) {
-    ep2d_record r;
-    std::memset(&r, 0, sizeof(r));
-    say("Time step ", it + 1, " from ", nT, "\n");
-    const double dUxdx = loadStepValue * loadType[0] / (double)nT;   // :349-352
-    const double dUydy = loadStepValue * loadType[1] / (double)nT;
-    const double dUxdy = loadStepValue * loadType[2] / (double)nT;
-    const double dUydx = dUxdy;
-    curEffStrain_[0] += dUxdx;
-    curEffStrain_[1] += dUydy;
-    curEffStrain_[2] += dUxdy;
-    for (int k = 0; k < 3; k++) r.epsilon[k] = curEffStrain_[k];
-    say("Macro strain: (", curEffStrain_[0], ", ", curEffStrain_[1], ", ", curEffStrain_[2], ")\n");
-    applyAffine(dUxdx, dUydy, dUxdy, dUydx);
-
-    // iteration loop with a residual check every checkInterval iterations (:389-428)
-    double error = 0.0;
-    long done = 0;
-    bool conv = false;
-    const long K = opt_.checkInterval;
-    while (done < opt_.nIter) {
-      const long chunk = std::min(K - (done % K), opt_.nIter - done);
-      iterate(chunk);
-      done += chunk;
-      if (done % K == 0) {
-        long bad = 0;
-        error = residual(curEffStrain_, &bad);
-        if (bad)
-          throw StatusError(EP2D_ERR_NONFINITE, "non-finite velocity at iteration " +
-                                                    std::to_string(done) + " of load step " +
-                                                    std::to_string(step));
-        say("    Iteration ", done, ": Error is ", error, "\n");
-        if (error < opt_.eIter) {
-          say("Number of iterations is ", done, "\n\n");
-          conv = true;
-          break;
-        } else if (done >= opt_.nIter) {
-          say("WARNING: Maximum number of iterations reached!\nError is ", error, "\n\n");
-        }
-      }
-    }
-    r.error = error;
-    r.iters = (double)done;
-    totalIters_ += done;
-    if (!conv) allConverged_ = false;
-
-    averages(r);
-    boundary(r.deltaP, r.deltaPper, r.tauInfty);
-    holeArea(r);
-    logOnly("    dPhi = ", r.dPhi, "\n");
-    if (nPores_ > 2) logOnly("    dPhiPer = ", r.dPhiPer, "\n");
-    say("    P / Y = ", r.deltaP / Y_, "\n");
-    if (nPores_ > 2) say("    Pper / Y = ", r.deltaPper / Y_, "\n");
-    say("    tau / Y = ", r.tauInfty / Y_, "\n");
-
-    // analytic moduli of this step (:643-696)
-    const long nX = nX_, nY = nY_;
-    const double Phi0 = r.Phi0, Phi = r.Phi;
-    const double phi = 3.1415926 * rad_ * rad_ / (dX_ * (nX - 1) * dY_ * (nY - 1));
-    const double Y = Y_, G0 = G0_, K0 = K0_;
-    if (NL <= 1) {
-      res->KphiExactElast = G0 / Phi0;
-      logOnly("    KphiExactElast = ", res->KphiExactElast, "\n");
-      res->an.Kphi = G0 / Phi0;
-      res->KdExactElast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactElast);
-      logOnly("    KdExactElast = ", res->KdExactElast, "\n");
-      res->an.Kd = G0 / Phi0;
-    } else if (step == 0) {
-      res->KphiExactPlast = G0 / Phi / std::exp(std::fabs(r.deltaP) / Y - 1.0) /
-                            (1.0 + r.tauInfty * r.tauInfty / Y / Y);
-      logOnly("    KphiExactPlast = ", res->KphiExactPlast, "\n");
-      res->an.Kphi = res->KphiExactPlast;
-      res->KdExactPlast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlast);
-      logOnly("    KdExactPlast = ", res->KdExactPlast, "\n");
-      res->an.Kd = res->KdExactPlast;
-      res->KphiExactPlastPer = G0 / Phi0 / std::exp(std::fabs(r.deltaPper) / Y - 1.0) /
-                               (1.0 + r.tauInfty * r.tauInfty / Y / Y);
-      logOnly("    KphiExactPlastPer = ", res->KphiExactPlastPer, "\n");
-      res->an_per.Kphi = res->KphiExactPlastPer;
-      res->KdExactPlastPer = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlastPer);
-      logOnly("    KdExactPlastPer = ", res->KdExactPlastPer, "\n");
-      res->an_per.Kd = res->KdExactPlastPer;
-    }
-    if (NL == 3 && step == 0) {
-      res->GexactElast = G0 / (1.0 + Phi0);
-      logOnly("    GexactElast = ", res->GexactElast, "\n");
-      res->GexactPlast = G0 / (1.0 + Phi * std::exp(std::fabs(r.deltaP) / Y - 1.0));
-      logOnly("    GexactPlast = ", res->GexactPlast, "\n");
-      res->an.G = res->GexactPlast;
-      res->GexactPlastPer = G0 / (1.0 + Phi0 * std::exp(std::fabs(r.deltaPper) / Y - 1.0));
-      logOnly("    GexactPlastPer = ", res->GexactPlastPer, "\n");
-      res->an_per.G = res->GexactPlastPer;
-    }
-    records_.push_back(r);
-    recordStep_.push_back(step);
-  }
-}
-
-// EffPlast2D::ComputeEffModuli (EffPlast2D.cu:179-229) and the moduli getters
-// (:1412-1539).
-void Solver::computeEffModuli(double initLoadValue, const double loadType[3], ep2d_result* out) {
-  const int NL = opt_.nTasks;
-  if (NL > 3) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
+  if (kind == 0 && NL > 3) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
+  if (kind == 1 && NL < 4) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
   EP2D_HIP(hipSetDevice(device_));
-  const auto t0 = std::chrono::steady_clock::now();
-  ep2d_result res;
-  std::memset(&res, 0, sizeof(res));
+  jobT0_ = std::chrono::steady_clock::now();
+  std::memset(&res_, 0, sizeof(res_));
   records_.clear();
   recordStep_.clear();
   totalIters_ = 0;
   allConverged_ = true;
   std::memcpy(loadType_, loadType, sizeof(loadType_));
-  const double incPercent = 0.005;   // EffPlast2D.h:105
-  const double sph[3] = {0.5 * (loadType[0] + loadType[1]), 0.5 * (loadType[0] + loadType[1]), 0.0};
-  double dev[3] = {loadType[0] - sph[0], loadType[1] - sph[1], loadType[2]};
-  if (opt_.useShearLoadType) std::memcpy(dev, opt_.shearLoadType, sizeof(dev));
+  jobKind_ = kind;
+  plan_.clear();
   const long T = opt_.nTimeSteps;
-
+  plan_.push_back({initLoadValue, {loadType[0], loadType[1], loadType[2]}, T});
+  if (kind == 0) {   // ComputeEffModuli, :186-205
+    const double incPercent = 0.005;   // EffPlast2D.h:105
+    const double sph[3] = {0.5 * (loadType[0] + loadType[1]), 0.5 * (loadType[0] + loadType[1]), 0.0};
+    double dev[3] = {loadType[0] - sph[0], loadType[1] - sph[1], loadType[2]};
+    if (opt_.useShearLoadType) std::memcpy(dev, opt_.shearLoadType, sizeof(dev));
+    if (NL > 1) plan_.push_back({initLoadValue * incPercent, {sph[0], sph[1], sph[2]}, 1});
+    if (NL == 3) plan_.push_back({initLoadValue * incPercent, {dev[0], dev[1], dev[2]}, 1});
+  } else {           // ComputeEffDamping, :239-243
+    for (long i = 1; i < NL; i++) {
+      const double v = loadValue * (std::sin(i * 2.0 * 3.1415926 / (NL - 1)) -
+                                    std::sin((i - 1) * 2.0 * 3.1415926 / (NL - 1)));
+      plan_.push_back({v, {loadType[0], loadType[1], loadType[2]}, 1});
+    }
+  }
+  task_ = 0;
+  tstep_ = 0;
   printCalculationType();
-  computeEffParams(0, initLoadValue, loadType, T, &res);
-  const ep2d_record r0 = records_[T - 1];
-  if (NL == 1) {   // calcBulkModuli_PureElast, :1412-1481
-    res.num.Kphi = r0.deltaP / r0.dPhi;
-    logOnly("    ==============\n", "    Kphi = ", res.num.Kphi, "\n");
-    res.num_per.Kphi = r0.deltaPper / r0.dPhiPer;
-    logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
-    const double pInc = -0.33333333 * (r0.sigma[0] + r0.sigma[1] + r0.sigma[2]);
-    const double epsInc = r0.epsilon[0] + r0.epsilon[1];
-    res.num.Kd = -pInc / epsInc;
-    logOnly("    Kd = ", res.num.Kd, "\n");
-    const double pIncPer = -0.33333333 * (r0.sigmaPer[0] + r0.sigmaPer[1] + r0.sigmaPer[2]);
-    const double epsIncPer = r0.epsilonPer[0] + r0.epsilonPer[1];
-    res.num_per.Kd = -pIncPer / epsIncPer;
-    logOnly("    KdPer = ", res.num_per.Kd, "\n");
-  } else {         // calcBulkModuli_ElastPlast, :1429-1514
-    computeEffParams(1, initLoadValue * incPercent, sph, 1, &res);
-    const ep2d_record r1 = records_[T];
-    res.num.Kphi = (r1.deltaP - r0.deltaP) / (r1.dPhi - r0.dPhi);
-    logOnly("==============\n", "    Kphi = ", res.num.Kphi, "\n");
-    res.num_per.Kphi = (r1.deltaPper - r0.deltaPper) / (r1.dPhiPer - r0.dPhiPer);
-    logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
-    const double pInc = -0.33333333 * (r1.sigma[0] + r1.sigma[1] + r1.sigma[2] - r0.sigma[0] - r0.sigma[1] - r0.sigma[2]);
-    const double epsInc = r1.epsilon[0] + r1.epsilon[1] - r0.epsilon[0] - r0.epsilon[1];
-    res.num.Kd = -pInc / epsInc;
-    logOnly("    Kd = ", res.num.Kd, "\n");
-    const double pIncPer = -0.33333333 * (r1.sigmaPer[0] + r1.sigmaPer[1] + r1.sigmaPer[2] - r0.sigmaPer[0] - r0.sigmaPer[1] - r0.sigmaPer[2]);
-    const double epsIncPer = r1.epsilonPer[0] + r1.epsilonPer[1] - r0.epsilonPer[0] - r0.epsilonPer[1];
-    res.num_per.Kd = -pIncPer / epsIncPer;
-    logOnly("    KdPer = ", res.num_per.Kd, "\n");
+}
+
+// Start of a time step (EffPlast2D.cu:312-384): banner, macro strain, affine U increment.
+void Solver::tsBegin() {
+  const TaskPlan& tp = plan_[task_];
+  const int step = (int)task_;
+  if (tstep_ == 0) {
+    printStepInfo(step);
+    if (step == 0) {   // :329-333: the macro strain and U restart, V does not
+      curEffStrain_[0] = curEffStrain_[1] = curEffStrain_[2] = 0.0;
+      launch_fill(r_.Ux[cur_], (size_t)(nX_ + 1) * nY_, 0.0, stream_);
+      launch_fill(r_.Uy[cur_], (size_t)nX_ * (nY_ + 1), 0.0, stream_);
+    }
   }
-  if (NL == 3) {   // calcShearModulus, :1446-1539
-    computeEffParams(2, initLoadValue * incPercent, dev, 1, &res);
-    const ep2d_record r1 = records_[T];
-    const ep2d_record r2 = records_[T + 1];
-    res.num.G = 0.5 * (r2.sigma[0] - r1.sigma[0] - r2.sigma[1] + r1.sigma[1]) /
-                (r2.epsilon[0] - r1.epsilon[0] - r2.epsilon[1] + r1.epsilon[1]);
-    logOnly("==============\n", "    G = ", res.num.G, "\n");
-    res.num_per.G = 0.5 * (r2.sigmaPer[0] - r1.sigmaPer[0] - r2.sigmaPer[1] + r1.sigmaPer[1]) /
-                    (r2.epsilonPer[0] - r1.epsilonPer[0] - r2.epsilonPer[1] + r1.epsilonPer[1]);
-    logOnly("    Gper = ", res.num_per.G, "\n");
-    res.Gxy = 0.5 * (r2.sigmaXYclean - r1.sigmaXYclean) / (r2.epsilon[2] - r1.epsilon[2]);
-    res.GxyPer = 0.5 * (r2.sigmaPer[2] - r1.sigmaPer[2]) / (r2.epsilonPer[2] - r1.epsilonPer[2]);
+  std::memset(&rec_, 0, sizeof(rec_));
+  const long nT = tp.nT;
+  say("Time step ", tstep_ + 1, " from ", nT, "\n");
+  const double dUxdx = tp.load * tp.lt[0] / (double)nT;   // :349-352
+  const double dUydy = tp.load * tp.lt[1] / (double)nT;
+  const double dUxdy = tp.load * tp.lt[2] / (double)nT;
+  const double dUydx = dUxdy;
+  curEffStrain_[0] += dUxdx;
+  curEffStrain_[1] += dUydy;
+  curEffStrain_[2] += dUxdy;
+  for (int k = 0; k < 3; k++) rec_.epsilon[k] = curEffStrain_[k];
+  say("Macro strain: (", curEffStrain_[0], ", ", curEffStrain_[1], ", ", curEffStrain_[2], ")\n");
+  applyAffine(dUxdx, dUydy, dUxdy, dUydx);
+  tsDone_ = 0;
+  tsError_ = 0.0;
+  tsConv_ = false;
+}
+
+long Solver::tsChunk() const {
+  const long K = opt_.checkInterval;
+  return std::min(K - (tsDone_ % K), opt_.nIter - tsDone_);
+}
+
+bool Solver::tsOver() const { return tsConv_ || tsDone_ >= opt_.nIter; }
+
+// Residual of EffPlast2D.cu:407-412: launch the reduction and the copy of its
+// three words; tsCheckFinish reads them after the stream is synchronised.
+void Solver::tsCheckLaunch() {
+  launch_maxabs(r_, mode_ == EP2D_MODE_SPLIT, dMax_, stream_);
+  EP2D_HIP(hipMemcpyAsync(hMax_, dMax_, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          stream_));
+}
+
+bool Solver::tsCheckFinish() {
+  double mx, my;
+  std::memcpy(&mx, &hMax_[0], 8);
+  std::memcpy(&my, &hMax_[1], 8);
+  const double* e = curEffStrain_;
+  const double error = (mx / (dX_ * (nX_ - 1)) + my / (dY_ * (nY_ - 1))) * dT_ /
+                       (std_max(std::fabs(e[0]), std_max(e[1], e[2])));
+  if (hMax_[2])
+    throw StatusError(EP2D_ERR_NONFINITE, "non-finite velocity at iteration " +
+                                              std::to_string(tsDone_) + " of load step " +
+                                              std::to_string(task_));
+  tsError_ = error;
+  say("    Iteration ", tsDone_, ": Error is ", error, "\n");
+  if (error < opt_.eIter) {
+    say("Number of iterations is ", tsDone_, "\n\n");
+    tsConv_ = true;
+  } else if (tsDone_ >= opt_.nIter) {
+    say("WARNING: Maximum number of iterations reached!\nError is ", error, "\n\n");
   }
-  printEffectiveModuli(res);
-  printWarnings();
+  return tsOver();
+}
+
+// End of a time step (EffPlast2D.cu:430-696): averages, boundary means, hole
+// area, per-step analytic moduli; then move to the next time step or task.
+void Solver::tsFinish() {
+  const int NL = opt_.nTasks;
+  const int step = (int)task_;
+  ep2d_record& r = rec_;
+  ep2d_result* res = &res_;
+  r.error = tsError_;
+  r.iters = (double)tsDone_;
+  totalIters_ += tsDone_;
+  if (!tsConv_) allConverged_ = false;
+
+  averages(r);
+  boundary(r.deltaP, r.deltaPper, r.tauInfty);
+  holeArea(r);
+  logOnly("    dPhi = ", r.dPhi, "\n");
+  if (nPores_ > 2) logOnly("    dPhiPer = ", r.dPhiPer, "\n");
+  say("    P / Y = ", r.deltaP / Y_, "\n");
+  if (nPores_ > 2) say("    Pper / Y = ", r.deltaPper / Y_, "\n");
+  say("    tau / Y = ", r.tauInfty / Y_, "\n");
+
+  // analytic moduli of this step (:643-696)
+  const long nX = nX_, nY = nY_;
+  const double Phi0 = r.Phi0, Phi = r.Phi;
+  const double phi = 3.1415926 * rad_ * rad_ / (dX_ * (nX - 1) * dY_ * (nY - 1));
+  const double Y = Y_, G0 = G0_, K0 = K0_;
+  if (NL <= 1) {
+    res->KphiExactElast = G0 / Phi0;
+    logOnly("    KphiExactElast = ", res->KphiExactElast, "\n");
+    res->an.Kphi = G0 / Phi0;
+    res->KdExactElast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactElast);
+    logOnly("    KdExactElast = ", res->KdExactElast, "\n");
+    res->an.Kd = G0 / Phi0;
+  } else if (step == 0) {
+    res->KphiExactPlast = G0 / Phi / std::exp(std::fabs(r.deltaP) / Y - 1.0) /
+                          (1.0 + r.tauInfty * r.tauInfty / Y / Y);
+    logOnly("    KphiExactPlast = ", res->KphiExactPlast, "\n");
+    res->an.Kphi = res->KphiExactPlast;
+    res->KdExactPlast = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlast);
+    logOnly("    KdExactPlast = ", res->KdExactPlast, "\n");
+    res->an.Kd = res->KdExactPlast;
+    res->KphiExactPlastPer = G0 / Phi0 / std::exp(std::fabs(r.deltaPper) / Y - 1.0) /
+                             (1.0 + r.tauInfty * r.tauInfty / Y / Y);
+    logOnly("    KphiExactPlastPer = ", res->KphiExactPlastPer, "\n");
+    res->an_per.Kphi = res->KphiExactPlastPer;
+    res->KdExactPlastPer = (1.0 - phi) / (1.0 / K0 + 1.0 / res->KphiExactPlastPer);
+    logOnly("    KdExactPlastPer = ", res->KdExactPlastPer, "\n");
+    res->an_per.Kd = res->KdExactPlastPer;
+  }
+  if (NL == 3 && step == 0) {
+    res->GexactElast = G0 / (1.0 + Phi0);
+    logOnly("    GexactElast = ", res->GexactElast, "\n");
+    res->GexactPlast = G0 / (1.0 + Phi * std::exp(std::fabs(r.deltaP) / Y - 1.0));
+    logOnly("    GexactPlast = ", res->GexactPlast, "\n");
+    res->an.G = res->GexactPlast;
+    res->GexactPlastPer = G0 / (1.0 + Phi0 * std::exp(std::fabs(r.deltaPper) / Y - 1.0));
+    logOnly("    GexactPlastPer = ", res->GexactPlastPer, "\n");
+    res->an_per.G = res->GexactPlastPer;
+  }
+  records_.push_back(r);
+  recordStep_.push_back(step);
+  if (++tstep_ >= plan_[task_].nT) {
+    tstep_ = 0;
+    task_++;
+  }
+}
+
+// Moduli getters (EffPlast2D.cu:1412-1539) or the tensor summary of
+// ComputeEffDamping (:245-299), then the closing output.
+void Solver::jobFinish(ep2d_result* out) {
+  const int NL = opt_.nTasks;
+  const long T = opt_.nTimeSteps;
+  ep2d_result& res = res_;
+  if (jobKind_ == 0) {
+    const ep2d_record r0 = records_[T - 1];
+    if (NL == 1) {   // calcBulkModuli_PureElast, :1412-1481
+      res.num.Kphi = r0.deltaP / r0.dPhi;
+      logOnly("    ==============\n", "    Kphi = ", res.num.Kphi, "\n");
+      res.num_per.Kphi = r0.deltaPper / r0.dPhiPer;
+      logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
+      const double pInc = -0.33333333 * (r0.sigma[0] + r0.sigma[1] + r0.sigma[2]);
+      const double epsInc = r0.epsilon[0] + r0.epsilon[1];
+      res.num.Kd = -pInc / epsInc;
+      logOnly("    Kd = ", res.num.Kd, "\n");
+      const double pIncPer = -0.33333333 * (r0.sigmaPer[0] + r0.sigmaPer[1] + r0.sigmaPer[2]);
+      const double epsIncPer = r0.epsilonPer[0] + r0.epsilonPer[1];
+      res.num_per.Kd = -pIncPer / epsIncPer;
+      logOnly("    KdPer = ", res.num_per.Kd, "\n");
+    } else {         // calcBulkModuli_ElastPlast, :1429-1514
+      const ep2d_record r1 = records_[T];
+      res.num.Kphi = (r1.deltaP - r0.deltaP) / (r1.dPhi - r0.dPhi);
+      logOnly("==============\n", "    Kphi = ", res.num.Kphi, "\n");
+      res.num_per.Kphi = (r1.deltaPper - r0.deltaPper) / (r1.dPhiPer - r0.dPhiPer);
+      logOnly("    KphiPer = ", res.num_per.Kphi, "\n");
+      const double pInc = -0.33333333 * (r1.sigma[0] + r1.sigma[1] + r1.sigma[2] - r0.sigma[0] - r0.sigma[1] - r0.sigma[2]);
+      const double epsInc = r1.epsilon[0] + r1.epsilon[1] - r0.epsilon[0] - r0.epsilon[1];
+      res.num.Kd = -pInc / epsInc;
+      logOnly("    Kd = ", res.num.Kd, "\n");
+      const double pIncPer = -0.33333333 * (r1.sigmaPer[0] + r1.sigmaPer[1] + r1.sigmaPer[2] - r0.sigmaPer[0] - r0.sigmaPer[1] - r0.sigmaPer[2]);
+      const double epsIncPer = r1.epsilonPer[0] + r1.epsilonPer[1] - r0.epsilonPer[0] - r0.epsilonPer[1];
+      res.num_per.Kd = -pIncPer / epsIncPer;
+      logOnly("    KdPer = ", res.num_per.Kd, "\n");
+    }
+    if (NL == 3) {   // calcShearModulus, :1446-1539
+      const ep2d_record r1 = records_[T];
+      const ep2d_record r2 = records_[T + 1];
+      res.num.G = 0.5 * (r2.sigma[0] - r1.sigma[0] - r2.sigma[1] + r1.sigma[1]) /
+                  (r2.epsilon[0] - r1.epsilon[0] - r2.epsilon[1] + r1.epsilon[1]);
+      logOnly("==============\n", "    G = ", res.num.G, "\n");
+      res.num_per.G = 0.5 * (r2.sigmaPer[0] - r1.sigmaPer[0] - r2.sigmaPer[1] + r1.sigmaPer[1]) /
+                      (r2.epsilonPer[0] - r1.epsilonPer[0] - r2.epsilonPer[1] + r1.epsilonPer[1]);
+      logOnly("    Gper = ", res.num_per.G, "\n");
+      res.Gxy = 0.5 * (r2.sigmaXYclean - r1.sigmaXYclean) / (r2.epsilon[2] - r1.epsilon[2]);
+      res.GxyPer = 0.5 * (r2.sigmaPer[2] - r1.sigmaPer[2]) / (r2.epsilonPer[2] - r1.epsilonPer[2]);
+    }
+    printEffectiveModuli(res);
+    printWarnings();
+  } else {
+    auto printTensors = [&](const char* title, auto get, int n) {
+      say("\n", title, ":\n", "0) ");
+      for (int j = 0; j < n; j++) say(get(records_[T - 1])[j], " ");
+      say("\n");
+      for (long i = 1; i < NL; i++) {
+        say(i, ") ");
+        for (int j = 0; j < n; j++) say(get(records_[T - 1 + i])[j], " ");
+        say("\n");
+      }
+    };
+    printTensors("Strain tensors", [](const ep2d_record& r) { return r.epsilon; }, 3);
+    printTensors("Strain tensors periodic", [](const ep2d_record& r) { return r.epsilonPer; }, 3);
+    printTensors("Stress tensors", [](const ep2d_record& r) { return r.sigma; }, 4);
+    printTensors("Stress tensors periodic", [](const ep2d_record& r) { return r.sigmaPer; }, 4);
+  }
   const auto t1 = std::chrono::steady_clock::now();
-  res.seconds = std::chrono::duration<double>(t1 - t0).count();
+  res.seconds = std::chrono::duration<double>(t1 - jobT0_).count();
   res.nRecords = (int)records_.size();
   res.totalIters = totalIters_;
   res.converged = allConverged_ ? 1 : 0;
@@ -737,48 +830,44 @@ void Solver::computeEffModuli(double initLoadValue, const double loadType[3], ep
   if (out) *out = res;
 }
 
-// EffPlast2D::ComputeEffDamping (EffPlast2D.cu:230-310).
+// One RVE on its own: the iteration loop of EffPlast2D.cu:389-428 with a
+// residual check every checkInterval iterations.
+void Solver::runJob(ep2d_result* out) {
+  while (!jobDone()) {
+    tsBegin();
+    while (!tsOver()) {
+      const long c = tsChunk();
+      iterate(c);
+      tsDone_ += c;
+      if (tsDone_ % opt_.checkInterval == 0) {
+        tsCheckLaunch();
+        EP2D_HIP(hipStreamSynchronize(stream_));
+        tsCheckFinish();
+      }
+    }
+    tsFinish();
+  }
+  jobFinish(out);
+}
+
+void Solver::computeEffModuli(double initLoadValue, const double loadType[3], ep2d_result* out) {
+  jobBegin(0, initLoadValue, loadType, 0.0);
+  runJob(out);
+}
+
 void Solver::computeEffDamping(double initLoadValue, const double loadType[3], double loadValue,
                                ep2d_result* out) {
-  const int NL = opt_.nTasks;
-  if (NL < 4) throw ArgError("ERROR:  Wrong number of loads in ComputeEffModuli()!");
-  EP2D_HIP(hipSetDevice(device_));
-  const auto t0 = std::chrono::steady_clock::now();
-  ep2d_result res;
-  std::memset(&res, 0, sizeof(res));
-  records_.clear();
-  recordStep_.clear();
-  totalIters_ = 0;
-  allConverged_ = true;
-  std::memcpy(loadType_, loadType, sizeof(loadType_));
-  printCalculationType();
-  const long T = opt_.nTimeSteps;
-  computeEffParams(0, initLoadValue, loadType, T, &res);
-  for (long i = 1; i < NL; i++) {
-    const double v = loadValue * (std::sin(i * 2.0 * 3.1415926 / (NL - 1)) -
-                                  std::sin((i - 1) * 2.0 * 3.1415926 / (NL - 1)));
-    computeEffParams((int)i, v, loadType, 1, &res);
+  jobBegin(1, initLoadValue, loadType, loadValue);
+  runJob(out);
+}
+
+// Iterations executed by a batched launch on this RVE's buffers.
+void Solver::noteBatchIterations(long n) {
+  if (mode_ != EP2D_MODE_PLAIN) {
+    if (n & 1) cur_ ^= 1;
+    if (n) stressStale_ = true;
   }
-  auto printTensors = [&](const char* title, auto get, int n) {
-    say("\n", title, ":\n", "0) ");
-    for (int j = 0; j < n; j++) say(get(records_[T - 1])[j], " ");
-    say("\n");
-    for (long i = 1; i < NL; i++) {
-      say(i, ") ");
-      for (int j = 0; j < n; j++) say(get(records_[T - 1 + i])[j], " ");
-      say("\n");
-    }
-  };
-  printTensors("Strain tensors", [](const ep2d_record& r) { return r.epsilon; }, 3);
-  printTensors("Strain tensors periodic", [](const ep2d_record& r) { return r.epsilonPer; }, 3);
-  printTensors("Stress tensors", [](const ep2d_record& r) { return r.sigma; }, 4);
-  printTensors("Stress tensors periodic", [](const ep2d_record& r) { return r.sigmaPer; }, 4);
-  const auto t1 = std::chrono::steady_clock::now();
-  res.seconds = std::chrono::duration<double>(t1 - t0).count();
-  res.nRecords = (int)records_.size();
-  res.totalIters = totalIters_;
-  res.converged = allConverged_ ? 1 : 0;
-  if (out) *out = res;
+  tsDone_ += n;
 }
 
 // ---- fields -------------------------------------------------------------------

@@ -1,6 +1,7 @@
 // ep2d_solver.h -- host driver of one RVE (the body of an ep2d_handle).
 #pragma once
 
+#include <chrono>
 #include <fstream>
 #include <string>
 #include <vector>
@@ -23,7 +24,9 @@ void derive_params(const ep2d_physics& ph, double pa[EP2D_NPARS], int* nX);
 
 class Solver {
  public:
-  Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt);
+  // stream: run on this stream (owned by the caller) instead of a new one
+  Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt,
+         hipStream_t stream = nullptr);
   ~Solver();
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -33,6 +36,29 @@ class Solver {
   void computeEffDamping(double initLoadValue, const double loadType[3], double loadValue,
                          ep2d_result* out);
   const std::vector<ep2d_record>& records() const { return records_; }
+
+  // The same sequence as a resumable job (kind 0: moduli, 1: damping), for
+  // drivers that run many RVEs in one launch.
+  void jobBegin(int kind, double initLoadValue, const double loadType[3], double loadValue);
+  bool jobDone() const { return task_ >= plan_.size(); }
+  void tsBegin();                 // start a time step (affine increment)
+  long tsChunk() const;           // iterations to the next check or the cap
+  bool tsOver() const;            // converged or iteration cap reached
+  long tsDoneIters() const { return tsDone_; }
+  bool tsAtCheck() const { return tsDone_ > 0 && tsDone_ % opt_.checkInterval == 0; }
+  void tsCheckLaunch();           // residual reduction + async copy of its result
+  bool tsCheckFinish();           // after the stream is synchronised
+  void tsFinish();                // averages, boundary, hole area, analytic moduli
+  void jobFinish(ep2d_result* out);
+  void runJob(ep2d_result* out);
+  void noteBatchIterations(long n);
+  const RveDev& dev() const { return r_; }
+  const FusedPlan& fusedPlan() const { return fplan_; }
+  int cur() const { return cur_; }
+  int mode() const { return mode_; }
+  bool plastic() const { return opt_.nTasks > 1; }
+  int nX() const { return nX_; }
+  hipStream_t stream() const { return stream_; }
 
   // step-level pieces
   void iterate(long n);
@@ -54,9 +80,8 @@ class Solver {
   size_t deviceBytes() const { return devBytes_; }
 
  private:
-  // ComputeEffParams (EffPlast2D.cu:311-698)
-  void computeEffParams(int step, double loadStepValue, const double loadType[3], long nT,
-                        ep2d_result* res);
+  void init(const double pa[EP2D_NPARS], hipStream_t stream);
+  void release();
   void ensureStress();
   void launchIteration();
   void resolveTiming(bool all);
@@ -85,10 +110,11 @@ class Solver {
 
   // device state
   hipStream_t stream_ = nullptr;
+  bool ownStream_ = true;
   int device_ = 0;
   RveDev r_{};
   StressDev f_{};
-  FusedPlan plan_{};
+  FusedPlan fplan_{};
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;
@@ -117,6 +143,23 @@ class Solver {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_, freeEvents_;
   double timedMs_ = 0.0;
   long timedSamples_ = 0;
+
+  // job state
+  struct TaskPlan {
+    double load;
+    double lt[3];
+    long nT;
+  };
+  std::vector<TaskPlan> plan_;
+  size_t task_ = 0;
+  long tstep_ = 0;
+  long tsDone_ = 0;
+  double tsError_ = 0.0;
+  bool tsConv_ = false;
+  int jobKind_ = 0;
+  ep2d_record rec_{};
+  ep2d_result res_{};
+  std::chrono::steady_clock::time_point jobT0_;
 
   // results
   double curEffStrain_[3] = {0, 0, 0};

@@ -221,6 +221,77 @@ int ep2d_sync(ep2d_handle* h);
 /* Device bytes held by the handle. */
 size_t ep2d_device_bytes(ep2d_handle* h);
 
+/* ---- many RVEs on one GPU: batched launches with admission and backfill ----
+ * Extension, no single counterpart in the reference (which runs one RVE per
+ * process, eff_props_2D.m:44-46).  Each request is one ComputeEffModuli
+ * (nTasks <= 3) or ComputeEffDamping (nTasks > 3) of its own RVE, with the
+ * arithmetic of the single-RVE path: a result does not depend on what else is
+ * in the batch.  Every launch advances all admitted RVEs by one iteration;
+ * every checkInterval iterations each RVE's residual is checked, converged
+ * load steps are post-processed, finished RVEs retire and queued requests are
+ * admitted while the device memory budget allows. */
+typedef struct ep2d_batch_options {
+  int mode;              /* EP2D_MODE_FP64 or EP2D_MODE_SPLIT                 */
+  long checkInterval;    /* residual check interval, common to all RVEs        */
+  int tileRows;          /* fused tile height (0: default)                     */
+  int device;
+  size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
+  int maxActive;         /* cap on admitted RVEs (0: none)                     */
+  int useGraphs;         /* capture iteration chunks in HIP graphs             */
+  long timingEvery;      /* time every n-th launch with dispatch events (0: off) */
+} ep2d_batch_options;
+
+typedef struct ep2d_request {
+  double pa[EP2D_NPARS]; /* parameter vector of this RVE                       */
+  int nX;                /* grid (nY = nX)                                     */
+  int nTasks;            /* NL                                                 */
+  long nIter;
+  double eIter;
+  int nTimeSteps;
+  double initLoadValue;
+  double loadType[3];
+  double loadValue;      /* damping amplitude (nTasks > 3)                     */
+  int useShearLoadType;
+  double shearLoadType[3];
+  long long tag;         /* caller's identifier, returned with the result      */
+} ep2d_request;
+
+typedef struct ep2d_batch_result {
+  long long id, tag;
+  int status;            /* EP2D_OK or the error of this request               */
+  ep2d_result result;
+  double tSubmit, tAdmit, tDone;  /* seconds on the batch clock               */
+  char message[160];     /* error text when status != EP2D_OK                  */
+} ep2d_batch_result;
+
+typedef struct ep2d_batch_stats {
+  long long launches, chunks, tilesLaunched, submitted, finished, failed;
+  long long cellIterations;       /* sum over launches of RVE cells advanced  */
+  double bytesAlgorithmic;        /* U and V read and written, all launches   */
+  double timedMs, timedBytes;     /* sampled launches: duration and bytes     */
+  long long timedLaunches;
+  int active, queued;
+  size_t deviceBytes;             /* device memory held by admitted RVEs      */
+} ep2d_batch_stats;
+
+typedef struct ep2d_batch ep2d_batch;
+
+void ep2d_batch_default_options(ep2d_batch_options* opt);
+int ep2d_batch_create(const ep2d_batch_options* opt, ep2d_batch** out);
+void ep2d_batch_destroy(ep2d_batch* b);
+/* Queue a request; *id receives its identifier.  Validation errors return
+ * immediately; device errors are reported in the request's result. */
+int ep2d_batch_submit(ep2d_batch* b, const ep2d_request* rq, long long* id);
+/* One chunk: admit, run up to checkInterval iterations of every admitted RVE,
+ * check, post-process, retire.  Returns the number admitted and queued. */
+int ep2d_batch_step(ep2d_batch* b, int* nActive, int* nQueued);
+/* Step until nothing is queued or admitted. */
+int ep2d_batch_run(ep2d_batch* b);
+/* Move up to capacity finished results into out. */
+int ep2d_batch_poll(ep2d_batch* b, ep2d_batch_result* out, int capacity, int* count);
+double ep2d_batch_clock(ep2d_batch* b);
+int ep2d_batch_stats_get(ep2d_batch* b, ep2d_batch_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -139,3 +139,23 @@ def test_oracle_is_not_imported_by_the_package():
         if fn.endswith((".cpp", ".hip", ".h", "Makefile")):
             assert "ep2d_oracle" not in open(os.path.join(pkg_dir, "csrc", fn)).read(), fn
     assert ora.LIB_PATH not in [getattr(m, "__file__", "") for m in sys.modules.values()]
+
+
+def test_batch_options_and_errors():
+    L = _lib.lib()
+    o = _lib.BatchOptions()
+    L.ep2d_batch_default_options(C.byref(o))
+    assert o.mode == _lib.MODES["split"] and o.useGraphs == 1 and o.checkInterval > 0
+    h = C.c_void_p()
+    o.mode = _lib.MODES["plain"]
+    assert L.ep2d_batch_create(C.byref(o), C.byref(h)) == -1 and not h.value
+    assert b"mode" in L.ep2d_last_error()
+    L.ep2d_batch_default_options(C.byref(o))
+    o.tileRows = 7
+    assert L.ep2d_batch_create(C.byref(o), C.byref(h)) == -1
+    if ep.device_count() == 0:
+        L.ep2d_batch_default_options(C.byref(o))
+        assert L.ep2d_batch_create(C.byref(o), C.byref(h)) in (-1, -2) and not h.value
+    assert L.ep2d_batch_step(None, None, None) == -1
+    assert L.ep2d_batch_submit(None, None, None) == -1
+    L.ep2d_batch_destroy(None)

@@ -1,0 +1,111 @@
+"""GPU: the batched multi-RVE engine against the single-RVE path.
+
+An RVE's arithmetic in the batched launch is the single-RVE kernel's, so its
+result must not depend on the other RVEs it shares launches with: moduli,
+per-step records and iteration counts are compared for exact equality with
+the single-RVE solve of the same request and check interval ("alone vs in the
+mix"), with and without graph-captured chunks, and under an admission budget
+that forces retirement and backfill.  The fp64 batch is also checked against
+the CPU oracle (relative 1e-8 on the moduli, iteration counts equal).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from effplast2d_amd import Batch, EffPlast2D
+from tests.pairs import make_pa
+
+pytestmark = pytest.mark.gpu
+
+LT = [1.0, 1.0, 0.0]
+
+# (nX, nPores, porosity, Y, nTasks, initLoad)
+REQS = [
+    (64, 1, 0.01, 2.5e-3, 3, -3e-3),
+    (96, 2, 0.03, 3e-3, 3, -2e-3),
+    (128, 1, 0.005, 2.0e-3, 1, -1e-4),
+    (70, 3, 0.04, 4e-3, 2, -2.5e-3),
+]
+K = 200
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _single(req, mode):
+    nX, nP, phi, Y, nl, load = req
+    pa = make_pa(nX, nPores=nP, porosity=phi, Y=Y)
+    with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=K,
+                    mode=mode) as g:
+        res = g.ComputeEffModuli(load, LT)
+        return res
+
+
+def _batch(reqs, mode, **kw):
+    out = {}
+    with Batch(mode=mode, checkInterval=K, **kw) as b:
+        for t, (nX, nP, phi, Y, nl, load) in enumerate(reqs):
+            pa = make_pa(nX, nPores=nP, porosity=phi, Y=Y)
+            b.submit(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, initLoad=load, loadType=LT,
+                     tag=t)
+        b.run()
+        for r in b.poll():
+            out[r["tag"]] = r
+        st = b.stats()
+    return out, st
+
+
+def _same(a, b):
+    for grp in ["num", "an", "num_per"]:
+        for k in ["Kphi", "Kd", "G"]:
+            x, y = a[grp][k], b[grp][k]
+            assert (x == y) or (np.isnan(x) and np.isnan(y)), (grp, k, x, y)
+    assert a["totalIters"] == b["totalIters"]
+    assert a["nRecords"] == b["nRecords"]
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batch_equals_single(mode, graphs):
+    _note(f"batch mode={mode} graphs={graphs}")
+    got, st = _batch(REQS, mode, useGraphs=graphs)
+    assert st["finished"] == len(REQS) and st["failed"] == 0
+    for t, rq in enumerate(REQS):
+        assert got[t]["status"] == 0, got[t]["message"]
+        _same(got[t]["result"], _single(rq, mode))
+
+
+def test_alone_vs_in_the_mix():
+    _note("alone vs mix")
+    alone, _ = _batch(REQS[:1], "split")
+    mix, _ = _batch(REQS[::-1], "split")
+    _same(alone[0]["result"], mix[len(REQS) - 1]["result"])
+
+
+def test_budget_forces_backfill():
+    _note("budget backfill")
+    got, st = _batch(REQS, "fp64", maxActive=2)
+    assert st["finished"] == len(REQS)
+    ref, _ = _batch(REQS, "fp64")
+    for t in range(len(REQS)):
+        _same(got[t]["result"], ref[t]["result"])
+    # admission is FIFO: the third request is admitted only after a retirement
+    admits = sorted(got.values(), key=lambda r: r["tAdmit"])
+    assert admits[2]["tAdmit"] >= min(admits[0]["tDone"], admits[1]["tDone"])
+
+
+def test_failed_request_reports_and_others_finish():
+    _note("bad request")
+    with Batch(mode="fp64", checkInterval=K) as b:
+        pa = make_pa(64)
+        bad = pa.copy()
+        bad[3] = np.nan  # K0: the first residual check sees non-finite velocities
+        b.submit(pa, 64, nTasks=1, nIter=50_000, eIter=1e-8, initLoad=-1e-4, tag=0)
+        b.submit(bad, 64, nTasks=1, nIter=50_000, eIter=1e-8, initLoad=-1e-4, tag=1)
+        b.run()
+        res = {r["tag"]: r for r in b.poll()}
+    assert res[0]["status"] == 0
+    assert res[1]["status"] != 0 and res[1]["message"]
