@@ -1,23 +1,37 @@
 #!/usr/bin/env python3
 """Benchmark: converged RVE solves/s of the EffPlast2D hot path on one MI355X.
 
-Workload (one "step" = one converged solve): the 1024x1024 elastoplastic RVE
-with all three moduli -- nGrid=32, Lx=Ly=20, one pore, porosity 0.01,
-Y = 2.5e-3*K0, K0=1, G0=0.5, rho0=1, CFL=0.5, preload initLoadValue=-3e-3
-along loadType [1,1,0], then the 0.5% hydrostatic increment and a pure xy
-shear increment (shear loadType [0,0,1]), nTimeSteps=1, eIter=1e-8,
-nIter=100000.  Each step creates the solver (pore masks, buffers) and runs
-ComputeEffModuli to convergence through the C ABI; inputs are generated on
-the device, nothing crosses PCIe but scalars and sampled boundary values.
+Headline workload (one "step" = one converged solve): the 1024x1024
+elastoplastic RVE with all three moduli -- nGrid=32, Lx=Ly=20, one pore,
+porosity 0.01, Y = 2.5e-3*K0, K0=1, G0=0.5, rho0=1, CFL=0.5, preload
+initLoadValue=-3e-3 along loadType [1,1,0], then the 0.5% hydrostatic
+increment and a pure xy shear increment (shear loadType [0,0,1]),
+nTimeSteps=1, eIter=1e-8, nIter=100000, residual checked every 500
+iterations.  The timed region submits K such requests to the batched engine
+in the FP32/FP64 split scheme (graph-captured chunks, tile height tuned on the
+running batch) and runs them to convergence; value = K / elapsed.  Inputs are
+generated on the device; nothing crosses PCIe but scalars, tables and sampled
+boundary values.
 
-Printed: ONE JSON line with value = solves/s of the FP64 fused scheme (the
-configuration's precision), the dominant kernel's live HIP-event timing
-against the HBM roofline, the other schemes (plain four-kernel FP64 and the
-FP32/FP64 split) as extra fields, and the CPU oracle's rate on this host.
+Beside it, in the same run:
+  * single-RVE solves of the same request: fused FP64, fused split, and the
+    plain four-kernel FP64 solver (the reference's structure);
+  * the batched FP64 engine on the same K requests;
+  * the batched sweep: 256 RVEs of seeded random size/porosity/pores/yield/
+    preload (sizes 128..1024), split scheme, one engine;
+  * serving: Poisson arrivals of moduli queries of sizes 256..2048 under an
+    HBM budget, p50/p99 latency and solves/s, beside the sequential plain FP64
+    solver's figures modelled from its measured per-iteration time;
+  * the CPU oracle on this host's cores on a bounded sample.
+
+Printed: ONE JSON line; value and roofline are the headline's, the roofline's
+kernel time is the live dispatch-event timing of the batched kernel.
 
 Other entry points:
   --probe      a fixed 200-iteration run of one scheme plus a copy kernel of
                known size (for rocprofv3 --pmc counter runs)
+  --sweep      fused-kernel duration per grid size, scheme and tile height
+  --config3    4096^2, 16 pores, porosity 0.05: split against FP64 moduli
   --summarize  turn rocprofv3 csv output into the short text summaries kept
                under profiles/
 """
@@ -112,6 +126,213 @@ def run_mode(pa, nx, mode, steps, warmup, check, timing_every):
     }
 
 
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return float("nan")
+    k = (len(xs) - 1) * q
+    lo, hi = int(k), min(int(k) + 1, len(xs) - 1)
+    return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
+
+
+def batch_stats_summary(st, mode):
+    """Kernel figures of a batched run from its sampled dispatch events."""
+    kms = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else float("nan")
+    gbs = st["timedBytes"] / st["timedLaunches"] / (kms * 1e-3) / 1e9 if st["timedLaunches"] \
+        else float("nan")
+    return {"kernel": f"k_fused_batch<{'float' if mode == 'split' else 'double'}, TY>",
+            "kernel_avg_us": kms * 1e3, "kernel_samples": st["timedLaunches"],
+            "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]
+            else float("nan"),
+            "achieved_GBs": gbs, "launches": st["launches"], "chunks": st["chunks"],
+            "tile_rows": st["tileRows"], "tile_us_by_rows": dict(zip((6, 8, 14, 30), st["tileUs"])),
+            "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"]}
+
+
+def run_batch(reqs, mode, check, timing_every=1, **opts):
+    """Submit every request at once and run the engine dry.  Returns results
+    (by tag), engine statistics and elapsed seconds."""
+    from effplast2d_amd import Batch
+    with Batch(mode=mode, checkInterval=check, timingEvery=timing_every, **opts) as b:
+        t0 = time.perf_counter()
+        for r in reqs:
+            b.submit(**r)
+        b.run()
+        dt = time.perf_counter() - t0
+        res = {r["tag"]: r for r in b.poll()}
+        st = b.stats()
+    bad = [r for r in res.values() if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"batched requests failed: {bad[0]['message']}")
+    return res, st, dt
+
+
+def headline_request(pa, nx, tag):
+    return dict(pa=pa, nX=nx, nTasks=3, nIter=100_000, eIter=1e-8, initLoad=LOAD["initLoad"],
+                loadType=LOAD["loadType"], shearLoadType=LOAD["shearLoadType"], tag=tag)
+
+
+def run_batched_headline(pa, nx, mode, steps, warmup, check):
+    if warmup:
+        run_batch([headline_request(pa, nx, k) for k in range(warmup)], mode, check)
+    res, st, dt = run_batch([headline_request(pa, nx, k) for k in range(steps)], mode, check)
+    lat = [r["tDone"] - r["tSubmit"] for r in res.values()]
+    tot_iters = sum(r["result"]["totalIters"] for r in res.values())
+    r0 = res[0]["result"]
+    log(f"[bench] batched {mode} x{steps}: {dt:.2f} s, {steps / dt:.2f} solves/s, "
+        f"tile rows {st['tileRows']} {['%.1f' % u for u in st['tileUs']]}")
+    out = {"mode": mode, "batched": steps, "solves_per_s": steps / dt, "elapsed_s": dt,
+           "ms_per_solve": 1e3 * dt / steps, "p50_ms": 1e3 * percentile(lat, 0.5),
+           "p99_ms": 1e3 * percentile(lat, 0.99), "iters_per_solve": tot_iters / steps,
+           "cell_iter_per_s": tot_iters * nx * nx / dt,
+           "moduli": {"Kphi": r0["num"]["Kphi"], "Kd": r0["num"]["Kd"], "Gxy": r0["Gxy"]},
+           "converged": all(bool(r["result"]["converged"]) for r in res.values())}
+    out.update(batch_stats_summary(st, mode))
+    return out
+
+
+def c4_requests(n, seed):
+    """Batched sweep: sizes uniform in {128,256,512,1024}, porosity U[0.005,0.05],
+    nPores in {1,2,3}, Y/K0 U[1e-3,5e-3], preload magnitude U[1e-3,5e-3] along a
+    random unit direction in (Exx, Eyy, Exy); nTasks=3."""
+    import numpy as np
+    from effplast2d_amd import derive_params
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(n):
+        N = int(rng.choice([128, 256, 512, 1024]))
+        phi = float(rng.uniform(0.005, 0.05))
+        npores = int(rng.integers(1, 4))
+        Y = float(rng.uniform(1e-3, 5e-3))
+        mag = float(rng.uniform(1e-3, 5e-3))
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
+                               G0=0.5, rho0=1.0, CFL=0.5)
+        assert pa[9] < 20.0 / (2 * npores), "pore radius guard"
+        reqs.append(dict(pa=pa, nX=nx, nTasks=3, nIter=100_000, eIter=1e-8, initLoad=-mag,
+                         loadType=[float(x) for x in u], tag=k))
+    return reqs
+
+
+def run_config4(n, seed, check):
+    reqs = c4_requests(n, seed)
+    res, st, dt = run_batch(reqs, "split", check)
+    lat = [r["tDone"] - r["tSubmit"] for r in res.values()]
+    cells = st["cellIterations"]
+    out = {"workload": f"{n} RVEs, sizes U{{128,256,512,1024}}, porosity U[0.005,0.05], "
+                       f"nPores U{{1,2,3}}, Y U[1e-3,5e-3], preload U[1e-3,5e-3] x random unit "
+                       f"(Exx,Eyy,Exy), nTasks=3, seed {seed}, split, one engine",
+           "solves": n, "elapsed_s": dt, "solves_per_s": n / dt, "cell_iter_per_s": cells / dt,
+           "hbm_frac_whole_run": st["bytesAlgorithmic"] / dt / 1e9 / HBM_PEAK_GBS,
+           "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
+           "converged": sum(bool(r["result"]["converged"]) for r in res.values())}
+    out.update(batch_stats_summary(st, "split"))
+    log(f"[bench] config4 {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
+        f"{cells / dt / 1e9:.1f} G cell-iter/s, kernel {out['achieved_GBs']:.0f} GB/s")
+    return out, res
+
+
+def c5_stream(rate, duration, seed):
+    import numpy as np
+    from effplast2d_amd import derive_params
+    rng = np.random.default_rng(seed)
+    t, reqs = 0.0, []
+    while True:
+        t += float(rng.exponential(1.0 / rate))
+        if t > duration:
+            break
+        N = int(rng.choice([256, 512, 1024, 2048], p=[0.4, 0.3, 0.2, 0.1]))
+        phi = float(rng.uniform(0.005, 0.05))
+        npores = int(rng.integers(1, 4))
+        Y = float(rng.uniform(1e-3, 5e-3))
+        mag = float(rng.uniform(1e-3, 5e-3))
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        nl = int(rng.integers(1, 4))
+        pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
+                               G0=0.5, rho0=1.0, CFL=0.5)
+        reqs.append((t, dict(pa=pa, nX=nx, nTasks=nl, nIter=100_000, eIter=1e-8, initLoad=-mag,
+                             loadType=[float(x) for x in u], tag=len(reqs))))
+    return reqs
+
+
+def plain_iteration_ms(nx, iters=200):
+    """Per-iteration time of the plain four-kernel FP64 solver at size nx
+    (sampled dispatch events, all four kernels)."""
+    from effplast2d_amd import EffPlast2D
+    pa, _ = make_pa(nx)
+    with EffPlast2D(pa, nx, nTasks=3, mode="plain") as g:
+        g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
+        g.iterate(20)
+        g.sync()
+        t0 = time.perf_counter()
+        g.iterate(iters)
+        g.sync()
+        return 1e3 * (time.perf_counter() - t0) / iters
+
+
+def run_config5(rate, duration, seed, check, budget_gib):
+    """Continuous batching under Poisson arrivals: requests are submitted at
+    their arrival times between chunks; each chunk admits what the HBM budget
+    allows, runs every admitted RVE to its next residual check, retires the
+    finished ones and backfills."""
+    from effplast2d_amd import Batch
+    stream = c5_stream(rate, duration, seed)
+    done = {}
+    with Batch(mode="split", checkInterval=check, timingEvery=1,
+               memoryBudget=int(budget_gib * (1 << 30))) as b:
+        t0 = b.clock()
+        i, peak = 0, 0
+        while i < len(stream) or len(done) < i:
+            now = b.clock() - t0
+            while i < len(stream) and stream[i][0] <= now:
+                b.submit(**stream[i][1])
+                i += 1
+            if len(done) == i:  # idle until the next arrival
+                time.sleep(max(0.0, stream[i][0] - (b.clock() - t0)))
+                continue
+            active, _ = b.step()
+            peak = max(peak, active)
+            for r in b.poll():
+                done[r["tag"]] = r
+        st = b.stats()
+    bad = [r for r in done.values() if r["status"] != 0]
+    if bad:
+        raise RuntimeError(f"served request failed: {bad[0]['message']}")
+    lat = [r["tDone"] - r["tSubmit"] for r in done.values()]
+    span = max(r["tDone"] for r in done.values()) - t0 - stream[0][0]
+    # sequential plain FP64 solver on the same stream: service time of each
+    # request = its iteration count x the plain solver's measured per-iteration
+    # time at its size; FIFO queue over the same arrival times
+    per_iter = {n: plain_iteration_ms(n) for n in sorted({rq["nX"] for _, rq in stream})}
+    t_free, lat_seq = 0.0, []
+    for (ta, rq) in stream:
+        svc = 1e-3 * per_iter[rq["nX"]] * done[rq["tag"]]["result"]["totalIters"]
+        start = max(ta, t_free)
+        t_free = start + svc
+        lat_seq.append(t_free - ta)
+    span_seq = t_free - stream[0][0]
+    out = {"workload": f"Poisson arrivals {rate}/s for {duration} s (seed {seed}), sizes "
+                       f"{{256,512,1024,2048}} w [.4,.3,.2,.1], nTasks U{{1,2,3}}, other "
+                       f"parameters as the batched sweep, split, HBM budget {budget_gib} GiB",
+           "requests": len(stream), "solves_per_s": len(stream) / span,
+           "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
+           "peak_active": peak, "cell_iter_per_s": st["cellIterations"] / span,
+           "sequential_plain_fp64_modelled": {
+               "solves_per_s": len(stream) / span_seq,
+               "p50_ms": 1e3 * percentile(lat_seq, 0.5), "p99_ms": 1e3 * percentile(lat_seq, 0.99),
+               "per_iteration_ms": per_iter,
+               "how": "each request's GPU iteration count x the plain FP64 solver's measured "
+                      "per-iteration time at its size, FIFO over the same arrivals"}}
+    out.update(batch_stats_summary(st, "split"))
+    log(f"[bench] serving: {len(stream)} requests, {out['solves_per_s']:.2f} solves/s, "
+        f"p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms (sequential plain FP64 "
+        f"{out['sequential_plain_fp64_modelled']['solves_per_s']:.2f} solves/s, "
+        f"p99 {out['sequential_plain_fp64_modelled']['p99_ms']:.0f} ms)")
+    return out
+
+
 def read_traffic(mode, nx):
     """Measured HBM bytes per launch of the dominant kernel, from the committed
     rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
@@ -150,22 +371,29 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib):
+def probe(mode, nx, iters, copy_mib, batch=0):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
-    preloaded RVE, then 5 copies of copy_mib MiB."""
+    preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
+    over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
+    copy_mib MiB."""
     import ctypes as C
 
     from effplast2d_amd import EffPlast2D
     from effplast2d_amd._lib import check, lib
     pa, _ = make_pa(nx)
-    with EffPlast2D(pa, nx, nTasks=3, mode=mode) as g:
-        g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
-        g.iterate(iters)
-        g.sync()
+    if batch > 0:
+        reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
+                for k in range(batch)]
+        run_batch(reqs, mode, 10 ** 9, timing_every=0, tileRows=8, useGraphs=False)
+    else:
+        with EffPlast2D(pa, nx, nTasks=3, mode=mode) as g:
+            g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
+            g.iterate(iters)
+            g.sync()
     ms = C.c_double()
     check(lib().ep2d_diag_copy(0, copy_mib << 20, 5, C.byref(ms)))
-    log(f"[probe] {mode} {nx}: {iters} iterations; copy {copy_mib} MiB: {ms.value:.3f} ms "
-        f"({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
+    log(f"[probe] {mode} {nx} batch {batch}: {iters} iterations; copy {copy_mib} MiB: "
+        f"{ms.value:.3f} ms ({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
 
 
 def sweep(sizes, modes, tys, iters):
@@ -263,27 +491,72 @@ def pmc_traffic(fetch_dir, write_dir, kernel_key, copy_mib, dest, command):
     print(text)
 
 
+def run_config3(check):
+    """4096^2, 16 pores (nPores=4), porosity 0.05, Y=2.5e-3, nTasks=3: the
+    split scheme's moduli (including the periodic internal-cell ones) against
+    an FP64 run of the same case."""
+    from effplast2d_amd import EffPlast2D, derive_params
+    pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=4096, nPores=4, porosity=0.05, Y=2.5e-3, K0=1.0,
+                           G0=0.5, rho0=1.0, CFL=0.5)
+    out = {}
+    for mode in ("split", "fp64"):
+        t0 = time.perf_counter()
+        with EffPlast2D(pa, nx, nTasks=3, nIter=200_000, eIter=1e-8, checkInterval=check,
+                        mode=mode) as g:
+            g.set_timing(64)
+            res = g.ComputeEffModuli(LOAD["initLoad"], LOAD["loadType"])
+            ms, n, name = g.timing()
+            recs = g.records()
+        dt = time.perf_counter() - t0
+        out[mode] = {"seconds": dt, "kernel_avg_us": ms * 1e3, "kernel": name,
+                     "iters": [int(r["iters"]) for r in recs],
+                     "num": res["num"], "num_per": res["num_per"], "Gxy": res["Gxy"],
+                     "GxyPer": res["GxyPer"]}
+        log(f"[bench] config3 {mode}: {dt:.1f} s, iters {out[mode]['iters']}")
+    err = {}
+    for grp in ("num", "num_per"):
+        for k in ("Kphi", "Kd", "G"):
+            a, b = out["split"][grp][k], out["fp64"][grp][k]
+            err[f"{grp}.{k}"] = abs(a - b) / abs(b) if b else None
+    for k in ("Gxy", "GxyPer"):
+        a, b = out["split"][k], out["fp64"][k]
+        err[k] = abs(a - b) / abs(b) if b else None
+    out["split_rel_err_vs_fp64"] = err
+    print(json.dumps({"config3": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=8, help="requests in the timed batch")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="fp64", choices=["fp64", "split", "plain"])
+    ap.add_argument("--mode", default="split", choices=["fp64", "split"],
+                    help="scheme of the batched headline")
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--check-interval", type=int, default=500)
     ap.add_argument("--timing-every", type=int, default=16)
-    ap.add_argument("--no-modes", action="store_true", help="skip the other schemes")
-    ap.add_argument("--mode-steps", type=int, default=2)
+    ap.add_argument("--no-legs", action="store_true", help="headline and CPU baseline only")
+    ap.add_argument("--single-steps", type=int, default=2)
+    ap.add_argument("--c4-rves", type=int, default=256)
+    ap.add_argument("--c4-seed", type=int, default=2024)
+    ap.add_argument("--serve-rate", type=float, default=3.0)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-seed", type=int, default=7)
+    ap.add_argument("--serve-budget-gib", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--probe-mode", default="fp64", choices=["fp64", "split", "plain"])
     ap.add_argument("--probe-iters", type=int, default=200)
+    ap.add_argument("--probe-batch", type=int, default=0,
+                    help="probe the batched kernel with this many headline RVEs")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--sweep-sizes", default="256,1024,4096")
     ap.add_argument("--sweep-iters", type=int, default=300)
+    ap.add_argument("--config3", action="store_true")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None)
     ap.add_argument("--pmc-kernel", default="k_fused<double")
     ap.add_argument("--pmc-command", default="")
@@ -302,11 +575,14 @@ def main():
               "part of this build", file=sys.stderr)
         return 2
     if a.probe:
-        probe(a.mode, a.nx, a.probe_iters, a.copy_mib)
+        probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch)
         return 0
     if a.sweep:
         sweep([int(x) for x in a.sweep_sizes.split(",")], ["fp64", "split"], [6, 8, 14, 30],
               a.sweep_iters)
+        return 0
+    if a.config3:
+        run_config3(a.check_interval)
         return 0
 
     from effplast2d_amd import device_count
@@ -314,54 +590,62 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     pa, nx = make_pa(a.nx)
-    main_run = run_mode(pa, nx, a.mode, a.steps, a.warmup, a.check_interval, a.timing_every)
-    modes = {}
-    if not a.no_modes:
+    head = run_batched_headline(pa, nx, a.mode, a.steps, a.warmup, a.check_interval)
+    legs = {}
+    if not a.no_legs:
         for m in ("fp64", "split", "plain"):
-            if m == a.mode:
-                continue
-            r = run_mode(pa, nx, m, a.mode_steps, 0, a.check_interval, a.timing_every)
-            ref = main_run["moduli"]
-            r["moduli_rel_diff_vs_" + a.mode] = {
-                k: abs(r["moduli"][k] - ref[k]) / abs(ref[k]) if ref[k] else None
-                for k in ("Kphi", "Kd", "Gxy")}
-            modes[m] = r
-    frac = main_run["achieved_GBs"] / HBM_PEAK_GBS
-    traffic = read_traffic(a.mode, nx)
+            legs[f"single_{m}"] = run_mode(pa, nx, m, a.single_steps, 1 if m != "plain" else 0,
+                                           a.check_interval, a.timing_every)
+        other = "fp64" if a.mode == "split" else "split"
+        legs[f"batched_{other}"] = run_batched_headline(pa, nx, other, a.steps, 0,
+                                                        a.check_interval)
+        ref = legs["single_fp64"]["moduli"]
+        legs["moduli_rel_diff_vs_single_fp64"] = {
+            name: {k: abs(r["moduli"][k] - ref[k]) / abs(ref[k]) for k in ("Kphi", "Kd", "Gxy")}
+            for name, r in [("batched_" + a.mode, head)] + list(legs.items())
+            if isinstance(r, dict) and "moduli" in r}
+        legs["speedup_vs_single_plain_fp64"] = head["solves_per_s"] / \
+            legs["single_plain"]["solves_per_s"]
+        legs["config4_batched_sweep"], _ = run_config4(a.c4_rves, a.c4_seed, a.check_interval)
+        legs["config5_serving"] = run_config5(a.serve_rate, a.serve_seconds, a.serve_seed,
+                                              a.check_interval, a.serve_budget_gib)
+    frac = head["achieved_GBs"] / HBM_PEAK_GBS
+    traffic = read_traffic(f"batch_{a.mode}", nx)
     out = {
         "metric": METRIC,
-        "value": main_run["solves_per_s"],
+        "value": head["solves_per_s"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": main_run["ms_per_solve"],
+        "ms_per_step": head["ms_per_solve"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.mode != "split" else "f32/f64-split",
+        "dtype": "f32/f64-split" if a.mode == "split" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"1024x1024 elastoplastic RVE, nTasks=3 (preload [1,1,0] "
-                               f"-3e-3, hydrostatic +0.5%, xy shear +0.5%), one pore phi=0.01, "
-                               f"Y=2.5e-3, eIter=1e-8, scheme={a.mode}",
-                   "nx": nx, "nTasks": 3, "check_interval": a.check_interval,
-                   "eIter": 1e-8, "nIter": 100000, "G0": 0.5, "shear_load_type": [0, 0, 1]},
-        "roofline": {"bound": "hbm", "achieved": main_run["achieved_GBs"], "peak": HBM_PEAK_GBS,
+        "config": {"workload": f"1024x1024 elastoplastic RVE, nTasks=3 (preload [1,1,0] -3e-3, "
+                               f"hydrostatic +0.5%, xy shear +0.5%), one pore phi=0.01, "
+                               f"Y=2.5e-3, G0=0.5, eIter=1e-8; {a.steps} requests batched, "
+                               f"scheme={a.mode}",
+                   "nx": nx, "nTasks": 3, "check_interval": a.check_interval, "eIter": 1e-8,
+                   "nIter": 100000, "batched_requests": a.steps,
+                   "shear_load_type": [0, 0, 1]},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": frac, "traffic": traffic,
-                     "kernel": main_run["kernel"], "kernel_avg_us": main_run["kernel_avg_us"],
-                     "bytes_per_launch": main_run["bytes_per_launch"],
-                     "kernel_samples": main_run["kernel_samples"]},
-        "cell_iter_per_s": main_run["cell_iter_per_s"],
-        "p50_ms": main_run["p50_ms"],
-        "p99_ms": main_run["p99_ms"],
-        "iters_per_solve": main_run["iters_per_solve"],
-        "iters_by_task": main_run["iters_by_task"],
-        "moduli": main_run["moduli"],
-        "converged": main_run["converged"],
-        "modes": modes,
+                     "kernel": head["kernel"], "kernel_avg_us": head["kernel_avg_us"],
+                     "bytes_per_launch": head["bytes_per_launch"],
+                     "kernel_samples": head["kernel_samples"], "tile_rows": head["tile_rows"]},
+        "cell_iter_per_s": head["cell_iter_per_s"],
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
+        "iters_per_solve": head["iters_per_solve"],
+        "moduli": head["moduli"],
+        "converged": head["converged"],
+        "legs": legs,
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(pa, nx, main_run["iters_per_solve"], a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(pa, nx, head["iters_per_solve"], a.cpu_seconds)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -98,7 +98,8 @@ class Request(C.Structure):
                 ("nIter", C.c_long), ("eIter", C.c_double), ("nTimeSteps", C.c_int),
                 ("initLoadValue", C.c_double), ("loadType", C.c_double * 3),
                 ("loadValue", C.c_double), ("useShearLoadType", C.c_int),
-                ("shearLoadType", C.c_double * 3), ("tag", C.c_longlong)]
+                ("shearLoadType", C.c_double * 3), ("checkInterval", C.c_long),
+                ("tag", C.c_longlong)]
 
 
 class BatchResult(C.Structure):
@@ -118,10 +119,13 @@ class BatchStats(C.Structure):
                                             "finished", "failed", "cellIterations")] + \
                [("bytesAlgorithmic", C.c_double), ("timedMs", C.c_double),
                 ("timedBytes", C.c_double), ("timedLaunches", C.c_longlong),
-                ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t)]
+                ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t),
+                ("tileRows", C.c_int), ("tunings", C.c_int), ("tileUs", C.c_double * 4)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["tileUs"] = list(self.tileUs)
+        return d
 
 
 class Ep2dError(RuntimeError):

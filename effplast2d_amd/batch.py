@@ -53,7 +53,8 @@ class Batch:
         self.close()
 
     def submit(self, pa, nX, nTasks=3, nIter=100_000, eIter=1e-8, nTimeSteps=1, initLoad=-3e-3,
-               loadType=(1.0, 1.0, 0.0), loadValue=0.0, shearLoadType=None, tag=0):
+               loadType=(1.0, 1.0, 0.0), loadValue=0.0, shearLoadType=None, checkInterval=0,
+               tag=0):
         rq = L.Request()
         pa = np.ascontiguousarray(pa, dtype=np.float64)
         for k in range(L.NPARS):
@@ -66,6 +67,7 @@ class Batch:
             rq.useShearLoadType = 1
             for k in range(3):
                 rq.shearLoadType[k] = float(shearLoadType[k])
+        rq.checkInterval = int(checkInterval)
         rq.tag = int(tag)
         rid = C.c_longlong()
         check(lib().ep2d_batch_submit(self._h, C.byref(rq), C.byref(rid)))

@@ -46,8 +46,7 @@ Batch::~Batch() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   active_.clear();
-  if (gexec_) (void)hipGraphExecDestroy(gexec_);
-  if (graph_) (void)hipGraphDestroy(graph_);
+  dropGraphs();
   for (auto& e : evPending_) {
     (void)hipEventDestroy(e.a);
     (void)hipEventDestroy(e.b);
@@ -71,6 +70,7 @@ long long Batch::submit(const ep2d_request& rq) {
   if (rq.nX < 8) throw ArgError("request: nX must be >= 8");
   if (rq.nTasks < 1 || rq.nIter < 1 || rq.nTimeSteps < 1)
     throw ArgError("request: nTasks, nIter and nTimeSteps must be >= 1");
+  if (rq.checkInterval < 0) throw ArgError("request: checkInterval must be >= 0");
   queue_.push_back({nextId_, rq, clock()});
   st_.submitted++;
   return nextId_++;
@@ -101,7 +101,7 @@ void Batch::admit() {
     o.nTasks = jb.rq.nTasks;
     o.nIter = jb.rq.nIter;
     o.eIter = jb.rq.eIter;
-    o.checkInterval = opt_.checkInterval;
+    o.checkInterval = jb.rq.checkInterval > 0 ? jb.rq.checkInterval : opt_.checkInterval;
     o.mode = opt_.mode;
     o.device = device_;
     o.nTimeSteps = jb.rq.nTimeSteps;
@@ -164,6 +164,7 @@ void Batch::finish(Slot& sl) {
 void Batch::ensureCapacity(size_t nRves, size_t nTiles) {
   if (nRves > capRves_) {
     const size_t cap = std::max<size_t>(nRves, 2 * capRves_);
+    dropGraphs();
     if (dRves_) EP2D_HIP(hipFree(dRves_));
     if (hRves_) EP2D_HIP(hipHostFree(hRves_));
     dRves_ = nullptr;
@@ -171,10 +172,10 @@ void Batch::ensureCapacity(size_t nRves, size_t nTiles) {
     EP2D_HIP(hipMalloc((void**)&dRves_, cap * sizeof(RveDev)));
     EP2D_HIP(hipHostMalloc((void**)&hRves_, cap * sizeof(RveDev), hipHostMallocDefault));
     capRves_ = cap;
-    gLen_ = -1;
   }
   if (nTiles > capTiles_) {
     const size_t cap = std::max<size_t>(nTiles, 2 * capTiles_);
+    dropGraphs();
     if (dTiles_) EP2D_HIP(hipFree(dTiles_));
     if (hTiles_) EP2D_HIP(hipHostFree(hTiles_));
     dTiles_ = nullptr;
@@ -182,89 +183,157 @@ void Batch::ensureCapacity(size_t nRves, size_t nTiles) {
     EP2D_HIP(hipMalloc((void**)&dTiles_, cap * sizeof(int4)));
     EP2D_HIP(hipHostMalloc((void**)&hTiles_, cap * sizeof(int4), hipHostMallocDefault));
     capTiles_ = cap;
-    gLen_ = -1;
   }
 }
 
+void Batch::dropGraphs() {
+  for (auto& g : graphs_) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  graphs_.clear();
+}
+
+// Tiles of every admitted RVE for tile height ty, in the fused kernel's
+// geometry: ceil(nX/62) columns of 62 outputs by ceil(nY/ty) rows.  A tile
+// is (RVE index, i0, j0, unused); the RVE's current displacement buffer is in
+// its table entry, so the tiles stay valid while the RVEs iterate.
+int Batch::buildTiles(int ty, int4* out) const {
+  int t = 0;
+  for (size_t k = 0; k < active_.size(); k++) {
+    const Solver& s = *active_[k].s;
+    const int ntx = (s.nX() + kTileX - 1) / kTileX;
+    const int nty = (s.nY() + ty - 1) / ty;
+    for (int j = 0; j < nty; j++)
+      for (int i = 0; i < ntx; i++) {
+        if (out) out[t] = make_int4((int)k, i * kTileX, j * ty, 0);
+        t++;
+      }
+  }
+  return t;
+}
+
+// Graph of `len` launches over the current tables (parities 0, 1, 0, ...),
+// captured once per length while the tables stay the same.
+hipGraphExec_t Batch::graphFor(long len) {
+  for (auto& g : graphs_)
+    if (g.len == len) return g.exec;
+  if (graphs_.size() >= 4) dropGraphs();
+  GraphEntry g;
+  g.len = len;
+  EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  for (long q = 0; q < len; q++)
+    launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_);
+  EP2D_HIP(hipStreamEndCapture(stream_, &g.graph));
+  EP2D_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  graphs_.push_back(g);
+  return g.exec;
+}
+
 // c launches of the batched kernel over every admitted RVE.  Launch q uses
-// parity q&1; RVE k starts from buffer cur_k, so its table offset is cur_k.
+// parity q&1 and RVE k starts from its buffer cur_k (stored in its tiles), so
+// launch q reads buffer cur_k ^ (q&1) of RVE k.  Tiling does not change any
+// result, so a tuning chunk spends its first launches timing each tile height.
 void Batch::launchChunk(long c) {
-  std::vector<long long> key;
-  key.reserve(2 * active_.size() + 1);
-  size_t nt = 0;
-  for (auto& sl : active_) {
-    key.push_back(sl.id);
-    key.push_back(sl.s->cur());
-    nt += (size_t)sl.s->fusedPlan().ntiles;
-  }
-  key.push_back(c);
-  const bool sameTables = key == gKey_;
-  if (!sameTables) {
-    ensureCapacity(active_.size(), nt);
-    size_t t = 0;
-    for (size_t k = 0; k < active_.size(); k++) {
-      const Solver& s = *active_[k].s;
-      hRves_[k] = s.dev();
-      const FusedPlan& fp = s.fusedPlan();
-      const int nty = fp.ntiles / fp.ntx;
-      for (int ty = 0; ty < nty; ty++)
-        for (int tx = 0; tx < fp.ntx; tx++)
-          hTiles_[t++] = make_int4((int)k, tx * kTileX, ty * fp.ty, s.cur());
-    }
-    nTiles_ = (int)t;
-    EP2D_HIP(hipMemcpyAsync(dRves_, hRves_, active_.size() * sizeof(RveDev),
-                            hipMemcpyHostToDevice, stream_));
-    EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, t * sizeof(int4), hipMemcpyHostToDevice, stream_));
-  }
+  static const int kRows[4] = {6, 8, 14, 30};
+  std::vector<long long> ids;
+  ids.reserve(active_.size() + 2);
   double bytes = 0.0, cells = 0.0;
   for (auto& sl : active_) {
+    ids.push_back(sl.id);
     const double n = sl.s->nX();
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
   }
-  if (opt_.useGraphs) {
-    if (!sameTables || gLen_ != c) {
-      if (gexec_) EP2D_HIP(hipGraphExecDestroy(gexec_));
-      if (graph_) EP2D_HIP(hipGraphDestroy(graph_));
-      gexec_ = nullptr;
-      graph_ = nullptr;
-      EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      for (long q = 0; q < c; q++)
-        launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_);
-      EP2D_HIP(hipStreamEndCapture(stream_, &graph_));
-      EP2D_HIP(hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0));
-      gLen_ = c;
-      gKey_ = key;
+  if (opt_.tileRows == 0 && !tuneDue_ &&
+      (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
+    tuneDue_ = true;
+  constexpr long kTuneEach = 4;  // timed launches per candidate
+  const bool tuning = opt_.tileRows == 0 && tuneDue_ && c >= 4 * kTuneEach + 2;
+  ids.push_back(tuning ? 1 : 0);
+  ids.push_back(ty_);
+  if (ids != key_) {
+    dropGraphs();
+    size_t need = (size_t)buildTiles(ty_, nullptr);
+    if (tuning)
+      for (int k = 0; k < 4; k++) need += (size_t)buildTiles(kRows[k], nullptr);
+    ensureCapacity(active_.size(), need);
+    nTiles_ = buildTiles(ty_, hTiles_);
+    int off = nTiles_;
+    if (tuning)
+      for (int k = 0; k < 4; k++) {
+        tileOff_[k] = off;
+        tileCnt_[k] = buildTiles(kRows[k], hTiles_ + off);
+        off += tileCnt_[k];
+      }
+    EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, (size_t)off * sizeof(int4), hipMemcpyHostToDevice,
+                            stream_));
+    key_ = ids;
+    curs_.clear();
+  }
+  // RVE table: re-uploaded when an RVE's current buffer changed (odd chunks)
+  std::vector<int> curs(active_.size());
+  for (size_t k = 0; k < active_.size(); k++) curs[k] = active_[k].s->cur();
+  if (curs != curs_) {
+    for (size_t k = 0; k < active_.size(); k++) {
+      hRves_[k] = active_[k].s->dev();
+      hRves_[k].cur = curs[k];
     }
-    // with graphs the chunk is timed whole and counted as c launches
-    const bool timed = opt_.timingEvery > 0;
-    Pending ev{};
-    if (timed) {
-      ev = takeEvents();
-      ev.bytes = c * bytes;
-      ev.launches = c;
-      EP2D_HIP(hipEventRecord(ev.a, stream_));
+    EP2D_HIP(hipMemcpyAsync(dRves_, hRves_, active_.size() * sizeof(RveDev),
+                            hipMemcpyHostToDevice, stream_));
+    curs_ = curs;
+  }
+  long q = 0;
+  if (tuning) {
+    // one untimed launch per candidate first (code and tables warm), then
+    // kTuneEach timed launches of each
+    for (int k = 0; k < 4; k++) {
+      launch_fused_batch(dRves_, dTiles_ + tileOff_[k], tileCnt_[k], opt_.mode, kRows[k],
+                         (int)(q & 1), stream_);
+      q++;
     }
-    EP2D_HIP(hipGraphLaunch(gexec_, stream_));
+    for (long r = 0; r < kTuneEach; r++)
+      for (int k = 0; k < 4; k++) {
+        Pending ev = takeEvents();
+        ev.cand = k;
+        launch_fused_batch(dRves_, dTiles_ + tileOff_[k], tileCnt_[k], opt_.mode, kRows[k],
+                           (int)(q & 1), stream_, ev.a, ev.b);
+        evPending_.push_back(ev);
+        q++;
+      }
+    for (int k = 0; k < 4; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
+    tunedCells_ = cells;
+    tuneDue_ = false;
+    st_.tunings++;
+  }
+  const long rest = c - q;
+  const bool timed = opt_.timingEvery > 0;
+  if (opt_.useGraphs && !tuning) {
+    // graph of the chunk; with timing on, its last launch runs outside the
+    // graph between dispatch events
+    const long glen = timed ? rest - 1 : rest;
+    if (glen > 0) EP2D_HIP(hipGraphLaunch(graphFor(glen), stream_));
     if (timed) {
-      EP2D_HIP(hipEventRecord(ev.b, stream_));
+      Pending ev = takeEvents();
+      ev.bytes = bytes;
+      ev.launches = 1;
+      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)((q + glen) & 1), stream_,
+                         ev.a, ev.b);
       evPending_.push_back(ev);
     }
   } else {
-    for (long q = 0; q < c; q++) {
-      const bool timed = opt_.timingEvery > 0 && (st_.launches + q) % opt_.timingEvery == 0;
+    for (long r = 0; r < rest; r++, q++) {
+      const bool t = timed && (st_.launches + q) % opt_.timingEvery == 0;
       Pending ev{};
-      if (timed) {
+      if (t) {
         ev = takeEvents();
         ev.bytes = bytes;
         ev.launches = 1;
       }
-      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_,
-                         ev.a, ev.b);
-      if (timed) evPending_.push_back(ev);
+      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_, ev.a,
+                         ev.b);
+      if (t) evPending_.push_back(ev);
     }
-    gKey_ = key;
-    gLen_ = -1;  // tables may be reused, no graph to reuse
   }
   st_.launches += c;
   st_.chunks++;
@@ -286,15 +355,32 @@ Batch::Pending Batch::takeEvents() {
 }
 
 void Batch::resolveEvents() {
+  static const int kRows[4] = {6, 8, 14, 30};
+  bool tuned = false;
   for (auto& e : evPending_) {
     float ms = 0.f;
     EP2D_HIP(hipEventElapsedTime(&ms, e.a, e.b));
-    st_.timedMs += ms;
-    st_.timedBytes += e.bytes;
-    st_.timedLaunches += e.launches;
+    if (e.cand >= 0) {
+      tuneMs_[e.cand] += ms;
+      tuneN_[e.cand]++;
+      tuned = true;
+    } else {
+      st_.timedMs += ms;
+      st_.timedBytes += e.bytes;
+      st_.timedLaunches += e.launches;
+    }
+    e.cand = -1;
     evPool_.push_back(e);
   }
   evPending_.clear();
+  if (tuned) {
+    int best = -1;
+    for (int k = 0; k < 4; k++) {
+      st_.tileUs[k] = tuneN_[k] ? 1e3 * tuneMs_[k] / tuneN_[k] : 0.0;
+      if (tuneN_[k] && (best < 0 || st_.tileUs[k] < st_.tileUs[best])) best = k;
+    }
+    if (best >= 0) ty_ = kRows[best];
+  }
 }
 
 void Batch::step(int* nActive, int* nQueued) {
@@ -369,6 +455,7 @@ void Batch::stats(ep2d_batch_stats* s) const {
   s->active = (int)active_.size();
   s->queued = (int)queue_.size();
   s->deviceBytes = used_;
+  s->tileRows = ty_;
 }
 
 }  // namespace ep2d

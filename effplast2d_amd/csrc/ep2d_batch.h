@@ -38,13 +38,29 @@ class Batch {
     double tSubmit = 0, tAdmit = 0;
     size_t bytes = 0;
   };
+  struct Pending {
+    hipEvent_t a = nullptr, b = nullptr;
+    double bytes = 0;  // algorithmic bytes of the timed launches
+    long launches = 0;
+    int cand = -1;     // tile-height candidate under tuning, -1 for statistics
+  };
+  struct GraphEntry {
+    long len = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
 
   size_t estimateBytes(const ep2d_request& rq) const;
   void admit();
   void fail(Slot& sl, int status, const char* what);
   void finish(Slot& sl);
   void launchChunk(long c);
+  int buildTiles(int ty, int4* out) const;  // tile list of all admitted RVEs
   void ensureCapacity(size_t nRves, size_t nTiles);
+  void dropGraphs();
+  hipGraphExec_t graphFor(long len);
+  Pending takeEvents();
+  void resolveEvents();
 
   ep2d_batch_options opt_;
   int device_ = 0;
@@ -56,7 +72,9 @@ class Batch {
   long long nextId_ = 1;
   size_t budget_ = 0, used_ = 0;
 
-  // device tables of the batched launch (pinned host staging)
+  // device tables of the batched launch (pinned host staging); while a tile
+  // height is being tuned the tile lists of all candidates are stored back
+  // to back, candidate k at tileOff_[k]
   RveDev* dRves_ = nullptr;
   int4* dTiles_ = nullptr;
   RveDev* hRves_ = nullptr;
@@ -64,24 +82,23 @@ class Batch {
   size_t capRves_ = 0, capTiles_ = 0;
   int nTiles_ = 0;
   int ty_ = 8;
+  std::vector<long long> key_;    // RVE ids, tuning flag and tile height of the tiles
+  std::vector<int> curs_;         // current buffers in the uploaded RVE table
+  int tileOff_[4] = {0, 0, 0, 0}, tileCnt_[4] = {0, 0, 0, 0};
 
-  // graph of one chunk, reused while the tile table and chunk length repeat
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t gexec_ = nullptr;
-  long gLen_ = -1;
-  std::vector<long long> gKey_;
+  // tile-height tuning on real iterations
+  bool tuneDue_ = true;
+  double tunedCells_ = 0;
+  double tuneMs_[4] = {0, 0, 0, 0};
+  long tuneN_[4] = {0, 0, 0, 0};
+
+  // graphs of chunk lengths seen since the tables were last uploaded
+  std::vector<GraphEntry> graphs_;
 
   // statistics
   ep2d_batch_stats st_{};
-  struct Pending {
-    hipEvent_t a = nullptr, b = nullptr;
-    double bytes = 0;  // algorithmic bytes of the timed launches
-    long launches = 0;
-  };
   std::vector<Pending> evPool_;
   std::vector<Pending> evPending_;
-  Pending takeEvents();
-  void resolveEvents();
 };
 
 }  // namespace ep2d

@@ -49,6 +49,7 @@ struct RveDev {
   double vdampX, vdampY;    // 1 - dT*dampX, 1 - dT*dampY
   double dTrho;             // dT / rho
   int plastic;              // 1: return mapping on (nTasks > 1, EffPlast2D.cu:392)
+  int cur;      // batched path: buffer index of the current displacements
   double coh2lo;            // below coh^2: a smaller squared J2 cannot exceed coh
   float coh2lof;            // the same bound for the FP32 arithmetic of the split scheme
   double gtab[16];          // vertex shear modulus by vertex class (mask bits 3..6)

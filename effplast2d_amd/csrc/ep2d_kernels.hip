@@ -713,7 +713,7 @@ __global__ __launch_bounds__(64 * NW) void k_fused_batch(const RveDev* __restric
   __shared__ __attribute__((aligned(16))) char smem[FP::SMEM];
   const int4 t = tiles[tile_of_block(blockIdx.x, ntiles)];
   const RveDev& r = rves[t.x];
-  const int cur = parity ^ t.w;
+  const int cur = parity ^ r.cur;
   const int i0 = t.y, j0 = t.z;
   const bool interior = i0 >= 2 && i0 + FP::TX + 2 <= r.nX && j0 >= 2 && j0 + TY + 2 <= r.nY;
   if (r.plastic) {

@@ -176,6 +176,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   r_.P0 = 0.0;                         // SetInitPressure, :770-783
   r_.coh = coh_;
   r_.plastic = opt_.nTasks > 1 ? 1 : 0;
+  r_.cur = 0;
   r_.vdampX = 1.0 - dT_ * dampX_;      // :19
   r_.vdampY = 1.0 - dT_ * dampY_;      // :26
   r_.dTrho = dT_ / rho_;               // :19

@@ -58,6 +58,7 @@ class Solver {
   int mode() const { return mode_; }
   bool plastic() const { return opt_.nTasks > 1; }
   int nX() const { return nX_; }
+  int nY() const { return nY_; }
   hipStream_t stream() const { return stream_; }
 
   // step-level pieces

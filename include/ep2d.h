@@ -232,8 +232,8 @@ size_t ep2d_device_bytes(ep2d_handle* h);
  * admitted while the device memory budget allows. */
 typedef struct ep2d_batch_options {
   int mode;              /* EP2D_MODE_FP64 or EP2D_MODE_SPLIT                 */
-  long checkInterval;    /* residual check interval, common to all RVEs        */
-  int tileRows;          /* fused tile height (0: default)                     */
+  long checkInterval;    /* default residual check interval of requests         */
+  int tileRows;          /* fused tile height (0: tuned on the running batch)   */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
@@ -253,6 +253,7 @@ typedef struct ep2d_request {
   double loadValue;      /* damping amplitude (nTasks > 3)                     */
   int useShearLoadType;
   double shearLoadType[3];
+  long checkInterval;    /* residual check interval (0: the engine's)          */
   long long tag;         /* caller's identifier, returned with the result      */
 } ep2d_request;
 
@@ -272,6 +273,9 @@ typedef struct ep2d_batch_stats {
   long long timedLaunches;
   int active, queued;
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
+  int tileRows;                   /* tile height in use                       */
+  int tunings;                    /* tile-height tunings run                  */
+  double tileUs[4];               /* last tuning: mean launch us, rows 6/8/14/30 */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;
@@ -282,8 +286,12 @@ void ep2d_batch_destroy(ep2d_batch* b);
 /* Queue a request; *id receives its identifier.  Validation errors return
  * immediately; device errors are reported in the request's result. */
 int ep2d_batch_submit(ep2d_batch* b, const ep2d_request* rq, long long* id);
-/* One chunk: admit, run up to checkInterval iterations of every admitted RVE,
- * check, post-process, retire.  Returns the number admitted and queued. */
+/* One chunk: admit, run every admitted RVE to its next residual check (the
+ * nearest one over the batch), check, post-process, retire, admit.  With
+ * tileRows 0 the first chunks after the batch's cell count changes by more
+ * than a quarter time each tile height on real iterations (tiling does not
+ * change results) and keep the fastest.  Returns the number admitted and
+ * queued. */
 int ep2d_batch_step(ep2d_batch* b, int* nActive, int* nQueued);
 /* Step until nothing is queued or admitted. */
 int ep2d_batch_run(ep2d_batch* b);

@@ -109,3 +109,44 @@ def test_failed_request_reports_and_others_finish():
         res = {r["tag"]: r for r in b.poll()}
     assert res[0]["status"] == 0
     assert res[1]["status"] != 0 and res[1]["message"]
+
+
+def test_tuned_tile_height_equals_fixed():
+    """Tile height is tuned on real iterations of the running batch; tiling
+    does not change results."""
+    _note("tuned vs fixed tile rows")
+    tuned, st = _batch(REQS, "split")
+    assert st["tunings"] >= 1 and st["tileRows"] in (6, 8, 14, 30)
+    assert all(u > 0 for u in st["tileUs"])
+    for ty in (6, 30):
+        fixed, st2 = _batch(REQS, "split", tileRows=ty)
+        assert st2["tunings"] == 0 and st2["tileRows"] == ty
+        for t in range(len(REQS)):
+            _same(tuned[t]["result"], fixed[t]["result"])
+
+
+def test_request_check_interval():
+    """A request's own check interval overrides the engine's and gives the
+    single-RVE result at that interval, whatever the others use."""
+    _note("per-request check interval")
+    nX, nP, phi, Y, nl, load = REQS[1]
+    pa = make_pa(nX, nPores=nP, porosity=phi, Y=Y)
+    with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=300,
+                    mode="split") as g:
+        ref = g.ComputeEffModuli(load, LT)
+    with Batch(mode="split", checkInterval=K) as b:
+        for t, (n2, p2, f2, y2, l2, ld2) in enumerate(REQS):
+            b.submit(make_pa(n2, nPores=p2, porosity=f2, Y=y2), n2, nTasks=l2, nIter=200_000,
+                     eIter=1e-8, initLoad=ld2, loadType=LT, tag=t,
+                     checkInterval=300 if t == 1 else 0)
+        b.run()
+        got = {r["tag"]: r for r in b.poll()}
+    _same(got[1]["result"], ref)
+    _same(got[0]["result"], _single(REQS[0], "split"))
+
+
+def test_timing_with_graphs_counts_launches():
+    _note("graph timing")
+    _, st = _batch(REQS[:2], "fp64", timingEvery=1)
+    assert st["timedLaunches"] >= st["chunks"] - st["tunings"] - 1
+    assert st["timedMs"] > 0 and st["timedBytes"] > 0
