@@ -135,17 +135,28 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
+CAND_NAMES = ["tiles6", "tiles8", "tiles14", "tiles30", "march16", "march32", "march64",
+              "march128"]
+
+
+def shape_name(kind, rows):
+    return f"{'tiles' if kind == 1 else 'march'}{rows}"
+
+
 def batch_stats_summary(st, mode):
     """Kernel figures of a batched run from its sampled dispatch events."""
     kms = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else float("nan")
     gbs = st["timedBytes"] / st["timedLaunches"] / (kms * 1e-3) / 1e9 if st["timedLaunches"] \
         else float("nan")
-    return {"kernel": f"k_fused_batch<{'float' if mode == 'split' else 'double'}, TY>",
+    ct = "float" if mode == "split" else "double"
+    kern = f"k_march<{ct}, {ct}>" if st["kernel"] == 2 else f"k_fused_batch<{ct}, {ct}, TY>"
+    return {"kernel": kern,
             "kernel_avg_us": kms * 1e3, "kernel_samples": st["timedLaunches"],
             "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]
             else float("nan"),
             "achieved_GBs": gbs, "launches": st["launches"], "chunks": st["chunks"],
-            "tile_rows": st["tileRows"], "tile_us_by_rows": dict(zip((6, 8, 14, 30), st["tileUs"])),
+            "kernel_shape": shape_name(st["kernel"], st["kernelRows"]),
+            "tuning_us": dict(zip(CAND_NAMES, st["candUs"])),
             "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"]}
 
 
@@ -172,15 +183,18 @@ def headline_request(pa, nx, tag):
                 loadType=LOAD["loadType"], shearLoadType=LOAD["shearLoadType"], tag=tag)
 
 
-def run_batched_headline(pa, nx, mode, steps, warmup, check):
+def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
     if warmup:
-        run_batch([headline_request(pa, nx, k) for k in range(warmup)], mode, check)
-    res, st, dt = run_batch([headline_request(pa, nx, k) for k in range(steps)], mode, check)
+        run_batch([headline_request(pa, nx, k) for k in range(warmup)], mode, check,
+                  useGraphs=graphs)
+    res, st, dt = run_batch([headline_request(pa, nx, k) for k in range(steps)], mode, check,
+                            useGraphs=graphs)
     lat = [r["tDone"] - r["tSubmit"] for r in res.values()]
     tot_iters = sum(r["result"]["totalIters"] for r in res.values())
     r0 = res[0]["result"]
     log(f"[bench] batched {mode} x{steps}: {dt:.2f} s, {steps / dt:.2f} solves/s, "
-        f"tile rows {st['tileRows']} {['%.1f' % u for u in st['tileUs']]}")
+        f"kernel {shape_name(st['kernel'], st['kernelRows'])} "
+        f"{ {k: round(v, 1) for k, v in zip(CAND_NAMES, st['candUs'])} }")
     out = {"mode": mode, "batched": steps, "solves_per_s": steps / dt, "elapsed_s": dt,
            "ms_per_solve": 1e3 * dt / steps, "p50_ms": 1e3 * percentile(lat, 0.5),
            "p99_ms": 1e3 * percentile(lat, 0.99), "iters_per_solve": tot_iters / steps,
@@ -396,29 +410,29 @@ def probe(mode, nx, iters, copy_mib, batch=0):
         f"{ms.value:.3f} ms ({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
 
 
-def sweep(sizes, modes, tys, iters):
-    """Average fused-kernel duration per (scheme, tile height, grid size) on the
-    preloaded RVE, every launch timed with dispatch events.  This is the data
-    the default tile-height table is chosen from."""
+def sweep(sizes, modes, shapes, iters):
+    """Average fused-kernel duration per (scheme, kernel shape, grid size) on
+    the preloaded single RVE, every launch timed with dispatch events."""
     from effplast2d_amd import EffPlast2D
     rows = []
     for nx in sizes:
         pa, _ = make_pa(nx)
         for mode in modes:
-            for ty in tys:
-                with EffPlast2D(pa, nx, nTasks=3, mode=mode, tileRows=ty) as g:
+            for kind, n in shapes:
+                kw = {"marchRows": n} if kind == "march" else {"tileRows": n}
+                with EffPlast2D(pa, nx, nTasks=3, mode=mode, **kw) as g:
                     g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
                     g.iterate(20)
                     g.set_timing(1)
                     g.iterate(iters)
-                    ms, n, name = g.timing()
+                    ms, n_, name = g.timing()
                 b = BYTES_PER_POINT[mode] * (nx + 1) * nx
-                row = {"nx": nx, "mode": mode, "ty": ty, "kernel_us": ms * 1e3,
+                row = {"nx": nx, "mode": mode, "shape": f"{kind}{n}", "kernel_us": ms * 1e3,
                        "GBs": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "kernel": name}
                 rows.append(row)
-                log(f"[sweep] nx={nx} {mode} TY={ty}: {ms * 1e3:.2f} us, "
-                    f"{row['GBs']:.0f} GB/s ({100 * row['frac']:.1f}%) {name}")
+                log(f"[sweep] nx={nx} {mode} {kind}{n}: {ms * 1e3:.2f} us, "
+                    f"{row['GBs']:.0f} GB/s ({100 * row['frac']:.1f}%)")
     print(json.dumps({"sweep": rows}), flush=True)
 
 
@@ -536,6 +550,9 @@ def main():
     ap.add_argument("--check-interval", type=int, default=500)
     ap.add_argument("--timing-every", type=int, default=16)
     ap.add_argument("--no-legs", action="store_true", help="headline and CPU baseline only")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the batched chunks on the stream instead of replaying graphs "
+                         "(for kernel tracers that cannot follow graph replays)")
     ap.add_argument("--single-steps", type=int, default=2)
     ap.add_argument("--c4-rves", type=int, default=256)
     ap.add_argument("--c4-seed", type=int, default=2024)
@@ -578,7 +595,8 @@ def main():
         probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch)
         return 0
     if a.sweep:
-        sweep([int(x) for x in a.sweep_sizes.split(",")], ["fp64", "split"], [6, 8, 14, 30],
+        shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]
+        sweep([int(x) for x in a.sweep_sizes.split(",")], ["fp64", "split"], shapes,
               a.sweep_iters)
         return 0
     if a.config3:
@@ -590,7 +608,8 @@ def main():
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
     pa, nx = make_pa(a.nx)
-    head = run_batched_headline(pa, nx, a.mode, a.steps, a.warmup, a.check_interval)
+    head = run_batched_headline(pa, nx, a.mode, a.steps, a.warmup, a.check_interval,
+                                graphs=not a.no_graphs)
     legs = {}
     if not a.no_legs:
         for m in ("fp64", "split", "plain"):
@@ -630,12 +649,14 @@ def main():
                                f"scheme={a.mode}",
                    "nx": nx, "nTasks": 3, "check_interval": a.check_interval, "eIter": 1e-8,
                    "nIter": 100000, "batched_requests": a.steps,
+                   "graphs": not a.no_graphs,
                    "shear_load_type": [0, 0, 1]},
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": frac, "traffic": traffic,
                      "kernel": head["kernel"], "kernel_avg_us": head["kernel_avg_us"],
                      "bytes_per_launch": head["bytes_per_launch"],
-                     "kernel_samples": head["kernel_samples"], "tile_rows": head["tile_rows"]},
+                     "kernel_samples": head["kernel_samples"],
+                     "kernel_shape": head["kernel_shape"]},
         "cell_iter_per_s": head["cell_iter_per_s"],
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],

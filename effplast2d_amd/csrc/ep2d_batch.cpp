@@ -20,6 +20,15 @@ namespace ep2d {
 
 namespace {
 constexpr int kTileX = 62;  // fused tile width (Fused::TX)
+}  // namespace
+
+// Kernel shapes the tuner times: the fused tile kernel at four tile heights
+// and the row-marching kernel at four segment lengths.
+const Batch::Shape Batch::kCand[Batch::kNCand] = {
+    {kTiles, 6}, {kTiles, 8}, {kTiles, 14}, {kTiles, 30},
+    {kMarch, 16}, {kMarch, 32}, {kMarch, 64}, {kMarch, 128}};
+
+namespace {
 
 bool valid_mode(int m) { return m == EP2D_MODE_FP64 || m == EP2D_MODE_SPLIT; }
 }  // namespace
@@ -29,7 +38,12 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.checkInterval <= 0) opt_.checkInterval = 10000;
   if (opt_.tileRows != 0 && !valid_tile_rows(opt_.tileRows))
     throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
-  ty_ = opt_.tileRows ? opt_.tileRows : 8;
+  if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
+  if (opt_.tileRows != 0 && opt_.marchRows != 0)
+    throw ArgError("give tileRows or marchRows, not both");
+  fixed_ = opt_.tileRows != 0 || opt_.marchRows != 0;
+  if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
+  else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -107,7 +121,7 @@ void Batch::admit() {
     o.nTimeSteps = jb.rq.nTimeSteps;
     o.useShearLoadType = jb.rq.useShearLoadType;
     std::memcpy(o.shearLoadType, jb.rq.shearLoadType, sizeof(o.shearLoadType));
-    o.tileRows = ty_;
+    o.tileRows = 0;
     try {
       sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_);
       sl.bytes = sl.s->deviceBytes();
@@ -194,23 +208,37 @@ void Batch::dropGraphs() {
   graphs_.clear();
 }
 
-// Tiles of every admitted RVE for tile height ty, in the fused kernel's
-// geometry: ceil(nX/62) columns of 62 outputs by ceil(nY/ty) rows.  A tile
-// is (RVE index, i0, j0, unused); the RVE's current displacement buffer is in
-// its table entry, so the tiles stay valid while the RVEs iterate.
-int Batch::buildTiles(int ty, int4* out) const {
+// Work list of every admitted RVE for a kernel shape.  Tiles (fused tile
+// kernel): ceil(nX/62) columns of 62 outputs by ceil(nY/rows) rows, entries
+// (RVE index, i0, j0, unused).  Segments (row-marching kernel): strips of 60
+// output columns by runs of `rows` rows, entries (RVE index, s, j0, j1).  The
+// RVE's current displacement buffer is in its table entry, so the list stays
+// valid while the RVEs iterate.
+int Batch::buildWork(Shape sh, int4* out) const {
   int t = 0;
   for (size_t k = 0; k < active_.size(); k++) {
     const Solver& s = *active_[k].s;
+    if (sh.kind == kMarch) {
+      t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
+      continue;
+    }
     const int ntx = (s.nX() + kTileX - 1) / kTileX;
-    const int nty = (s.nY() + ty - 1) / ty;
+    const int nty = (s.nY() + sh.rows - 1) / sh.rows;
     for (int j = 0; j < nty; j++)
       for (int i = 0; i < ntx; i++) {
-        if (out) out[t] = make_int4((int)k, i * kTileX, j * ty, 0);
+        if (out) out[t] = make_int4((int)k, i * kTileX, j * sh.rows, 0);
         t++;
       }
   }
   return t;
+}
+
+void Batch::launchWork(Shape sh, const int4* work, int n, int parity, hipEvent_t e0,
+                       hipEvent_t e1) {
+  if (sh.kind == kMarch)
+    launch_march(dRves_, work, n, opt_.mode, parity, stream_, e0, e1);
+  else
+    launch_fused_batch(dRves_, work, n, opt_.mode, sh.rows, parity, stream_, e0, e1);
 }
 
 // Graph of `len` launches over the current tables (parities 0, 1, 0, ...),
@@ -222,8 +250,7 @@ hipGraphExec_t Batch::graphFor(long len) {
   GraphEntry g;
   g.len = len;
   EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  for (long q = 0; q < len; q++)
-    launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_);
+  for (long q = 0; q < len; q++) launchWork(shape_, dTiles_, nTiles_, (int)(q & 1));
   EP2D_HIP(hipStreamEndCapture(stream_, &g.graph));
   EP2D_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
   graphs_.push_back(g);
@@ -231,13 +258,13 @@ hipGraphExec_t Batch::graphFor(long len) {
 }
 
 // c launches of the batched kernel over every admitted RVE.  Launch q uses
-// parity q&1 and RVE k starts from its buffer cur_k (stored in its tiles), so
-// launch q reads buffer cur_k ^ (q&1) of RVE k.  Tiling does not change any
-// result, so a tuning chunk spends its first launches timing each tile height.
+// parity q&1 and RVE k starts from its buffer cur_k (in its table entry), so
+// launch q reads buffer cur_k ^ (q&1) of RVE k.  The kernel shape does not
+// change any result, so a tuning chunk spends its first launches timing each
+// candidate shape.
 void Batch::launchChunk(long c) {
-  static const int kRows[4] = {6, 8, 14, 30};
   std::vector<long long> ids;
-  ids.reserve(active_.size() + 2);
+  ids.reserve(active_.size() + 3);
   double bytes = 0.0, cells = 0.0;
   for (auto& sl : active_) {
     ids.push_back(sl.id);
@@ -245,26 +272,26 @@ void Batch::launchChunk(long c) {
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
   }
-  if (opt_.tileRows == 0 && !tuneDue_ &&
-      (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
+  if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
   constexpr long kTuneEach = 4;  // timed launches per candidate
-  const bool tuning = opt_.tileRows == 0 && tuneDue_ && c >= 4 * kTuneEach + 2;
+  const bool tuning = !fixed_ && tuneDue_ && c >= (kTuneEach + 1) * kNCand + 2;
   ids.push_back(tuning ? 1 : 0);
-  ids.push_back(ty_);
+  ids.push_back(shape_.kind);
+  ids.push_back(shape_.rows);
   if (ids != key_) {
     dropGraphs();
-    size_t need = (size_t)buildTiles(ty_, nullptr);
+    size_t need = (size_t)buildWork(shape_, nullptr);
     if (tuning)
-      for (int k = 0; k < 4; k++) need += (size_t)buildTiles(kRows[k], nullptr);
+      for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
     ensureCapacity(active_.size(), need);
-    nTiles_ = buildTiles(ty_, hTiles_);
+    nTiles_ = buildWork(shape_, hTiles_);
     int off = nTiles_;
     if (tuning)
-      for (int k = 0; k < 4; k++) {
-        tileOff_[k] = off;
-        tileCnt_[k] = buildTiles(kRows[k], hTiles_ + off);
-        off += tileCnt_[k];
+      for (int k = 0; k < kNCand; k++) {
+        candOff_[k] = off;
+        candCnt_[k] = buildWork(kCand[k], hTiles_ + off);
+        off += candCnt_[k];
       }
     EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, (size_t)off * sizeof(int4), hipMemcpyHostToDevice,
                             stream_));
@@ -286,22 +313,17 @@ void Batch::launchChunk(long c) {
   long q = 0;
   if (tuning) {
     // one untimed launch per candidate first (code and tables warm), then
-    // kTuneEach timed launches of each
-    for (int k = 0; k < 4; k++) {
-      launch_fused_batch(dRves_, dTiles_ + tileOff_[k], tileCnt_[k], opt_.mode, kRows[k],
-                         (int)(q & 1), stream_);
-      q++;
-    }
+    // kTuneEach timed launches of each, interleaved
+    for (int k = 0; k < kNCand; k++, q++)
+      launchWork(kCand[k], dTiles_ + candOff_[k], candCnt_[k], (int)(q & 1));
     for (long r = 0; r < kTuneEach; r++)
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < kNCand; k++, q++) {
         Pending ev = takeEvents();
         ev.cand = k;
-        launch_fused_batch(dRves_, dTiles_ + tileOff_[k], tileCnt_[k], opt_.mode, kRows[k],
-                           (int)(q & 1), stream_, ev.a, ev.b);
+        launchWork(kCand[k], dTiles_ + candOff_[k], candCnt_[k], (int)(q & 1), ev.a, ev.b);
         evPending_.push_back(ev);
-        q++;
       }
-    for (int k = 0; k < 4; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
+    for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
     tunedCells_ = cells;
     tuneDue_ = false;
     st_.tunings++;
@@ -317,8 +339,7 @@ void Batch::launchChunk(long c) {
       Pending ev = takeEvents();
       ev.bytes = bytes;
       ev.launches = 1;
-      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)((q + glen) & 1), stream_,
-                         ev.a, ev.b);
+      launchWork(shape_, dTiles_, nTiles_, (int)((q + glen) & 1), ev.a, ev.b);
       evPending_.push_back(ev);
     }
   } else {
@@ -330,8 +351,7 @@ void Batch::launchChunk(long c) {
         ev.bytes = bytes;
         ev.launches = 1;
       }
-      launch_fused_batch(dRves_, dTiles_, nTiles_, opt_.mode, ty_, (int)(q & 1), stream_, ev.a,
-                         ev.b);
+      launchWork(shape_, dTiles_, nTiles_, (int)(q & 1), ev.a, ev.b);
       if (t) evPending_.push_back(ev);
     }
   }
@@ -355,7 +375,6 @@ Batch::Pending Batch::takeEvents() {
 }
 
 void Batch::resolveEvents() {
-  static const int kRows[4] = {6, 8, 14, 30};
   bool tuned = false;
   for (auto& e : evPending_) {
     float ms = 0.f;
@@ -375,11 +394,11 @@ void Batch::resolveEvents() {
   evPending_.clear();
   if (tuned) {
     int best = -1;
-    for (int k = 0; k < 4; k++) {
-      st_.tileUs[k] = tuneN_[k] ? 1e3 * tuneMs_[k] / tuneN_[k] : 0.0;
-      if (tuneN_[k] && (best < 0 || st_.tileUs[k] < st_.tileUs[best])) best = k;
+    for (int k = 0; k < kNCand; k++) {
+      st_.candUs[k] = tuneN_[k] ? 1e3 * tuneMs_[k] / tuneN_[k] : 0.0;
+      if (tuneN_[k] && (best < 0 || st_.candUs[k] < st_.candUs[best])) best = k;
     }
-    if (best >= 0) ty_ = kRows[best];
+    if (best >= 0) shape_ = kCand[best];
   }
 }
 
@@ -455,7 +474,8 @@ void Batch::stats(ep2d_batch_stats* s) const {
   s->active = (int)active_.size();
   s->queued = (int)queue_.size();
   s->deviceBytes = used_;
-  s->tileRows = ty_;
+  s->kernel = shape_.kind;
+  s->kernelRows = shape_.rows;
 }
 
 }  // namespace ep2d

@@ -44,6 +44,13 @@ class Batch {
     long launches = 0;
     int cand = -1;     // tile-height candidate under tuning, -1 for statistics
   };
+  static constexpr int kTiles = 1, kMarch = 2;
+  struct Shape {
+    int kind = kTiles;  // fused tile kernel or row-marching kernel
+    int rows = 8;       // tile height or segment rows
+  };
+  static constexpr int kNCand = 8;
+  static const Shape kCand[kNCand];
   struct GraphEntry {
     long len = 0;
     hipGraph_t graph = nullptr;
@@ -55,7 +62,9 @@ class Batch {
   void fail(Slot& sl, int status, const char* what);
   void finish(Slot& sl);
   void launchChunk(long c);
-  int buildTiles(int ty, int4* out) const;  // tile list of all admitted RVEs
+  int buildWork(Shape sh, int4* out) const;  // work list of all admitted RVEs
+  void launchWork(Shape sh, const int4* work, int n, int parity, hipEvent_t e0 = nullptr,
+                  hipEvent_t e1 = nullptr);
   void ensureCapacity(size_t nRves, size_t nTiles);
   void dropGraphs();
   hipGraphExec_t graphFor(long len);
@@ -72,25 +81,26 @@ class Batch {
   long long nextId_ = 1;
   size_t budget_ = 0, used_ = 0;
 
-  // device tables of the batched launch (pinned host staging); while a tile
-  // height is being tuned the tile lists of all candidates are stored back
-  // to back, candidate k at tileOff_[k]
+  // device tables of the batched launch (pinned host staging); while the
+  // kernel shape is being tuned the work lists of all candidates follow the
+  // current one, candidate k at candOff_[k]
   RveDev* dRves_ = nullptr;
   int4* dTiles_ = nullptr;
   RveDev* hRves_ = nullptr;
   int4* hTiles_ = nullptr;
   size_t capRves_ = 0, capTiles_ = 0;
   int nTiles_ = 0;
-  int ty_ = 8;
-  std::vector<long long> key_;    // RVE ids, tuning flag and tile height of the tiles
+  Shape shape_{};
+  bool fixed_ = false;
+  std::vector<long long> key_;    // RVE ids, tuning flag and shape of the work list
   std::vector<int> curs_;         // current buffers in the uploaded RVE table
-  int tileOff_[4] = {0, 0, 0, 0}, tileCnt_[4] = {0, 0, 0, 0};
+  int candOff_[kNCand] = {}, candCnt_[kNCand] = {};
 
-  // tile-height tuning on real iterations
+  // kernel-shape tuning on real iterations
   bool tuneDue_ = true;
   double tunedCells_ = 0;
-  double tuneMs_[4] = {0, 0, 0, 0};
-  long tuneN_[4] = {0, 0, 0, 0};
+  double tuneMs_[kNCand] = {};
+  long tuneN_[kNCand] = {};
 
   // graphs of chunk lengths seen since the tables were last uploaded
   std::vector<GraphEntry> graphs_;

@@ -274,7 +274,8 @@ int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples, const ch
 
 int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs) {
   return guarded([&] {
-    if (!avgMs || reps < 1 || bytes < 16) throw ep2d::ArgError("bad argument");
+    if (!avgMs || reps < 1 || bytes < 16 || bytes > ((size_t)64 << 30))
+      throw ep2d::ArgError("bad argument");
     EP2D_HIP(hipSetDevice(device));
     const size_t n = bytes / 16 * 2;
     double *a = nullptr, *b = nullptr;

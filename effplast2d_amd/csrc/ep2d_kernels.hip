@@ -692,9 +692,8 @@ __global__ void k_gather(const double* const* __restrict__ bases, const int2* __
 // Plain streaming copy (bandwidth calibration).
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src,
                                               double2* __restrict__ dst, size_t n) {
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    dst[k] = src[k];
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[k];
 }
 
 inline dim3 grid2(int nx, int ny, dim3 blk) {
@@ -799,6 +798,337 @@ int dispatch_fused(int mode, bool plastic, int ty, A&&... a) {
     if (plastic) { EP2D_TY_CASES(double, double, true) } else { EP2D_TY_CASES(double, double, false) }
   }
 #undef EP2D_TY_CASES
+}
+
+// ---------------------------------------------------------------------------
+// Row-marching fused iteration.  One wave64 owns a strip of 64 columns
+// (lane L <-> column i = s + L) and walks down rows j0-2 .. j1-1 of one RVE.
+// Step j loads row j+1 of Ux and the mask, row j+2 of Uy and row j of V, and
+// produces, all in registers:
+//
+//   trial cell row j+1 and trial vertex row j      (ComputeStress, :37-98)
+//   return-mapped vertex row j and cell row j       (ComputeJ2 / Plasticity,
+//                                                     :100-177)
+//   V and U of row j, lanes 2..61                   (ComputeDisp, :3-35)
+//
+// Neighbours in x come from the adjacent lanes through DPP wave shifts;
+// neighbours in y are the rows carried from the previous steps, so nothing
+// is staged in LDS and no barrier is needed.  The loads of step j+1 are
+// issued before step j computes.  Steps j0-2 and j0-1 only fill the carried
+// rows (the halo of the segment); outputs start at row j0.
+//
+// Every expression is the fused tile kernel's (and so the reference's) in the
+// same order: the two kernels give identical results.  EDGE segments (strips
+// or rows touching the RVE boundary) keep the boundary conditions, zero the
+// stress outside the grid and use the clamped vertex block of the reference's
+// ComputeJ2 edge branches for cells on the boundary.
+
+// value of lane L+1 (0 in lane 63) and of lane L-1 (0 in lane 0)
+__device__ __forceinline__ int lane_next_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+__device__ __forceinline__ int lane_prev_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+__device__ __forceinline__ float lane_next(float v) {
+  return __builtin_bit_cast(float, lane_next_i(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ float lane_prev(float v) {
+  return __builtin_bit_cast(float, lane_prev_i(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double lane_next(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)lane_next_i((int)(unsigned)b);
+  const unsigned hi = (unsigned)lane_next_i((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ double lane_prev(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)lane_prev_i((int)(unsigned)b);
+  const unsigned hi = (unsigned)lane_prev_i((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// Global-memory access at a 32-bit element index: the base stays in SGPRs
+// (uniform per wave) and the byte offset in one VGPR (saddr form).
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, unsigned idx) {
+  using GT = const __attribute__((address_space(1))) T;
+  using GC = const __attribute__((address_space(1))) char;
+  return *(GT*)((GC*)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
+  using GT = __attribute__((address_space(1))) T;
+  using GC = __attribute__((address_space(1))) char;
+  *(GT*)((GC*)base + idx * (unsigned)sizeof(T)) = v;
+}
+
+template <typename CT, typename VT>
+struct March {
+  static __device__ __forceinline__ CT dx(double a, const RveDev& r) {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, r.dX, r.rdX);
+    else return (CT)a * (CT)r.rdX;
+  }
+  static __device__ __forceinline__ CT dy(double a, const RveDev& r) {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, r.dY, r.rdY);
+    else return (CT)a * (CT)r.rdY;
+  }
+  static __device__ __forceinline__ CT third(CT a) {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, 3.0, 1.0 / 3.0);
+    else return a * (CT)(1.0 / 3.0);
+  }
+
+  struct Ld {
+    double ux, uy;   // Ux row j+1, Uy row j+2
+    unsigned m;      // mask row j+1
+    VT vx, vy;       // V row j
+  };
+
+  template <bool EDGE>
+  static __device__ __forceinline__ Ld load(const RveDev& r, const double* __restrict__ Ux,
+                                            const double* __restrict__ Uy, int i, int j) {
+    const int nX = r.nX, nY = r.nY, sx = nX + 1;
+    const VT* Vx = static_cast<const VT*>(r.Vx);
+    const VT* Vy = static_cast<const VT*>(r.Vy);
+    Ld d;
+    if (!EDGE) {
+      d.ux = gld(Ux, (unsigned)((j + 1) * sx + i));
+      d.uy = gld(Uy, (unsigned)((j + 2) * nX + i));
+      d.m = gld(r.mask, (unsigned)((j + 1) * nX + i));
+      d.vx = gld(Vx, (unsigned)(j * sx + i));
+      d.vy = gld(Vy, (unsigned)(j * nX + i));
+    } else {
+      const bool ci = i >= 0 && i < nX;
+      d.ux = (i >= 0 && i <= nX && j + 1 >= 0 && j + 1 < nY) ? gld(Ux, (unsigned)((j + 1) * sx + i)) : 0.0;
+      d.uy = (ci && j + 2 >= 0 && j + 2 <= nY) ? gld(Uy, (unsigned)((j + 2) * nX + i)) : 0.0;
+      d.m = (ci && j + 1 >= 0 && j + 1 < nY) ? (unsigned)gld(r.mask, (unsigned)((j + 1) * nX + i)) : 0u;
+      const bool vr = ci && j >= 0 && j < nY;
+      d.vx = vr ? gld(Vx, (unsigned)(j * sx + i)) : (VT)0;
+      d.vy = vr ? gld(Vy, (unsigned)(j * nX + i)) : (VT)0;
+    }
+    return d;
+  }
+
+  // Return mapping of a vertex from its trial shear stress and the four trial
+  // cells around it, summed in the reference's order.
+  static __device__ __forceinline__ CT vertex_rm(CT txy, CT sxx, CT syy, CT coh, CT c2lo) {
+    const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+    if (s2 > c2lo) {
+      const CT J2XY = sqrt(s2);
+      if (J2XY > coh) txy *= coh / J2XY;
+    }
+    return txy;
+  }
+  static __device__ __forceinline__ void cell_rm(CT& txx, CT& tyy, CT av, CT coh, CT c2lo) {
+    const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+    if (s2 > c2lo) {
+      const CT J2 = sqrt(s2);
+      if (J2 > coh) {
+        const CT sc = coh / J2;
+        txx = txx * sc;
+        tyy = tyy * sc;
+      }
+    }
+  }
+
+  template <bool EDGE>
+  static __device__ void run(const RveDev& r, int cur, int s, int j0, int j1, const double* gt) {
+    const int nX = r.nX, nY = r.nY, sx = nX + 1;
+    const int L = threadIdx.x & 63;
+    const int i = s + L;
+    const double* __restrict__ Uxi = r.Ux[cur];
+    const double* __restrict__ Uyi = r.Uy[cur];
+    double* __restrict__ Uxo = r.Ux[cur ^ 1];
+    double* __restrict__ Uyo = r.Uy[cur ^ 1];
+    VT* Vx = static_cast<VT*>(r.Vx);
+    VT* Vy = static_cast<VT*>(r.Vy);
+    const bool plastic = r.plastic != 0;
+    const CT coh = (CT)r.coh;
+    CT c2lo;
+    if constexpr (sizeof(CT) == 8) c2lo = r.coh2lo;
+    else c2lo = r.coh2lof;
+    const CT K0 = (CT)r.K0, Kp = (CT)r.Kp, G0 = (CT)r.G0, Gp = (CT)r.Gp, P0 = (CT)r.P0;
+    const CT vdampX = (CT)r.vdampX, vdampY = (CT)r.vdampY, dTrho = (CT)r.dTrho;
+    const double dT = r.dT;
+    const bool outLane = L >= 2 && L < 62 && (!EDGE || i < nX);
+    const bool colC = !EDGE || (i >= 0 && i < nX);
+    const bool colV = !EDGE || (i >= 0 && i < nX - 1);
+    const bool leftEdge = EDGE && i == 0, rightEdge = EDGE && i == nX - 1;
+
+    // carried rows (see the step comments below)
+    const int jb = j0 - 2;
+    double ux_c = 0.0, uy_c = 0.0, uy_n = 0.0;
+    unsigned m_c = 0;
+    if (!EDGE) {
+      ux_c = gld(Uxi, (unsigned)(jb * sx + i));
+      uy_n = gld(Uyi, (unsigned)((jb + 1) * nX + i));
+      m_c = gld(r.mask, (unsigned)(jb * nX + i));
+    } else {
+      if (i >= 0 && i <= nX && jb >= 0 && jb < nY) ux_c = gld(Uxi, (unsigned)(jb * sx + i));
+      if (colC && jb + 1 >= 0 && jb + 1 <= nY) uy_n = gld(Uyi, (unsigned)((jb + 1) * nX + i));
+      if (colC && jb >= 0 && jb < nY) m_c = gld(r.mask, (unsigned)(jb * nX + i));
+    }
+    double uxs_c = lane_next(ux_c);
+    CT tP_c = 0, tXX_c = 0, tYY_c = 0, sXX_c = 0, sYY_c = 0;   // trial cell row j
+    CT tP_p = 0, tXX_p = 0, tYY_p = 0;                         // trial cell row j-1 (EDGE)
+    CT tXY_p = 0, tXY_pp = 0;                                  // trial vertex rows j-1, j-2
+    CT cP_p = 0, cYY_p = 0, cXY_p = 0;                         // corrected row j-1
+
+    Ld ld = load<EDGE>(r, Uxi, Uyi, i, jb);
+    for (int j = jb; j < j1; j++) {
+      Ld nx = ld;
+      if (j + 1 < j1) nx = load<EDGE>(r, Uxi, Uyi, i, j + 1);
+
+      // trial cell row j+1 (cell i, row j+1)
+      const double ux_n = ld.ux, uy_nn = ld.uy;
+      const unsigned m_n = ld.m;
+      const double uxs_n = lane_next(ux_n);
+      CT tP_n = 0, tXX_n = 0, tYY_n = 0;
+      {
+        const CT K = (m_n & M_CELL_MAT) ? Kp : K0;
+        const CT G = (m_n & M_CELL_MAT) ? Gp : G0;
+        const CT exx = dx(uxs_n - ux_n, r);
+        const CT eyy = dy(uy_nn - uy_n, r);
+        CT pp = P0 - K * (exx + eyy);
+        CT txx = (CT)2.0 * G * (exx - third(exx + eyy));
+        CT tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
+        const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
+        if ((m_n & M_CELL_ZERO) || !ok) {
+          pp = 0;
+          txx = 0;
+          tyy = 0;
+        }
+        tP_n = pp;
+        tXX_n = txx;
+        tYY_n = tyy;
+      }
+      const CT sXX_n = lane_next(tXX_n), sYY_n = lane_next(tYY_n);
+
+      // trial vertex row j (vertex i, row j)
+      const double uys_n = lane_next(uy_n);
+      CT tXY_c;
+      {
+        const CT g = (CT)gt[(m_c >> M_VCLASS_SHIFT) & 15];
+        CT txy = g * (dy(uxs_n - uxs_c, r) + dx(uys_n - uy_n, r));
+        const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
+        if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
+        tXY_c = txy;
+      }
+
+      // return mapping: vertex row j, cell row j (and, on the top boundary,
+      // cell row 0 once vertex row 1 exists)
+      CT cXY_c = tXY_c, cP_c = tP_c, cXX_c = tXX_c, cYY_c = tYY_c;
+      if (plastic) {
+        const bool vok = !EDGE || (colV && j >= 0 && j < nY - 1);
+        const CT sxx = tXX_c + sXX_c + tXX_n + sXX_n;
+        const CT syy = tYY_c + sYY_c + tYY_n + sYY_n;
+        if (vok) cXY_c = vertex_rm(tXY_c, sxx, syy, coh, c2lo);
+
+        const CT pr_p = lane_prev(tXY_p), pr_c = lane_prev(tXY_c);
+        if (!EDGE) {
+          const CT av = (CT)0.25 * (pr_p + tXY_p + pr_c + tXY_c);
+          cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+        } else {
+          // vertex pair of this lane's cell in a vertex row: (first, second)
+          const CT nx_p = lane_next(tXY_p), nx_c = lane_next(tXY_c), nx_pp = lane_next(tXY_pp);
+          const CT pr_pp = lane_prev(tXY_pp);
+          const CT pp_p = lane_prev(pr_p), pp_c = lane_prev(pr_c), pp_pp = lane_prev(pr_pp);
+          auto first = [&](CT t, CT pr, CT pp) { return leftEdge ? t : (rightEdge ? pp : pr); };
+          auto second = [&](CT t, CT pr, CT nxv) { return leftEdge ? nxv : (rightEdge ? pr : t); };
+          if (colC && j >= 1 && j < nY - 1) {
+            const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
+                                      first(tXY_c, pr_c, pp_c) + second(tXY_c, pr_c, nx_c));
+            cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+          } else if (colC && j == nY - 1 && j >= 1) {  // rows nY-3, nY-2
+            const CT av = (CT)0.25 * (first(tXY_pp, pr_pp, pp_pp) + second(tXY_pp, pr_pp, nx_pp) +
+                                      first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p));
+            cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+          }
+          if (j == 1) {  // cell row 0 from vertex rows 0 and 1
+            CT xx = tXX_p, yy = tYY_p;
+            if (colC) {
+              const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
+                                        first(tXY_c, pr_c, pp_c) + second(tXY_c, pr_c, nx_c));
+              cell_rm(xx, yy, av, coh, c2lo);
+            }
+            cP_p = tP_p;
+            cYY_p = yy;
+          }
+        }
+      }
+
+      // velocity and displacement of row j
+      const CT rP = lane_prev(cP_c), rXX = lane_prev(cXX_c);
+      const CT rXY_c = lane_prev(cXY_c), rXY_p = lane_prev(cXY_p);
+      if (j >= j0) {
+        CT vx = (CT)ld.vx, vy = (CT)ld.vy;
+        if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
+          vx = vx * vdampX + dTrho * (dx(-cP_c + rP + cXX_c - rXX, r) + dy(rXY_c - rXY_p, r));
+          if (outLane) gst(Vx, (unsigned)(j * sx + i), (VT)vx);
+        }
+        if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
+          vy = vy * vdampY + dTrho * (dy(-cP_c + cP_p + cYY_c - cYY_p, r) + dx(cXY_p - rXY_p, r));
+          if (outLane) gst(Vy, (unsigned)(j * nX + i), (VT)vy);
+        }
+        if (outLane && (!EDGE || j < nY)) {
+          gst(Uxo, (unsigned)(j * sx + i), ux_c + (double)vx * dT);
+          gst(Uyo, (unsigned)(j * nX + i), uy_c + (double)vy * dT);
+          if (EDGE) {
+            if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxs_c);
+            if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uy_n);
+          }
+        }
+      }
+
+      // rotate
+      ux_c = ux_n;
+      uxs_c = uxs_n;
+      uy_c = uy_n;
+      uy_n = uy_nn;
+      m_c = m_n;
+      tP_p = tP_c;
+      tXX_p = tXX_c;
+      tYY_p = tYY_c;
+      tP_c = tP_n;
+      tXX_c = tXX_n;
+      tYY_c = tYY_n;
+      sXX_c = sXX_n;
+      sYY_c = sYY_n;
+      tXY_pp = tXY_p;
+      tXY_p = tXY_c;
+      cP_p = cP_c;
+      cYY_p = cYY_c;
+      cXY_p = cXY_c;
+      ld = nx;
+    }
+  }
+};
+
+// One wave per segment (rve, s, j0, j1); four independent waves per block.
+// Blocks are dealt to the XCDs in contiguous runs of the segment list, which
+// is ordered RVE by RVE, strip by strip, so neighbouring segments (sharing
+// halo rows and columns) meet in the same L2.
+template <typename CT, typename VT>
+__global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
+                                               const int4* __restrict__ segs, int nsegs,
+                                               int parity) {
+  __shared__ double gt[4][16];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = tile_of_block(blockIdx.x, gridDim.x);
+  const int k = b * 4 + wave;
+  if (k >= nsegs) return;
+  const int4 sg = segs[k];
+  const RveDev& r = rves[sg.x];
+  if (lane < 16) gt[wave][lane] = r.gtab[lane];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const int cur = parity ^ r.cur;
+  const int s = sg.y, j0 = sg.z, j1 = sg.w;
+  const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 2 && j1 <= r.nY - 1;
+  if (interior) March<CT, VT>::template run<false>(r, cur, s, j0, j1, gt[wave]);
+  else March<CT, VT>::template run<true>(r, cur, s, j0, j1, gt[wave]);
 }
 
 }  // namespace
@@ -906,6 +1236,33 @@ void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int m
   EP2D_HIP(hipGetLastError());
 }
 
+int march_segments(int nX, int nY, int rows, int rve, int4* out) {
+  if (rows < 1) rows = 1;
+  const int ns = (nX + kMarchCols - 1) / kMarchCols;
+  int n = 0;
+  for (int k = 0; k < ns; k++)
+    for (int j0 = 0; j0 < nY; j0 += rows) {
+      if (out) out[n] = make_int4(rve, kMarchCols * k - 2, j0, std::min(j0 + rows, nY));
+      n++;
+    }
+  return n;
+}
+
+void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
+                  hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (nsegs <= 0) return;
+  const dim3 grid((nsegs + 3) / 4), block(256);
+  auto go = [&](auto kern) {
+    if (e0)
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity);
+    else
+      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity);
+  };
+  if (mode == 2) go(k_march<float, float>);  // split: FP32 stress and velocity
+  else go(k_march<double, double>);
+  EP2D_HIP(hipGetLastError());
+}
+
 const char* fused_kernel_name(int mode) {
   switch (mode) {
     case 0: return "k_plain_disp";
@@ -951,8 +1308,12 @@ void launch_gather(const double* const* bases, const int2* table, int n, int ucu
 
 void launch_copy(const double* src, double* dst, size_t n, hipStream_t s) {
   const size_t n2 = n / 2;
-  const int blocks = (int)std::min<size_t>((n2 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s, (const double2*)src, (double2*)dst, n2);
+  // one 16-byte element per thread, no grid-stride loop: the form that
+  // reaches the highest copy bandwidth on MI355X
+  const size_t blocks = (n2 + 255) / 256;
+  if (blocks == 0 || blocks > 0x7fffffffu) return;  // callers bound the size
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (const double2*)src,
+                     (double2*)dst, n2);
   EP2D_HIP(hipGetLastError());
 }
 

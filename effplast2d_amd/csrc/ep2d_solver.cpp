@@ -214,6 +214,23 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (ty != 0 && !valid_tile_rows(ty)) throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
     if (ty == 0) ty = default_tile_rows(mode_, nX);
     fplan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
+    if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
+    if (opt_.marchRows > 0) {
+      // row-marching kernel: the parameter table and the segment list live
+      // on the device (the kernel reads the current buffer as parity ^ cur)
+      marchRows_ = opt_.marchRows;
+      nSegs_ = march_segments(nX, nY, marchRows_, 0, nullptr);
+      std::vector<int4> segs((size_t)nSegs_);
+      march_segments(nX, nY, marchRows_, 0, segs.data());
+      dRve_ = (RveDev*)alloc(sizeof(RveDev));
+      dSegs_ = (int4*)alloc(segs.size() * sizeof(int4));
+      RveDev rd = r_;
+      rd.cur = 0;
+      EP2D_HIP(hipMemcpyAsync(dRve_, &rd, sizeof(RveDev), hipMemcpyHostToDevice, stream_));
+      EP2D_HIP(hipMemcpyAsync(dSegs_, segs.data(), segs.size() * sizeof(int4),
+                              hipMemcpyHostToDevice, stream_));
+      EP2D_HIP(hipStreamSynchronize(stream_));
+    }
   }
   EP2D_HIP(hipStreamSynchronize(stream_));
 
@@ -354,8 +371,12 @@ void Solver::launchIteration() {
     launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_, timed ? ev.first : nullptr,
                       timed ? ev.second : nullptr);
   } else {
-    launch_fused(r_, cur_, mode_, plastic, fplan_, stream_, timed ? ev.first : nullptr,
-                 timed ? ev.second : nullptr);
+    if (marchRows_ > 0)
+      launch_march(dRve_, dSegs_, nSegs_, mode_, cur_, stream_, timed ? ev.first : nullptr,
+                   timed ? ev.second : nullptr);
+    else
+      launch_fused(r_, cur_, mode_, plastic, fplan_, stream_, timed ? ev.first : nullptr,
+                   timed ? ev.second : nullptr);
     cur_ ^= 1;
     stressStale_ = true;
   }
@@ -400,7 +421,10 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   if (name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
-    if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(fplan_.ty) + " grid=" +
+    if (marchRows_ > 0)
+      nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march<float, float>" : "k_march<double, double>") +
+           " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs_);
+    else if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(fplan_.ty) + " grid=" +
                                         std::to_string(fplan_.grid) + " tiles=" +
                                         std::to_string(fplan_.ntiles) + " waves=" +
                                         std::to_string(fplan_.waves) + " blocks/CU=" +

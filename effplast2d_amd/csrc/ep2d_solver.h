@@ -116,6 +116,9 @@ class Solver {
   RveDev r_{};
   StressDev f_{};
   FusedPlan fplan_{};
+  int marchRows_ = 0, nSegs_ = 0;   // row-marching kernel when marchRows_ > 0
+  RveDev* dRve_ = nullptr;
+  int4* dSegs_ = nullptr;
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;

@@ -102,6 +102,7 @@ typedef struct ep2d_options {
   const char* logPath;  /* non-NULL: append the reference's log lines here
                            (EffPlast2D.log, :1607) */
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
+  int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
@@ -233,7 +234,9 @@ size_t ep2d_device_bytes(ep2d_handle* h);
 typedef struct ep2d_batch_options {
   int mode;              /* EP2D_MODE_FP64 or EP2D_MODE_SPLIT                 */
   long checkInterval;    /* default residual check interval of requests         */
-  int tileRows;          /* fused tile height (0: tuned on the running batch)   */
+  int tileRows;          /* fixed tile kernel of this height (6, 8, 14, 30)   */
+  int marchRows;         /* fixed row-marching kernel, segment rows (> 0)      */
+                         /* both 0: kernel and shape tuned on the running batch */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
@@ -273,9 +276,12 @@ typedef struct ep2d_batch_stats {
   long long timedLaunches;
   int active, queued;
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
-  int tileRows;                   /* tile height in use                       */
-  int tunings;                    /* tile-height tunings run                  */
-  double tileUs[4];               /* last tuning: mean launch us, rows 6/8/14/30 */
+  int kernel;                     /* kernel in use: 1 tiles, 2 row marching   */
+  int kernelRows;                 /* its tile height or segment rows          */
+  int tunings;                    /* kernel tunings run                       */
+  double candUs[8];               /* last tuning, mean launch us: tiles of 6, 8,
+                                     14, 30 rows; marching segments of 16, 32,
+                                     64, 128 rows                             */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;
@@ -287,11 +293,11 @@ void ep2d_batch_destroy(ep2d_batch* b);
  * immediately; device errors are reported in the request's result. */
 int ep2d_batch_submit(ep2d_batch* b, const ep2d_request* rq, long long* id);
 /* One chunk: admit, run every admitted RVE to its next residual check (the
- * nearest one over the batch), check, post-process, retire, admit.  With
- * tileRows 0 the first chunks after the batch's cell count changes by more
- * than a quarter time each tile height on real iterations (tiling does not
- * change results) and keep the fastest.  Returns the number admitted and
- * queued. */
+ * nearest one over the batch), check, post-process, retire, admit.  Unless a
+ * kernel is fixed, the first chunk after the batch's cell count changes by
+ * more than a quarter times each kernel shape on real iterations (the shape
+ * does not change results) and keeps the fastest.  Returns the number
+ * admitted and queued. */
 int ep2d_batch_step(ep2d_batch* b, int* nActive, int* nQueued);
 /* Step until nothing is queued or admitted. */
 int ep2d_batch_run(ep2d_batch* b);

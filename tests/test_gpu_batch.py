@@ -111,18 +111,29 @@ def test_failed_request_reports_and_others_finish():
     assert res[1]["status"] != 0 and res[1]["message"]
 
 
-def test_tuned_tile_height_equals_fixed():
-    """Tile height is tuned on real iterations of the running batch; tiling
-    does not change results."""
-    _note("tuned vs fixed tile rows")
+def test_tuned_kernel_equals_fixed():
+    """The kernel shape is tuned on real iterations of the running batch; the
+    shape (tile kernel or row-marching kernel, any size) does not change
+    results."""
+    _note("tuned vs fixed kernel shapes")
     tuned, st = _batch(REQS, "split")
-    assert st["tunings"] >= 1 and st["tileRows"] in (6, 8, 14, 30)
-    assert all(u > 0 for u in st["tileUs"])
-    for ty in (6, 30):
-        fixed, st2 = _batch(REQS, "split", tileRows=ty)
-        assert st2["tunings"] == 0 and st2["tileRows"] == ty
+    assert st["tunings"] >= 1 and st["kernel"] in (1, 2)
+    assert all(u > 0 for u in st["candUs"])
+    for kw in ({"tileRows": 6}, {"tileRows": 30}, {"marchRows": 16}, {"marchRows": 128}):
+        fixed, st2 = _batch(REQS, "split", **kw)
+        assert st2["tunings"] == 0
+        assert st2["kernelRows"] == list(kw.values())[0]
         for t in range(len(REQS)):
             _same(tuned[t]["result"], fixed[t]["result"])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+def test_march_batch_equals_single(mode):
+    _note(f"march batch {mode}")
+    got, st = _batch(REQS, mode, marchRows=32)
+    assert st["kernel"] == 2
+    for t, rq in enumerate(REQS):
+        _same(got[t]["result"], _single(rq, mode))
 
 
 def test_request_check_interval():

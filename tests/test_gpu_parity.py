@@ -100,6 +100,49 @@ def test_tile_heights_match_oracle(mode, ty):
                 assert e <= (1e-6 if k in ("Ux", "Uy") else 1e-4), f"{k}: {e:.3e}"
 
 
+@pytest.mark.parametrize("rows", [1, 2, 3, 16, 64])
+@pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (125, 3), (8, 1)])
+def test_march_matches_oracle(rows, nX, nPores):
+    """Row-marching kernel, FP64: every segment length (1 row: every segment
+    is all halo) on grids with one and several strips, edge and interior
+    segments, bit for bit against the oracle."""
+    nl, iters = 3, 12
+    _note(f"march fp64 rows={rows} nX={nX}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.03)
+    init = random_state(nX, 300 + rows + nX, pa=pa, boundary_v=True)
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode="fp64", marchRows=rows) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS:
+            got = g.get(k)
+            e = rel_err(got, s[k])
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
+
+
+@pytest.mark.parametrize("rows", [2, 32])
+@pytest.mark.parametrize("nl", [1, 3])
+def test_march_split_equals_tiles(rows, nl):
+    """The split scheme through the row-marching kernel and through the tile
+    kernel: the same expressions, so identical fields."""
+    nX, iters = 190, 20
+    _note(f"march split rows={rows} nl={nl}")
+    pa = make_pa(nX, nPores=2, porosity=0.04)
+    init = random_state(nX, 400 + rows, pa=pa, boundary_v=True)
+    out = []
+    for kw in ({"marchRows": rows}, {"tileRows": 8}):
+        with EffPlast2D(pa, nX, nTasks=nl, mode="split", **kw) as g:
+            for k in STATE_FIELDS:
+                g.set(k, init[k])
+            g.iterate(iters)
+            out.append({k: g.get(k) for k in STATE_FIELDS})
+    for k in STATE_FIELDS:
+        assert np.array_equal(out[0][k], out[1][k]), \
+            f"{k}: {np.count_nonzero(out[0][k] != out[1][k])} differ"
+
+
 @pytest.mark.parametrize("nX,nPores,nl,iters,seed", [(64, 1, 3, 10, 11), (130, 3, 3, 40, 12),
                                                      (96, 2, 1, 25, 13)])
 def test_split_iterations_close_to_oracle(nX, nPores, nl, iters, seed):
