@@ -385,7 +385,7 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib, batch=0):
+def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
     preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
     over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
@@ -398,9 +398,10 @@ def probe(mode, nx, iters, copy_mib, batch=0):
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
-        run_batch(reqs, mode, 10 ** 9, timing_every=0, tileRows=8, useGraphs=False)
+        shape = {"marchRows": march} if march else {"tileRows": tile or 8}
+        run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False, **shape)
     else:
-        with EffPlast2D(pa, nx, nTasks=3, mode=mode) as g:
+        with EffPlast2D(pa, nx, nTasks=3, mode=mode, tileRows=tile, marchRows=march) as g:
             g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
             g.iterate(iters)
             g.sync()
@@ -567,6 +568,8 @@ def main():
     ap.add_argument("--probe-iters", type=int, default=200)
     ap.add_argument("--probe-batch", type=int, default=0,
                     help="probe the batched kernel with this many headline RVEs")
+    ap.add_argument("--probe-tile", type=int, default=0, help="probe: tile height")
+    ap.add_argument("--probe-march", type=int, default=0, help="probe: marching segment rows")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
@@ -592,7 +595,8 @@ def main():
               "part of this build", file=sys.stderr)
         return 2
     if a.probe:
-        probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch)
+        probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
+              a.probe_march)
         return 0
     if a.sweep:
         shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]

@@ -112,8 +112,10 @@ const char* fused_kernel_name(int mode);
 // Row-marching fused iteration: one wave64 per segment (rve, s, j0, j1), a
 // strip of 64 columns from s (60 output columns from s+2) and rows [j0, j1).
 constexpr int kMarchCols = 60;
-// Segments of one RVE in strip-major order, `rows` rows each (the last one of
-// a strip shorter); returns the count and fills out when it is not null.
+// Segments of one RVE, `rows` rows each (the last one of a strip shorter),
+// ordered band by band (all strips of rows [j0, j0+rows) together, so the
+// cache lines neighbouring strips share are read while both run); returns the
+// count and fills out when it is not null.
 int march_segments(int nX, int nY, int rows, int rve, int4* out);
 void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                   hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);

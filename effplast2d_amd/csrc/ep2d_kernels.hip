@@ -237,6 +237,21 @@ __device__ __forceinline__ double qdiv(double a, double d, double rd) {
   return fma(fma(-q, d, a), rd, q);
 }
 
+// Global-memory access at a 32-bit element index: the base stays in SGPRs
+// (uniform per wave) and the byte offset in one VGPR (saddr form).
+template <typename T>
+__device__ __forceinline__ T gld(const T* base, unsigned idx) {
+  using GT = const __attribute__((address_space(1))) T;
+  using GC = const __attribute__((address_space(1))) char;
+  return *(GT*)((GC*)base + idx * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
+  using GT = __attribute__((address_space(1))) T;
+  using GC = __attribute__((address_space(1))) char;
+  *(GT*)((GC*)base + idx * (unsigned)sizeof(T)) = v;
+}
+
 // Workgroup barrier that waits for LDS traffic only: global loads issued
 // before it (the next tile's prefetch) stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -317,34 +332,32 @@ struct Fused {
     const int sx = nX + 1;  // Ux row length
 
     // ---- stage: issue every load of this thread first, then fill LDS ----
-    constexpr int KB = (BY + NW - 1) / NW;   // box rows per wave
-    double bux[KB][2], buy[KB][2];
-    unsigned int bm[KB][2];
+    // The box is read in row-major order, element f = threadIdx.x + NT*q at
+    // (f % BX, f / BX), so every lane of every load instruction is used.
+    constexpr int KB = (NBOX + NT - 1) / NT;   // box elements per thread
+    double bux[KB], buy[KB];
+    unsigned int bm[KB];
 #pragma unroll
-    for (int k = 0; k < KB; k++) {
-      const int b = wave + NW * k;
-      const int gj = j0 - 2 + b;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int a = lane + 64 * h;
-        const int gi = i0 - 2 + a;
-        double ux = 0.0, uy = 0.0;
-        unsigned int m = 0;
-        if (b < BY && a < BX) {
-          if (!EDGE) {
-            ux = Uxi[gj * sx + gi];
-            uy = Uyi[gj * nX + gi];
-            m = msk[gj * nX + gi];
-          } else if (gi >= 0 && gj >= 0) {
-            if (gi <= nX && gj < nY) ux = Uxi[gj * sx + gi];
-            if (gi < nX && gj <= nY) uy = Uyi[gj * nX + gi];
-            if (gi < nX && gj < nY) m = msk[gj * nX + gi];
-          }
+    for (int q = 0; q < KB; q++) {
+      const int f = threadIdx.x + NT * q;
+      const int b = f / BX, a = f - (f / BX) * BX;
+      const int gj = j0 - 2 + b, gi = i0 - 2 + a;
+      double ux = 0.0, uy = 0.0;
+      unsigned int m = 0;
+      if (f < NBOX) {
+        if (!EDGE) {
+          ux = gld(Uxi, (unsigned)(gj * sx + gi));
+          uy = gld(Uyi, (unsigned)(gj * nX + gi));
+          m = gld(msk, (unsigned)(gj * nX + gi));
+        } else if (gi >= 0 && gj >= 0) {
+          if (gi <= nX && gj < nY) ux = gld(Uxi, (unsigned)(gj * sx + gi));
+          if (gi < nX && gj <= nY) uy = gld(Uyi, (unsigned)(gj * nX + gi));
+          if (gi < nX && gj < nY) m = gld(msk, (unsigned)(gj * nX + gi));
         }
-        bux[k][h] = ux;
-        buy[k][h] = uy;
-        bm[k][h] = m;
       }
+      bux[q] = ux;
+      buy[q] = uy;
+      bm[q] = m;
     }
     // this thread's velocities (consumed in phase 3: the latency overlaps phases 1-2)
     const VT* __restrict__ Vxi = static_cast<const VT*>(r.Vx);
@@ -356,8 +369,8 @@ struct Fused {
       vxr[k] = 0;
       vyr[k] = 0;
       if (b < TY && lane < TX && (!EDGE || (i < nX && j < nY))) {
-        vxr[k] = Vxi[j * sx + i];
-        vyr[k] = Vyi[j * nX + i];
+        vxr[k] = gld(Vxi, (unsigned)(j * sx + i));
+        vyr[k] = gld(Vyi, (unsigned)(j * nX + i));
       }
     }
     if (threadIdx.x == 0) {
@@ -365,16 +378,12 @@ struct Fused {
       for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
     }
 #pragma unroll
-    for (int k = 0; k < KB; k++) {
-      const int b = wave + NW * k;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int a = lane + 64 * h;
-        if (b < BY && a < BX) {
-          sUx[b * BX + a] = bux[k][h];
-          sUy[b * BX + a] = buy[k][h];
-          sM[b * BX + a] = (unsigned char)bm[k][h];
-        }
+    for (int q = 0; q < KB; q++) {
+      const int f = threadIdx.x + NT * q;
+      if (f < NBOX) {
+        sUx[f] = bux[q];
+        sUy[f] = buy[q];
+        sM[f] = (unsigned char)bm[q];
       }
     }
     lds_barrier();
@@ -513,18 +522,18 @@ struct Fused {
       if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
         vx = vx * vdampX + dTrho * (dx(-sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a], r) +
                                     dy(Txy(a, b + 1) - Txy(a, b), r));
-        Vx[j * sx + i] = (VT)vx;
+        gst(Vx, (unsigned)(j * sx + i), (VT)vx);
       }
       if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
         vy = vy * vdampY + dTrho * (dy(-sP[(b + 1) * S + a + 1] + sP[b * S + a + 1] + sTyy[(b + 1) * S + a + 1] - sTyy[b * S + a + 1], r) +
                                     dx(Txy(a + 1, b) - Txy(a, b), r));
-        Vy[j * nX + i] = (VT)vy;
+        gst(Vy, (unsigned)(j * nX + i), (VT)vy);
       }
-      Uxo[j * sx + i] = uxo[k] + (double)vx * r.dT;
-      Uyo[j * nX + i] = uyo[k] + (double)vy * r.dT;
+      gst(Uxo, (unsigned)(j * sx + i), uxo[k] + (double)vx * r.dT);
+      gst(Uyo, (unsigned)(j * nX + i), uyo[k] + (double)vy * r.dT);
       if (EDGE) {
-        if (i == nX - 1) Uxo[j * sx + i + 1] = uxe[k];
-        if (j == nY - 1) Uyo[(j + 1) * nX + i] = uye[k];
+        if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxe[k]);
+        if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uye[k]);
       }
     }
   }
@@ -825,10 +834,10 @@ int dispatch_fused(int mode, bool plastic, int ty, A&&... a) {
 
 // value of lane L+1 (0 in lane 63) and of lane L-1 (0 in lane 0)
 __device__ __forceinline__ int lane_next_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+  return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);  // wave_shl:1, bound_ctrl:0
 }
 __device__ __forceinline__ int lane_prev_i(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+  return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);  // wave_shr:1, bound_ctrl:0
 }
 __device__ __forceinline__ float lane_next(float v) {
   return __builtin_bit_cast(float, lane_next_i(__builtin_bit_cast(int, v)));
@@ -847,21 +856,6 @@ __device__ __forceinline__ double lane_prev(double v) {
   const unsigned lo = (unsigned)lane_prev_i((int)(unsigned)b);
   const unsigned hi = (unsigned)lane_prev_i((int)(unsigned)(b >> 32));
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
-
-// Global-memory access at a 32-bit element index: the base stays in SGPRs
-// (uniform per wave) and the byte offset in one VGPR (saddr form).
-template <typename T>
-__device__ __forceinline__ T gld(const T* base, unsigned idx) {
-  using GT = const __attribute__((address_space(1))) T;
-  using GC = const __attribute__((address_space(1))) char;
-  return *(GT*)((GC*)base + idx * (unsigned)sizeof(T));
-}
-template <typename T>
-__device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
-  using GT = __attribute__((address_space(1))) T;
-  using GC = __attribute__((address_space(1))) char;
-  *(GT*)((GC*)base + idx * (unsigned)sizeof(T)) = v;
 }
 
 template <typename CT, typename VT>
@@ -975,10 +969,13 @@ struct March {
     CT tXY_p = 0, tXY_pp = 0;                                  // trial vertex rows j-1, j-2
     CT cP_p = 0, cYY_p = 0, cXY_p = 0;                         // corrected row j-1
 
-    Ld ld = load<EDGE>(r, Uxi, Uyi, i, jb);
-    for (int j = jb; j < j1; j++) {
-      Ld nx = ld;
-      if (j + 1 < j1) nx = load<EDGE>(r, Uxi, Uyi, i, j + 1);
+    // the loads are unconditional (past the last step they re-read rows in
+    // range: interior segments end at least one row above nY-1, edge loads
+    // are guarded), so no branch joins in front of a wait
+    auto fetch = [&](int j) {
+      return load<EDGE>(r, Uxi, Uyi, i, EDGE ? j : min(j, j1 - 1));
+    };
+    auto step = [&](const int j, const Ld& ld) {
 
       // trial cell row j+1 (cell i, row j+1)
       const double ux_n = ld.ux, uy_nn = ld.uy;
@@ -1100,15 +1097,26 @@ struct March {
       cP_p = cP_c;
       cYY_p = cYY_c;
       cXY_p = cXY_c;
-      ld = nx;
+    };
+    // loads run one step ahead of the arithmetic: the wait for a step's rows
+    // leaves the next step's loads in flight
+    // (two steps per trip, each buffer refilled right after the step that
+    // reads it, so no register copy has to wait for a load)
+    Ld A = fetch(jb);
+    for (int j = jb; j < j1; j += 2) {
+      const Ld B = fetch(j + 1);
+      step(j, A);
+      if (j + 1 >= j1) break;
+      A = fetch(j + 2);
+      step(j + 1, B);
     }
   }
 };
 
 // One wave per segment (rve, s, j0, j1); four independent waves per block.
 // Blocks are dealt to the XCDs in contiguous runs of the segment list, which
-// is ordered RVE by RVE, strip by strip, so neighbouring segments (sharing
-// halo rows and columns) meet in the same L2.
+// is ordered RVE by RVE and band by band, so neighbouring segments (sharing
+// halo columns) run together and meet in the same L2.
 template <typename CT, typename VT>
 __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
                                                const int4* __restrict__ segs, int nsegs,
@@ -1240,8 +1248,8 @@ int march_segments(int nX, int nY, int rows, int rve, int4* out) {
   if (rows < 1) rows = 1;
   const int ns = (nX + kMarchCols - 1) / kMarchCols;
   int n = 0;
-  for (int k = 0; k < ns; k++)
-    for (int j0 = 0; j0 < nY; j0 += rows) {
+  for (int j0 = 0; j0 < nY; j0 += rows)
+    for (int k = 0; k < ns; k++) {
       if (out) out[n] = make_int4(rve, kMarchCols * k - 2, j0, std::min(j0 + rows, nY));
       n++;
     }
