@@ -205,6 +205,13 @@ def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
     return out
 
 
+# Iteration cap per load step of the sampled RVEs: 40 N, about five times what
+# a converging RVE of size N needs (6.3 N for the preload at 1024^2).  Some
+# sampled loads exceed the limit load (the plastic zone never stops growing)
+# and would otherwise run to the 100000-iteration default.
+ITER_CAP_PER_N = 40
+
+
 def c4_requests(n, seed):
     """Batched sweep: sizes uniform in {128,256,512,1024}, porosity U[0.005,0.05],
     nPores in {1,2,3}, Y/K0 U[1e-3,5e-3], preload magnitude U[1e-3,5e-3] along a
@@ -224,8 +231,8 @@ def c4_requests(n, seed):
         pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
                                G0=0.5, rho0=1.0, CFL=0.5)
         assert pa[9] < 20.0 / (2 * npores), "pore radius guard"
-        reqs.append(dict(pa=pa, nX=nx, nTasks=3, nIter=100_000, eIter=1e-8, initLoad=-mag,
-                         loadType=[float(x) for x in u], tag=k))
+        reqs.append(dict(pa=pa, nX=nx, nTasks=3, nIter=ITER_CAP_PER_N * nx, eIter=1e-8,
+                         initLoad=-mag, loadType=[float(x) for x in u], tag=k))
     return reqs
 
 
@@ -234,13 +241,24 @@ def run_config4(n, seed, check):
     res, st, dt = run_batch(reqs, "split", check)
     lat = [r["tDone"] - r["tSubmit"] for r in res.values()]
     cells = st["cellIterations"]
+    by_size = {}
+    for rq in reqs:
+        r = res[rq["tag"]]["result"]
+        d = by_size.setdefault(rq["nX"], {"n": 0, "converged": 0, "iters": []})
+        d["n"] += 1
+        d["converged"] += int(bool(r["converged"]))
+        d["iters"].append(int(r["totalIters"]))
+    for d in by_size.values():
+        d["mean_iters"] = sum(d["iters"]) / len(d["iters"])
+        del d["iters"]
     out = {"workload": f"{n} RVEs, sizes U{{128,256,512,1024}}, porosity U[0.005,0.05], "
                        f"nPores U{{1,2,3}}, Y U[1e-3,5e-3], preload U[1e-3,5e-3] x random unit "
                        f"(Exx,Eyy,Exy), nTasks=3, seed {seed}, split, one engine",
            "solves": n, "elapsed_s": dt, "solves_per_s": n / dt, "cell_iter_per_s": cells / dt,
            "hbm_frac_whole_run": st["bytesAlgorithmic"] / dt / 1e9 / HBM_PEAK_GBS,
            "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
-           "converged": sum(bool(r["result"]["converged"]) for r in res.values())}
+           "converged": sum(bool(r["result"]["converged"]) for r in res.values()),
+           "by_size": by_size, "iteration_cap": f"{ITER_CAP_PER_N} N per load step"}
     out.update(batch_stats_summary(st, "split"))
     log(f"[bench] config4 {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
         f"{cells / dt / 1e9:.1f} G cell-iter/s, kernel {out['achieved_GBs']:.0f} GB/s")
@@ -266,8 +284,8 @@ def c5_stream(rate, duration, seed):
         nl = int(rng.integers(1, 4))
         pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
                                G0=0.5, rho0=1.0, CFL=0.5)
-        reqs.append((t, dict(pa=pa, nX=nx, nTasks=nl, nIter=100_000, eIter=1e-8, initLoad=-mag,
-                             loadType=[float(x) for x in u], tag=len(reqs))))
+        reqs.append((t, dict(pa=pa, nX=nx, nTasks=nl, nIter=ITER_CAP_PER_N * nx, eIter=1e-8,
+                             initLoad=-mag, loadType=[float(x) for x in u], tag=len(reqs))))
     return reqs
 
 
@@ -329,7 +347,8 @@ def run_config5(rate, duration, seed, check, budget_gib):
     span_seq = t_free - stream[0][0]
     out = {"workload": f"Poisson arrivals {rate}/s for {duration} s (seed {seed}), sizes "
                        f"{{256,512,1024,2048}} w [.4,.3,.2,.1], nTasks U{{1,2,3}}, other "
-                       f"parameters as the batched sweep, split, HBM budget {budget_gib} GiB",
+                       f"parameters as the batched sweep, iteration cap {ITER_CAP_PER_N} N per "
+                       f"load step, split, HBM budget {budget_gib} GiB",
            "requests": len(stream), "solves_per_s": len(stream) / span,
            "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
            "peak_active": peak, "cell_iter_per_s": st["cellIterations"] / span,
@@ -437,9 +456,11 @@ def sweep(sizes, modes, shapes, iters):
     print(json.dumps({"sweep": rows}), flush=True)
 
 
-def summarize(outdir, dest):
+def summarize(outdir, dest, command=""):
     """Condense rocprofv3 csv output in outdir into text lines."""
-    lines = []
+    lines = [f"# rocprofv3 summary of {os.path.basename(outdir.rstrip('/'))}"]
+    if command:
+        lines.append(f"# command: {command}")
     for path in sorted(glob.glob(os.path.join(outdir, "**", "*kernel_stats.csv"), recursive=True)):
         lines.append(f"# kernel stats: {os.path.relpath(path, outdir)}")
         with open(path) as f:
@@ -448,6 +469,18 @@ def summarize(outdir, dest):
             lines.append(f"{r.get('Name', '?')[:90]:90s} calls={r.get('Calls')} "
                          f"avg_ns={float(r.get('AverageNs', 0)):.0f} "
                          f"pct={float(r.get('Percentage', 0)):.2f}")
+    for path in sorted(glob.glob(os.path.join(outdir, "**", "*kernel_trace.csv"), recursive=True)):
+        # per (kernel, grid size): launches over different batch sizes apart
+        lines.append(f"# kernel trace by grid size: {os.path.relpath(path, outdir)}")
+        agg = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                key = (r.get("Kernel_Name", "?"), int(r.get("Grid_Size_X", 0)))
+                agg.setdefault(key, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        top = sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:8]
+        for (k, g), v in top:
+            lines.append(f"{k[:90]:90s} grid_x={g} calls={len(v)} avg_ns={statistics.mean(v):.0f} "
+                         f"median_ns={statistics.median(v):.0f}")
     for path in sorted(glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"),
                                  recursive=True)):
         lines.append(f"# counters: {os.path.relpath(path, outdir)}")
@@ -557,8 +590,8 @@ def main():
     ap.add_argument("--single-steps", type=int, default=2)
     ap.add_argument("--c4-rves", type=int, default=256)
     ap.add_argument("--c4-seed", type=int, default=2024)
-    ap.add_argument("--serve-rate", type=float, default=3.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=1.5)
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--serve-budget-gib", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -587,7 +620,7 @@ def main():
                     a.summary_out, a.pmc_command)
         return 0
     if a.summarize:
-        summarize(a.summarize, a.summary_out)
+        summarize(a.summarize, a.summary_out, a.pmc_command)
         return 0
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
