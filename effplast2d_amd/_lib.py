@@ -130,6 +130,17 @@ class BatchStats(C.Structure):
         return d
 
 
+AN_ARRAYS = ["UnuAbs", "J1nu", "J2nu", "UanAbs", "errorUabs", "J1an", "J2an", "errorJ1",
+             "errorJ2"]
+AN_STATS = ["errorUabsMax", "errorUabsAvg", "errorJ1Max", "errorJ1Avg", "errorJ2Max",
+            "errorJ2Avg"]
+
+
+class AnStatic(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in AN_ARRAYS] + \
+               [(n, C.c_double) for n in AN_STATS] + [("analytic", C.c_int)]
+
+
 class Ep2dError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -168,6 +179,10 @@ SIGNATURES = {
     "ep2d_set_timing": ([C.c_void_p, C.c_long], C.c_int),
     "ep2d_get_timing": ([C.c_void_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_char_p)], C.c_int),
     "ep2d_diag_copy": ([C.c_int, C.c_size_t, C.c_int, _dp], C.c_int),
+    "ep2d_an_static_fields": ([C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
+                               C.POINTER(AnStatic)], C.c_int),
+    "ep2d_save_an_static": ([C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
+                             C.c_char_p, C.POINTER(AnStatic)], C.c_int),
     "ep2d_sync": ([C.c_void_p], C.c_int),
     "ep2d_device_bytes": ([C.c_void_p], C.c_size_t),
     "ep2d_batch_default_options": ([C.POINTER(BatchOptions)], None),

@@ -200,6 +200,22 @@ int ep2d_set_field(ep2d_handle* h, int field, const double* host, size_t count) 
   });
 }
 
+int ep2d_an_static_fields(ep2d_handle* h, double deltaP, double tauInfty,
+                          const double loadType[3], ep2d_an_static* out) {
+  return guarded([&] {
+    if (!out || !loadType) throw ep2d::ArgError("null argument");
+    S(h)->anStatic(deltaP, tauInfty, loadType, out);
+  });
+}
+
+int ep2d_save_an_static(ep2d_handle* h, double deltaP, double tauInfty,
+                        const double loadType[3], const char* dir, ep2d_an_static* stats) {
+  return guarded([&] {
+    if (!loadType || !dir) throw ep2d::ArgError("null argument");
+    S(h)->saveAnStatic(deltaP, tauInfty, loadType, dir, stats);
+  });
+}
+
 int ep2d_save_field(ep2d_handle* h, int field, const char* path) {
   return guarded([&] {
     if (!path) throw ep2d::ArgError("null path");

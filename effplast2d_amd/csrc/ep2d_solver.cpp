@@ -8,6 +8,8 @@
 // host in the reference's operation order (compiled with -ffp-contract=off).
 #include "ep2d_solver.h"
 
+#include "ep2d_analytic.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -294,18 +296,7 @@ void Solver::buildGatherTable() {
   samplesValid_ = true;
   sampleInPore_.clear();
   const size_t nUx = (size_t)(nX + 1) * nY, nUy = (size_t)nX * (nY + 1);
-  const double Lx = dX_ * (nX - 1), Ly = dY_ * (nY - 1);
-  auto inPore = [&](bool isUy, long i, long j) {   // host U zeroing test, :517-547
-    const double x = isUy ? -0.5 * dX_ * (nX - 1) + dX_ * (int)i : -0.5 * dX_ * nX + dX_ * (int)i;
-    const double y = isUy ? -0.5 * dY_ * nY + dY_ * (int)j : -0.5 * dY_ * (nY - 1) + dY_ * (int)j;
-    bool in = false;
-    for (int k = 0; k < nPores_; k++)
-      for (int l = 0; l < nPores_; l++)
-        if (std::sqrt((x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) * (x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) +
-                      (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l) * (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l)) < rad_)
-          in = true;
-    return in;
-  };
+  auto inPore = [&](bool isUy, long i, long j) { return uNodeInPore(isUy, i, j); };
   auto addSample = [&](bool isUy, size_t flat) {
     const size_t n = isUy ? nUy : nUx;
     const size_t m = isUy ? (size_t)nX : (size_t)(nX + 1);
@@ -909,6 +900,103 @@ void Solver::fieldShape(int field, int& m, int& n) const {
       m = nX_; n = nY_ + 1; return;
     default:
       throw ArgError("unknown field id " + std::to_string(field));
+  }
+}
+
+bool Solver::uNodeInPore(bool isUy, long i, long j) const {
+  const long nX = nX_, nY = nY_;
+  const double Lx = dX_ * (nX - 1), Ly = dY_ * (nY - 1);
+  const double x = isUy ? -0.5 * dX_ * (nX - 1) + dX_ * (int)i : -0.5 * dX_ * nX + dX_ * (int)i;
+  const double y = isUy ? -0.5 * dY_ * nY + dY_ * (int)j : -0.5 * dY_ * (nY - 1) + dY_ * (int)j;
+  bool in = false;
+  for (int k = 0; k < nPores_; k++)
+    for (int l = 0; l < nPores_; l++)
+      if (std::sqrt((x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) * (x - 0.5 * Lx * (1.0 - 1.0 / nPores_) + (Lx / nPores_) * k) +
+                    (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l) * (y - 0.5 * Ly * (1.0 - 1.0 / nPores_) + (Ly / nPores_) * l)) < rad_)
+        in = true;
+  return in;
+}
+
+// SaveAnStatic2D (EffPlast2D.cu:1060-1292) on the host copies of the current
+// state: the stresses as ComputeEffParams leaves them and U with the pore
+// nodes zeroed (:514-547).
+void Solver::anStatic(double deltaP, double tauInfty, const double lt[3], ep2d_an_static* out) {
+  const size_t nC = (size_t)nX_ * nY_, nV = (size_t)(nX_ - 1) * (nY_ - 1);
+  const size_t nUx = (size_t)(nX_ + 1) * nY_, nUy = (size_t)nX_ * (nY_ + 1);
+  std::vector<double> P(nC), txx(nC), tyy(nC), txy(nV), Ux(nUx), Uy(nUy);
+  getField(EP2D_FIELD_P, P.data(), nC);
+  getField(EP2D_FIELD_TAUXX, txx.data(), nC);
+  getField(EP2D_FIELD_TAUYY, tyy.data(), nC);
+  getField(EP2D_FIELD_TAUXY, txy.data(), nV);
+  getField(EP2D_FIELD_UX, Ux.data(), nUx);
+  getField(EP2D_FIELD_UY, Uy.data(), nUy);
+  for (long j = 0; j < nY_; j++)
+    for (long i = 0; i <= nX_; i++)
+      if (uNodeInPore(false, i, j)) Ux[(size_t)j * (nX_ + 1) + i] = 0.0;
+  for (long j = 0; j <= nY_; j++)
+    for (long i = 0; i < nX_; i++)
+      if (uNodeInPore(true, i, j)) Uy[(size_t)j * nX_ + i] = 0.0;
+  AnGeometry g{nX_, nY_, dX_, dY_, rad_, coh_ / std::sqrt(2.0), G0_, K0_, (int)nPores_};
+  an_static_fields(g, P.data(), txx.data(), tyy.data(), txy.data(), Ux.data(), Uy.data(), deltaP,
+                   tauInfty, lt, out);
+}
+
+void Solver::saveAnStatic(double deltaP, double tauInfty, const double lt[3],
+                          const std::string& dir, ep2d_an_static* stats) {
+  const size_t nC = (size_t)nX_ * nY_, nV = (size_t)(nX_ - 1) * (nY_ - 1);
+  const bool an = nPores_ == 1;
+  std::vector<double> Unu(nC), J1nu(nV), J2nu(nV), Uan, eU, J1an, J2an, e1, e2;
+  ep2d_an_static o;
+  std::memset(&o, 0, sizeof(o));
+  o.UnuAbs = Unu.data();
+  o.J1nu = J1nu.data();
+  o.J2nu = J2nu.data();
+  if (an) {
+    for (auto* v : {&Uan, &eU}) v->resize(nC);
+    for (auto* v : {&J1an, &J2an, &e1, &e2}) v->resize(nV);
+    o.UanAbs = Uan.data();
+    o.errorUabs = eU.data();
+    o.J1an = J1an.data();
+    o.J2an = J2an.data();
+    o.errorJ1 = e1.data();
+    o.errorJ2 = e2.data();
+  }
+  anStatic(deltaP, tauInfty, lt, &o);
+  auto write = [&](const char* name, const std::vector<double>& a) {   // SaveVector, :800-803
+    const std::string path = dir + "/" + name + "_" + std::to_string(nX_) + "_.dat";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw StatusError(EP2D_ERR_IO, "cannot create " + path);
+    const size_t w = std::fwrite(a.data(), sizeof(double), a.size(), f);
+    std::fclose(f);
+    if (w != a.size()) throw StatusError(EP2D_ERR_IO, "write failed " + path);
+  };
+  if (an) {
+    say("\n", "Uabs max error  = ", o.errorUabsMax, ", avg = ", o.errorUabsAvg, "\n",
+        "J1 max error = ", o.errorJ1Max, ", avg = ", o.errorJ1Avg, "\n", "J2 max error = ",
+        o.errorJ2Max, ", avg = ", o.errorJ2Avg, "\n");
+    write("UanAbs", Uan);
+    write("errorUabs", eU);
+    write("J1an", J1an);
+    write("J2an", J2an);
+    write("errorJ1", e1);
+    write("errorJ2", e2);
+  }
+  write("UnuAbs", Unu);
+  write("J1nu", J1nu);
+  write("J2nu", J2nu);
+  if (stats) {
+    ep2d_an_static keep = *stats;
+    *stats = o;
+    // the caller's array pointers are not ours to replace
+    stats->UnuAbs = keep.UnuAbs;
+    stats->J1nu = keep.J1nu;
+    stats->J2nu = keep.J2nu;
+    stats->UanAbs = keep.UanAbs;
+    stats->errorUabs = keep.errorUabs;
+    stats->J1an = keep.J1an;
+    stats->J2an = keep.J2an;
+    stats->errorJ1 = keep.errorJ1;
+    stats->errorJ2 = keep.errorJ2;
   }
 }
 

@@ -72,6 +72,12 @@ class Solver {
   // fields
   void fieldShape(int field, int& m, int& n) const;
   void getField(int field, double* host, size_t count);
+  // analytic static fields beside the current numerical ones (SaveAnStatic2D)
+  void anStatic(double deltaP, double tauInfty, const double loadType[3], ep2d_an_static* out);
+  void saveAnStatic(double deltaP, double tauInfty, const double loadType[3],
+                    const std::string& dir, ep2d_an_static* stats);
+  // host U zeroing test of the reference (EffPlast2D.cu:517-547)
+  bool uNodeInPore(bool isUy, long i, long j) const;
   void setField(int field, const double* host, size_t count);
   void getMask(unsigned char* host, size_t count);
 

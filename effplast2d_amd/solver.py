@@ -141,6 +141,29 @@ class EffPlast2D:
         """SaveMatrix (EffPlast2D.cu:795-799)."""
         check(lib().ep2d_save_field(self._h, L.FIELD_ID[name], str(path).encode()))
 
+    def an_static(self, deltaP, tauInfty, loadType):
+        """SaveAnStatic2D's fields (EffPlast2D.cu:1060-1292) for the current
+        state, as arrays, with the error statistics; no files."""
+        nC, nV = self.nX * self.nY, (self.nX - 1) * (self.nY - 1)
+        arrs = {n: np.zeros(nC if n in ("UnuAbs", "UanAbs", "errorUabs") else nV)
+                for n in L.AN_ARRAYS}
+        o = L.AnStatic(**{n: _dptr(a) for n, a in arrs.items()})
+        lt = np.ascontiguousarray(loadType, dtype=np.float64)
+        check(lib().ep2d_an_static_fields(self._h, float(deltaP), float(tauInfty), _dptr(lt),
+                                          C.byref(o)))
+        out = dict(arrs)
+        out.update({n: getattr(o, n) for n in L.AN_STATS})
+        out["analytic"] = o.analytic
+        return out
+
+    def save_an_static(self, deltaP, tauInfty, loadType, directory):
+        """The same written as the reference's data/<name>_<nX>_.dat files."""
+        o = L.AnStatic()
+        lt = np.ascontiguousarray(loadType, dtype=np.float64)
+        check(lib().ep2d_save_an_static(self._h, float(deltaP), float(tauInfty), _dptr(lt),
+                                        str(directory).encode(), C.byref(o)))
+        return {n: getattr(o, n) for n in L.AN_STATS} | {"analytic": o.analytic}
+
     def mask(self):
         out = np.empty(self.nX * self.nY, dtype=np.uint8)
         check(lib().ep2d_get_mask(self._h, out.ctypes.data_as(C.POINTER(C.c_ubyte)), out.size))

@@ -10,7 +10,10 @@
  * names and defaults and become the runtime options of the solver; the
  * parameters are read from data/pa.dat as in ReadParams
  * (EffPlast2D.cu:700-707); the fields Pc and tauXXc are written to
- * data/<name>_<32*NGRID>_.dat after ComputeEffModuli (EffPlast2D.cu:214-215).
+ * data/<name>_<32*NGRID>_.dat after ComputeEffModuli (EffPlast2D.cu:214-215)
+ * and, as SaveAnStatic2D does (EffPlast2D.cu:210-212, 301-303), the
+ * analytic static fields UnuAbs, J1nu, J2nu (and with one pore UanAbs,
+ * errorUabs, J1an, J2an, errorJ1, errorJ2).
  *
  * Differences: device errors throw std::runtime_error instead of exit(-1)
  * (EffPlast2D.h:42-49); the object is not copyable; the results are also
@@ -55,7 +58,8 @@
 
 class EffPlast2D {
  public:
-  explicit EffPlast2D(const unsigned int nTimeSteps = 1) : h_(nullptr) {
+  explicit EffPlast2D(const unsigned int nTimeSteps = 1)
+      : h_(nullptr), nTasks_(NL), nTimeSteps_(nTimeSteps) {
     static_assert(NPARS == EP2D_NPARS, "pa.dat holds 11 parameters");
     double pa[EP2D_NPARS];
     if (ep2d_read_pa("data/pa.dat", pa) != EP2D_OK)
@@ -74,7 +78,8 @@ class EffPlast2D {
     check(ep2d_create(pa, 32 * NGRID, 32 * NGRID, &o, &h_));
   }
   /* Extension: configure from the physical inputs at run time. */
-  EffPlast2D(const ep2d_physics& phys, const ep2d_options& opt) : h_(nullptr) {
+  EffPlast2D(const ep2d_physics& phys, const ep2d_options& opt)
+      : h_(nullptr), nTasks_(opt.nTasks), nTimeSteps_((unsigned)opt.nTimeSteps) {
     check(ep2d_create_from_physics(&phys, &opt, &h_));
   }
   ~EffPlast2D() { ep2d_destroy(h_); }
@@ -84,6 +89,7 @@ class EffPlast2D {
   double ComputeEffModuli(double initLoadValue, const std::array<double, 3>& loadType,
                           double loadValue = 0.0) {
     check(ep2d_compute_eff_moduli(h_, initLoadValue, loadType.data(), loadValue, &res_));
+    if (nTasks_ > 1 && nTimeSteps_) saveAnStatic(loadType);
     save(EP2D_FIELD_P, "data/Pc");
     save(EP2D_FIELD_TAUXX, "data/tauXXc");
     return 0.0;
@@ -91,6 +97,7 @@ class EffPlast2D {
   double ComputeEffDamping(double initLoadValue, const std::array<double, 3>& loadType,
                            double loadValue = 0.0) {
     check(ep2d_compute_eff_damping(h_, initLoadValue, loadType.data(), loadValue, &res_));
+    if (nTimeSteps_) saveAnStatic(loadType);
     return 0.0;
   }
 
@@ -113,7 +120,15 @@ class EffPlast2D {
     check(ep2d_field_shape(h_, EP2D_FIELD_P, &m, &n));
     check(ep2d_save_field(h_, field, (stem + "_" + std::to_string(m) + "_.dat").c_str()));
   }
+  // SaveAnStatic2D with deltaP and tauInfty of load step 0, last time step
+  void saveAnStatic(const std::array<double, 3>& loadType) {
+    const std::vector<ep2d_record> r = records();
+    const ep2d_record& s0 = r.at(nTimeSteps_ - 1);
+    check(ep2d_save_an_static(h_, s0.deltaP, s0.tauInfty, loadType.data(), "data", nullptr));
+  }
   ep2d_handle* h_;
+  int nTasks_;
+  unsigned nTimeSteps_;
   ep2d_result res_{};
 };
 

@@ -217,6 +217,28 @@ int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples,
  * times on `device` and reports the mean kernel time.  Used to calibrate HBM
  * traffic counters and as the achievable-bandwidth reference. */
 int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs);
+/* Analytic static solution of the single-pore problem beside the numerical
+ * fields of the current state (SaveAnStatic2D, EffPlast2D.cu:1060-1292; the
+ * reference calls it after ComputeEffModuli with nTasks > 1 and after
+ * ComputeEffDamping, with deltaP and tauInfty of load step 0, last time
+ * step).  Caller-owned arrays, any may be NULL: nX*nY for UnuAbs, UanAbs,
+ * errorUabs; (nX-1)*(nY-1) for the J arrays.  The analytic fields and the
+ * error statistics need nPores == 1 (analytic = 1). */
+typedef struct ep2d_an_static {
+  double *UnuAbs, *J1nu, *J2nu;
+  double *UanAbs, *errorUabs, *J1an, *J2an, *errorJ1, *errorJ2;
+  double errorUabsMax, errorUabsAvg, errorJ1Max, errorJ1Avg, errorJ2Max, errorJ2Avg;
+  int analytic;
+} ep2d_an_static;
+int ep2d_an_static_fields(ep2d_handle* h, double deltaP, double tauInfty,
+                          const double loadType[3], ep2d_an_static* out);
+/* The same, written as the reference's files <dir>/<name>_<nX>_.dat (UnuAbs,
+ * J1nu, J2nu; with one pore also UanAbs, errorUabs, J1an, J2an, errorJ1,
+ * errorJ2) and the error lines printed/logged as the reference does;
+ * stats may be NULL. */
+int ep2d_save_an_static(ep2d_handle* h, double deltaP, double tauInfty,
+                        const double loadType[3], const char* dir, ep2d_an_static* stats);
+
 /* Synchronise the handle's stream. */
 int ep2d_sync(ep2d_handle* h);
 /* Device bytes held by the handle. */

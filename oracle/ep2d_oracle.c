@@ -716,3 +716,213 @@ int ora_compute_eff_damping(const ora_config* c, ora_fields* f,
   }
   return 0;
 }
+
+/* ---- analytic static fields, EffPlast2D.cu:988-1292 ---------------------- */
+#include <complex.h>
+#include <float.h>
+#include <stdlib.h>
+
+/* getAnalytic, :999-1015: conformal map of the elliptic plastic-zone boundary
+ * and the complex potentials Phi, Psi of the elastic zone. */
+static void an_maps(double x, double y, double xi, double kappa, double c0, double Y, double rad,
+                    double complex* zeta, double complex* w, double complex* dw,
+                    double complex* wv, double complex* Phi, double complex* Psi) {
+  const double complex z = x + I * y;
+  double sgn = x > 0.0 ? 1.0 : -1.0;
+  if (fabs(x) < DBL_EPSILON) sgn = 1.0;
+  *zeta = (z + sgn * csqrt(z * z + 4.0 * c0 * c0 * kappa)) / 2.0 / c0;
+  *w = c0 * (*zeta - kappa / *zeta);
+  *dw = c0 * (1.0 + kappa / (*zeta * *zeta));
+  *wv = c0 * (1.0 / *zeta - kappa * *zeta);
+  *Phi = -Y * xi / 2.0 - Y * xi * clog(*w / *zeta / rad);
+  *Psi = -Y * xi / *zeta * *wv / *dw;
+}
+
+double ora_an_ur_hydro(double r, double deltaP, double Y, double rad, double G0) {
+  return -0.5 * Y * rad * rad * exp((deltaP - Y) / Y) / (G0 * r);
+}
+
+void ora_an_u_elast(double x, double y, double tauInfty, double xi, double kappa, double c0,
+                    double Y, double rad, double G0, double K0, double* re, double* im) {
+  double complex zeta, w, dw, wv, Phi, Psi;
+  an_maps(x, y, xi, kappa, c0, Y, rad, &zeta, &w, &dw, &wv, &Phi, &Psi);
+  const double complex phi = -Y * xi * w * (clog(w / zeta / rad) + 0.5) - 2.0 * c0 * tauInfty / zeta;
+  const double complex psi = c0 * Y * xi * (1.0 / zeta + kappa * zeta);
+  const double complex dphi = Phi * dw;
+  const double complex U =
+      ((1.0 + 6.0 * G0 / (G0 + 3.0 * K0)) * phi - w / conj(dw) * conj(dphi) - conj(psi)) / 2.0 / G0;
+  *re = creal(U);
+  *im = cimag(U);
+}
+
+static double an_j1(double s11, double s22) { return 0.5 * (s11 + s22); }
+static double an_j2(double s11, double s22, double s12) {
+  return 0.5 * ((s11 - s22) * (s11 - s22) + 4.0 * s12 * s12);
+}
+
+void ora_an_j_elast(double x, double y, double xi, double kappa, double c0, double Y,
+                    double rad, double* J1, double* J2) {
+  double complex zeta, w, dw, wv, Phi, Psi;
+  an_maps(x, y, xi, kappa, c0, Y, rad, &zeta, &w, &dw, &wv, &Phi, &Psi);
+  const double complex z = x + I * y;
+  const double complex dPhi = -2.0 * xi * Y * kappa / zeta / (zeta * zeta - kappa);
+  const double complex F = 2.0 * (conj(w) / dw * dPhi + Psi) / cexp(-2.0 * carg(z) * I);
+  const double Srr = 2.0 * creal(Phi) - creal(F) / 2.0;
+  const double Sff = 2.0 * creal(Phi) + creal(F) / 2.0;
+  const double Srf = cimag(F) / 2.0;
+  *J1 = an_j1(Srr, Sff);
+  *J2 = an_j2(Srr, Sff, Srf);
+}
+
+void ora_an_j_plast(double r, double xi, double Y, double rad, double* J1, double* J2) {
+  const double Srr = -2.0 * xi * Y * log(r / rad);
+  const double Sff = -2.0 * xi * Y * (1.0 + log(r / rad));
+  *J1 = an_j1(Srr, Sff);
+  *J2 = an_j2(Srr, Sff, 0.0);
+}
+
+/* SaveAnStatic2D, :1060-1292 (fields and error statistics, no files). */
+int ora_an_static(const ora_config* c, const ora_fields* f, double deltaP, double tauInfty,
+                  const double lt[3], ora_an_out* o) {
+  const long nX = c->nX, nY = c->nY;
+  const double dX = c->pa[ORA_PA_DX], dY = c->pa[ORA_PA_DY];
+  const double K0 = c->pa[ORA_PA_K0], G0 = c->pa[ORA_PA_G0];
+  const double rad = c->pa[ORA_PA_RAD];
+  const int nPores = (int)c->pa[ORA_PA_NPORES];
+  const double Y = c->pa[ORA_PA_COH] / sqrt(2.0);
+  const int hydro = fabs(lt[0] - lt[1]) < DBL_EPSILON && fabs(lt[2]) < DBL_EPSILON;
+  const double Rmin = rad;
+  const double Rmax = 0.5 * dX * (nX - 1) - dX * 20.0;
+  const double eps = 1.0e-18;
+  const double xi = deltaP > 0.0 ? 1.0 : -1.0;
+  const double kappa = tauInfty / Y * xi;
+  const double c0 = rad * exp(fabs(deltaP) / 2.0 / Y - 0.5);
+  const double rx = rad + ora_an_ur_hydro(rad, deltaP, Y, rad, G0), ry = rx;
+  const double Rx = c0 * (1.0 - kappa), Ry = c0 * (1.0 + kappa);
+  const int an = nPores == 1;
+
+  /* host copies of U with the pore nodes zeroed (:514-547) */
+  const long nUx = (nX + 1) * nY, nUy = nX * (nY + 1);
+  double* Ux = (double*)malloc(sizeof(double) * nUx);
+  double* Uy = (double*)malloc(sizeof(double) * nUy);
+  for (long j = 0; j < nY; j++)
+    for (long i = 0; i < nX + 1; i++)
+      Ux[j * (nX + 1) + i] = ora_pore_in_u_node(c, 0, i, j) ? 0.0 : f->Ux[j * (nX + 1) + i];
+  for (long j = 0; j < nY + 1; j++)
+    for (long i = 0; i < nX; i++)
+      Uy[j * nX + i] = ora_pore_in_u_node(c, 1, i, j) ? 0.0 : f->Uy[j * nX + i];
+
+  double eUmax = 0.0, eUavg = 0.0, eJ1max = 0.0, eJ1avg = 0.0, eJ2max = 0.0, eJ2avg = 0.0;
+  size_t nU = 0, nJ = 0;
+#define CUT(e) do { if (fabs(e) > 0.5) (e) = -0.5; } while (0)
+  for (long i = 0; i < nX; i++) {
+    for (long j = 0; j < nY; j++) {
+      const double ux = 0.5 * (Ux[(nX + 1) * j + i] + Ux[(nX + 1) * j + (i + 1)]);
+      const double uy = 0.5 * (Uy[nX * j + i] + Uy[nX * (j + 1) + i]);
+      const double unu = sqrt(ux * ux + uy * uy);
+      if (o->UnuAbs) o->UnuAbs[j * nX + i] = unu;
+      double j1nu = 0.0, j2nu = 0.0;
+      const long v = j * (nX - 1) + i;
+      if (i < nX - 1 && j < nY - 1) {
+        const double* P = f->P;
+        const double* txx = f->tauXX;
+        const double* tyy = f->tauYY;
+        const double Sxx = 0.25 * (-P[j * nX + i] + txx[j * nX + i] + -P[j * nX + (i + 1)] +
+                                   txx[j * nX + (i + 1)] + -P[(j + 1) * nX + i] +
+                                   txx[(j + 1) * nX + i] + -P[(j + 1) * nX + (i + 1)] +
+                                   txx[(j + 1) * nX + (i + 1)]);
+        const double Syy = 0.25 * (-P[j * nX + i] + tyy[j * nX + i] + -P[j * nX + (i + 1)] +
+                                   tyy[j * nX + (i + 1)] + -P[(j + 1) * nX + i] +
+                                   tyy[(j + 1) * nX + i] + -P[(j + 1) * nX + (i + 1)] +
+                                   tyy[(j + 1) * nX + (i + 1)]);
+        const double Sxy = f->tauXY[v];
+        j1nu = an_j1(Sxx, Syy);
+        j2nu = an_j2(Sxx, Syy, Sxy);
+        if (o->J1nu) o->J1nu[v] = j1nu;
+        if (o->J2nu) o->J2nu[v] = j2nu;
+      }
+      if (!an) continue;
+      {
+        const double x = -0.5 * dX * (nX - 1) + dX * i;
+        const double y = -0.5 * dY * (nY - 1) + dY * j;
+        const double r = sqrt(x * x + y * y);
+        double uan;
+        if (x * x / (Rx * Rx) + y * y / (Ry * Ry) > 1.0) {
+          if (hydro) {
+            uan = fabs(ora_an_ur_hydro(r, deltaP, Y, rad, G0));
+          } else {
+            double re, im;
+            ora_an_u_elast(x, y, tauInfty, xi, kappa, c0, Y, rad, G0, K0, &re, &im);
+            uan = cabs(re + I * im);
+          }
+        } else if (x * x / (rx * rx) + y * y / (ry * ry) > 1.0) {
+          uan = fabs(ora_an_ur_hydro(r, deltaP, Y, rad, G0));
+        } else {
+          uan = 0.0;
+        }
+        double eu = 0.0;
+        if (x * x + y * y > Rmin * Rmin && x * x + y * y < Rmax * Rmax && fabs(unu) > eps) {
+          eu = fabs((uan - unu) / uan);
+          eUmax = fmax(eUmax, eu);
+          eUavg += eu;
+          nU++;
+          CUT(eu);
+        }
+        if (o->UanAbs) o->UanAbs[j * nX + i] = uan;
+        if (o->errorUabs) o->errorUabs[j * nX + i] = eu;
+      }
+      if (i < nX - 1 && j < nY - 1) {
+        const double x = -0.5 * dX * (nX - 1) + dX * i + 0.5 * dX;
+        const double y = -0.5 * dY * (nY - 1) + dY * j + 0.5 * dY;
+        const double r = sqrt(x * x + y * y);
+        double j1an, j2an;
+        if (x * x / (Rx * Rx) + y * y / (Ry * Ry) > 1.0) {
+          if (hydro) {
+            const double relR = rad / r;
+            const double Srr = -deltaP + relR * relR * Y * exp(deltaP / Y - 1);
+            const double Sff = -deltaP - relR * relR * Y * exp(deltaP / Y - 1);
+            j1an = an_j1(Srr, Sff);
+            j2an = an_j2(Srr, Sff, 0.0);
+          } else {
+            ora_an_j_elast(x, y, xi, kappa, c0, Y, rad, &j1an, &j2an);
+          }
+        } else if (x * x / (rx * rx) + y * y / (ry * ry) > 1.0) {
+          ora_an_j_plast(r, xi, Y, rad, &j1an, &j2an);
+        } else {
+          j1an = 0.0;
+          j2an = 0.0;
+        }
+        double e1 = 0.0, e2 = 0.0;
+        if (x * x + y * y > Rmin * Rmin && x * x + y * y < Rmax * Rmax) {
+          if (fabs(j1nu) > eps) {
+            e1 = fabs((j1an - j1nu) / j1an);
+            eJ1max = fmax(eJ1max, e1);
+            eJ1avg += e1;
+            CUT(e1);
+          }
+          if (fabs(j2nu) > eps) {
+            e2 = fabs((j2an - j2nu) / j2an);
+            eJ2max = fmax(eJ2max, e2);
+            eJ2avg += e2;
+            CUT(e2);
+          }
+          nJ++;
+        }
+        if (o->J1an) o->J1an[v] = j1an;
+        if (o->J2an) o->J2an[v] = j2an;
+        if (o->errorJ1) o->errorJ1[v] = e1;
+        if (o->errorJ2) o->errorJ2[v] = e2;
+      }
+    }
+  }
+#undef CUT
+  free(Ux);
+  free(Uy);
+  o->errorUabsMax = eUmax;
+  o->errorUabsAvg = an ? eUavg / (double)nU : 0.0;
+  o->errorJ1Max = eJ1max;
+  o->errorJ1Avg = an ? eJ1avg / (double)nJ : 0.0;
+  o->errorJ2Max = eJ2max;
+  o->errorJ2Avg = an ? eJ2avg / (double)nJ : 0.0;
+  return an;
+}

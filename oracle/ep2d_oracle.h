@@ -115,6 +115,30 @@ double ora_tau_infty_honest(const ora_config* c, const ora_fields* f); /* :925-9
 int ora_pore_in_u_node(const ora_config* c, int isUy, long i, long j); /* :517-547 */
 int ora_hole_area(const ora_config* c, const ora_fields* f, ora_step_record* r); /* :549-620 */
 
+/* ---- analytic static fields (SaveAnStatic2D, EffPlast2D.cu:988-1292) ------
+ * Point solutions of the single-pore elastoplastic problem (complex
+ * potentials for the elastic zone around an elliptic plastic zone, the
+ * logarithmic plastic-zone stress, the hydrostatic radial displacement) and
+ * the field comparison against the numerical fields.  Y = pa[8]/sqrt(2)
+ * (EffPlast2D.cu:1572); |.| on doubles is taken as the magnitude. */
+double ora_an_ur_hydro(double r, double deltaP, double Y, double rad, double G0); /* :1030-1032 */
+void ora_an_u_elast(double x, double y, double tauInfty, double xi, double kappa, double c0,
+                    double Y, double rad, double G0, double K0, double* re, double* im); /* :1016-1029 */
+void ora_an_j_elast(double x, double y, double xi, double kappa, double c0, double Y,
+                    double rad, double* J1, double* J2);                          /* :1039-1051 */
+void ora_an_j_plast(double r, double xi, double Y, double rad, double* J1, double* J2); /* :1052-1059 */
+typedef struct {
+  double *UnuAbs, *J1nu, *J2nu;                          /* always            */
+  double *UanAbs, *errorUabs, *J1an, *J2an, *errorJ1, *errorJ2; /* nPores == 1 */
+  double errorUabsMax, errorUabsAvg, errorJ1Max, errorJ1Avg, errorJ2Max, errorJ2Avg;
+} ora_an_out;
+/* The fields of the current state (U zeroed in the pores on a copy, as the
+ * reference's host copies are at that point); arrays are caller-owned, nX*nY
+ * for U, (nX-1)*(nY-1) for J; any may be NULL.  Returns 1 when the analytic
+ * fields were computed (nPores == 1). */
+int ora_an_static(const ora_config* c, const ora_fields* f, double deltaP, double tauInfty,
+                  const double loadType[3], ora_an_out* out);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* ComputeEffParams (EffPlast2D.cu:311-698).  recs has nTimeSteps entries. */
 int ora_compute_eff_params(const ora_config* c, ora_fields* f, int step,

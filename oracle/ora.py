@@ -52,6 +52,23 @@ class StepRecord(C.Structure):
         return out
 
 
+AN_ARRAYS = ["UnuAbs", "J1nu", "J2nu", "UanAbs", "errorUabs", "J1an", "J2an", "errorJ1",
+             "errorJ2"]
+AN_STATS = ["errorUabsMax", "errorUabsAvg", "errorJ1Max", "errorJ1Avg", "errorJ2Max",
+            "errorJ2Avg"]
+
+
+class AnOut(C.Structure):
+    _fields_ = [(n, _dp) for n in AN_ARRAYS] + [(n, C.c_double) for n in AN_STATS]
+
+
+def an_point(fn, *args):
+    """Point functions of the analytic static solution (two outputs)."""
+    a, b = C.c_double(), C.c_double()
+    getattr(lib(), fn)(*[float(x) for x in args], C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
 class Moduli(C.Structure):
     _fields_ = [("Kphi", C.c_double), ("Kd", C.c_double), ("G", C.c_double)]
 
@@ -118,6 +135,17 @@ def lib():
                                               _dp, C.c_double, C.c_long,
                                               C.POINTER(StepRecord), C.POINTER(Result)]
         L.ora_compute_eff_damping.restype = C.c_int
+        L.ora_an_ur_hydro.argtypes = [C.c_double] * 5
+        L.ora_an_ur_hydro.restype = C.c_double
+        L.ora_an_u_elast.argtypes = [C.c_double] * 10 + [_dp, _dp]
+        L.ora_an_u_elast.restype = None
+        L.ora_an_j_elast.argtypes = [C.c_double] * 7 + [_dp, _dp]
+        L.ora_an_j_elast.restype = None
+        L.ora_an_j_plast.argtypes = [C.c_double] * 4 + [_dp, _dp]
+        L.ora_an_j_plast.restype = None
+        L.ora_an_static.argtypes = [C.POINTER(Config), C.POINTER(Fields), C.c_double, C.c_double,
+                                    _dp, C.POINTER(AnOut)]
+        L.ora_an_static.restype = C.c_int
         _lib = L
     return _lib
 
@@ -224,6 +252,21 @@ class State:
         if rc:
             raise ValueError("pore-edge sample outside the grid")
         return r.as_dict()
+
+    def an_static(self, deltaP, tauInfty, loadType):
+        """SaveAnStatic2D's fields and error statistics (no files)."""
+        self.sync_config()
+        nC, nV = self.nX * self.nY, (self.nX - 1) * (self.nY - 1)
+        arrs = {n: np.zeros(nC if n in ("UnuAbs", "UanAbs", "errorUabs") else nV)
+                for n in AN_ARRAYS}
+        o = AnOut(**{n: _ptr(a) for n, a in arrs.items()})
+        lt = np.asarray(loadType, dtype=np.float64)
+        an = lib().ora_an_static(C.byref(self.config), C.byref(self.fields), float(deltaP),
+                                 float(tauInfty), _ptr(lt), C.byref(o))
+        out = dict(arrs)
+        out.update({n: getattr(o, n) for n in AN_STATS})
+        out["analytic"] = int(an)
+        return out
 
     # -- drivers -------------------------------------------------------------
     def compute_eff_moduli(self, initLoad, loadType, nTimeSteps=1):

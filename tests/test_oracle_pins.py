@@ -12,7 +12,13 @@ from something the oracle does not compute itself:
   * invariants: exact linearity of the elastic scheme, mirror and transpose
     symmetry of an isotropic load on centred pores, the yield bound after the
     return mapping, zero load stays zero,
-  * the analytic elastic moduli of a dilute pore (EffPlast2D.cu:643-654).
+  * the analytic elastic moduli of a dilute pore (EffPlast2D.cu:643-654),
+  * the analytic static fields (SaveAnStatic2D, EffPlast2D.cu:988-1292): the
+    complex-potential branch reduces to the hydrostatic closed form without
+    shear, the elastic and plastic stresses agree on the elliptic plastic-zone
+    boundary, the displacement is radial without shear and keeps the load's
+    mirror symmetries with it, and the numerical-vs-analytic J1 error falls
+    with the grid.
 """
 import json
 import math
@@ -302,3 +308,81 @@ def test_driver_records_and_load_sequence():
         assert r["iters"] % 200 == 0 and r["error"] < 1e-8
     # xy shear modulus: of the order of G0, lowered by the pore
     assert 0.3 < res["Gxy"] < pa[ora.PA_G0]
+
+
+# ---- analytic static fields (SaveAnStatic2D, EffPlast2D.cu:988-1292) ----------------
+def _an_case():
+    Y = 2.5e-3
+    rad, G0, K0, dP = 1.0, 0.5, 1.0, 2.4 * 2.5e-3
+    c0 = rad * np.exp(abs(dP) / 2 / Y - 0.5)
+    return Y, rad, G0, K0, dP, c0
+
+
+def test_analytic_kappa_zero_reduces_to_closed_form():
+    """With no far-field shear (kappa = 0) the complex-potential stresses of the
+    elastic zone must equal the hydrostatic closed form (the reference's two
+    branches of SaveAnStatic2D describe the same field)."""
+    nX = 64
+    pa = make_pa(nX, nPores=1, porosity=0.01, Y=2.5e-3)
+    s = ora.State(nX=nX, nY=nX, pa=pa, nl=3, nIter=200000, eIter=1e-8, outputStep=200)
+    res, recs = s.compute_eff_moduli(-3e-3, [1.0, 1.0, 0.0], 1)
+    dP = recs[0][0]["deltaP"]
+    assert dP > 0
+    hyd = s.an_static(dP, 0.0, [1.0, 1.0, 0.0])
+    cpx = s.an_static(dP, 0.0, [1.0, 0.999, 0.0])
+    m = hyd["J1an"] != 0
+    assert m.sum() > 1000
+    for k in ("J1an", "J2an"):
+        rel = np.abs(cpx[k][m] - hyd[k][m]) / np.abs(hyd[k][m])
+        assert rel.max() <= 1e-12, (k, rel.max())
+
+
+@pytest.mark.parametrize("kappa", [0.0, 0.1, 0.25])
+def test_analytic_stress_continuous_on_plastic_boundary(kappa):
+    """J1, J2 of the elastic-zone solution and of the plastic-zone solution
+    agree on the elliptic boundary x^2/Rx^2 + y^2/Ry^2 = 1 (Rx, Ry = c0 (1 -+
+    kappa)): the elastoplastic solution is continuous there."""
+    Y, rad, G0, K0, dP, c0 = _an_case()
+    Rx, Ry = c0 * (1 - kappa), c0 * (1 + kappa)
+    for t in np.linspace(0.05, 2 * np.pi, 13):
+        x, y = Rx * np.cos(t), Ry * np.sin(t)
+        je = ora.an_point("ora_an_j_elast", x * (1 + 1e-9), y * (1 + 1e-9), 1.0, kappa, c0, Y, rad)
+        jp = ora.an_point("ora_an_j_plast", np.hypot(x, y), 1.0, Y, rad)
+        assert je[0] == pytest.approx(jp[0], rel=1e-7)
+        assert je[1] == pytest.approx(jp[1], rel=1e-7)
+
+
+def test_analytic_displacement_symmetries():
+    """kappa = 0, no shear: the displacement is radial and depends on r only;
+    with shear it keeps the mirror symmetries of the load."""
+    Y, rad, G0, K0, dP, c0 = _an_case()
+
+    def U(x, y, tau, kappa):
+        re, im = ora.an_point("ora_an_u_elast", x, y, tau, 1.0, kappa, c0, Y, rad, G0, K0)
+        return complex(re, im)
+
+    for x, y in [(5, 1), (-3, 4), (0.5, -6)]:
+        z, u = complex(x, y), U(x, y, 0.0, 0.0)
+        assert abs((u * z.conjugate()).imag) <= 1e-12 * abs(u) * abs(z)
+        assert abs(u) == pytest.approx(abs(U(abs(z), 0.0, 0.0, 0.0)), rel=1e-12)
+    for kappa in (0.1, 0.25):
+        tau = kappa * Y
+        for x, y in [(5, 1), (-3, 4), (2.5, -6)]:
+            u = U(x, y, tau, kappa)
+            assert abs(U(x, -y, tau, kappa) - u.conjugate()) <= 1e-13 * abs(u)
+            assert abs(U(-x, -y, tau, kappa) + u) <= 1e-13 * abs(u)
+            assert abs(U(-x, y, tau, kappa) + u.conjugate()) <= 1e-13 * abs(u)
+
+
+def test_analytic_j1_error_shrinks_with_resolution():
+    """Numerical mean stress against the analytic one around a converged
+    elastoplastic pore: the average relative J1 error falls with the grid."""
+    errs = []
+    for nX in (64, 128):
+        pa = make_pa(nX, nPores=1, porosity=0.01, Y=2.5e-3)
+        s = ora.State(nX=nX, nY=nX, pa=pa, nl=3, nIter=200000, eIter=1e-8, outputStep=200)
+        res, recs = s.compute_eff_moduli(-3e-3, [1.0, 1.0, 0.0], 1)
+        o = s.an_static(recs[0][0]["deltaP"], recs[0][0]["tauInfty"], [1.0, 1.0, 0.0])
+        assert o["analytic"] == 1
+        errs.append(o["errorJ1Avg"])
+    assert errs[1] < 0.5 * errs[0] and errs[1] < 0.02, errs
