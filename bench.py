@@ -135,12 +135,12 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-CAND_NAMES = ["tiles6", "tiles8", "tiles14", "tiles30", "march16", "march32", "march64",
-              "march128"]
+CAND_NAMES = ["tiles8", "tiles14", "tiles30", "march16", "march32", "persist6", "persist8",
+              "persist14"]
 
 
 def shape_name(kind, rows):
-    return f"{'tiles' if kind == 1 else 'march'}{rows}"
+    return f"{ {1: 'tiles', 2: 'march', 3: 'persist'}[kind]}{rows}"
 
 
 def batch_stats_summary(st, mode):
@@ -149,7 +149,8 @@ def batch_stats_summary(st, mode):
     gbs = st["timedBytes"] / st["timedLaunches"] / (kms * 1e-3) / 1e9 if st["timedLaunches"] \
         else float("nan")
     ct = "float" if mode == "split" else "double"
-    kern = f"k_march<{ct}, {ct}>" if st["kernel"] == 2 else f"k_fused_batch<{ct}, {ct}, TY>"
+    kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
+            3: f"k_fused_persist<{ct}, {ct}, TY>"}[st["kernel"]]
     return {"kernel": kern,
             "kernel_avg_us": kms * 1e3, "kernel_samples": st["timedLaunches"],
             "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]

@@ -89,7 +89,7 @@ class Result(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("checkInterval", C.c_long), ("tileRows", C.c_int),
-                ("marchRows", C.c_int),
+                ("marchRows", C.c_int), ("persistRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
                 ("useGraphs", C.c_int), ("timingEvery", C.c_long)]
 

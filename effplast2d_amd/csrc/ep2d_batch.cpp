@@ -22,11 +22,12 @@ namespace {
 constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
-// Kernel shapes the tuner times: the fused tile kernel at four tile heights
-// and the row-marching kernel at four segment lengths.
+// Kernel shapes the tuner times: the fused tile kernel at three tile
+// heights, the row-marching kernel at two segment lengths and the persistent
+// tile kernel at three tile heights.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 6}, {kTiles, 8}, {kTiles, 14}, {kTiles, 30},
-    {kMarch, 16}, {kMarch, 32}, {kMarch, 64}, {kMarch, 128}};
+    {kTiles, 8}, {kTiles, 14}, {kTiles, 30}, {kMarch, 16}, {kMarch, 32},
+    {kPersist, 6}, {kPersist, 8}, {kPersist, 14}};
 
 namespace {
 
@@ -39,11 +40,14 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.tileRows != 0 && !valid_tile_rows(opt_.tileRows))
     throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
   if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
-  if (opt_.tileRows != 0 && opt_.marchRows != 0)
-    throw ArgError("give tileRows or marchRows, not both");
-  fixed_ = opt_.tileRows != 0 || opt_.marchRows != 0;
+  if (opt_.persistRows != 0 && !valid_tile_rows(opt_.persistRows))
+    throw ArgError("persistRows must be 0, 6, 8, 14 or 30");
+  const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0);
+  if (nfixed > 1) throw ArgError("give at most one of tileRows, marchRows, persistRows");
+  fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
+  else if (opt_.persistRows) shape_ = {kPersist, opt_.persistRows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -237,6 +241,8 @@ void Batch::launchWork(Shape sh, const int4* work, int n, int parity, hipEvent_t
                        hipEvent_t e1) {
   if (sh.kind == kMarch)
     launch_march(dRves_, work, n, opt_.mode, parity, stream_, e0, e1);
+  else if (sh.kind == kPersist)
+    launch_fused_persist(dRves_, work, n, opt_.mode, sh.rows, parity, stream_, e0, e1);
   else
     launch_fused_batch(dRves_, work, n, opt_.mode, sh.rows, parity, stream_, e0, e1);
 }

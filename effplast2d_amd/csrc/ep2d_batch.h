@@ -44,9 +44,9 @@ class Batch {
     long launches = 0;
     int cand = -1;     // tile-height candidate under tuning, -1 for statistics
   };
-  static constexpr int kTiles = 1, kMarch = 2;
+  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3;
   struct Shape {
-    int kind = kTiles;  // fused tile kernel or row-marching kernel
+    int kind = kTiles;  // fused tile, row-marching or persistent tile kernel
     int rows = 8;       // tile height or segment rows
   };
   static constexpr int kNCand = 8;

@@ -124,6 +124,11 @@ void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int
 void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,
                         int parity, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr);
+// The same tile list through the persistent kernel (one wave of resident
+// blocks, next tile's reads in flight during the current tile's arithmetic).
+void launch_fused_persist(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,
+                        int parity, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 // residual: out[0] = bits of max|Vx|, out[1] = bits of max|Vy|, out[2] = nonfinite count
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s);
 // volume averages: partial[nblocks*NAVG] then result[NAVG]

@@ -285,6 +285,16 @@ __device__ __forceinline__ void lds_barrier() {
 // CT is the arithmetic type of stress and velocity, VT the storage type of V.
 // With CT = double every expression is the plain kernels' expression, in the
 // same order, so the fused FP64 scheme reproduces the plain scheme bit for bit.
+// One thread's global reads of a tile (the same for every Fused variant of a
+// tile shape, so the persistent kernel can fetch before it knows whether the
+// tile's RVE is plastic).
+template <typename VT, int KB, int KP>
+struct TileReads {
+  double bux[KB], buy[KB];
+  unsigned int bm[KB];
+  VT vxr[KP], vyr[KP];
+};
+
 template <typename CT, typename VT, bool PLASTIC, int TY, int NW>
 struct Fused {
   static constexpr int S = 64;                 // row stride of the per-tile arrays
@@ -313,30 +323,22 @@ struct Fused {
     else return a * (CT)(1.0 / 3.0);
   }
 
+  // Global reads of one tile: the U box and mask (row-major, element
+  // f = threadIdx.x + NT*q at (f % BX, f / BX), so every lane of every load is
+  // used) and this thread's velocities.
+  static constexpr int KB = (NBOX + NT - 1) / NT;   // box elements per thread
+  using Pre = TileReads<VT, KB, KP>;
   template <bool EDGE>
-  static __device__ __forceinline__ void run(const RveDev& r, int cur, int i0, int j0, char* smem) {
+  static __device__ __forceinline__ void fetch(const RveDev& r, int cur, int i0, int j0, Pre& p) {
     const int nX = r.nX, nY = r.nY;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* gt = reinterpret_cast<double*>(smem);
-    double* sUx = reinterpret_cast<double*>(smem + OFF_BOX);
-    double* sUy = sUx + NBOX;
-    CT* sP = reinterpret_cast<CT*>(smem + OFF_C);
-    CT* sTxx = sP + CY * S;
-    CT* sTyy = sTxx + CY * S;
-    CT* sTxy = sTyy + CY * S;
-    CT* cTxy = reinterpret_cast<CT*>(smem + OFF_BOX);  // written after phase 1
-    unsigned char* sM = reinterpret_cast<unsigned char*>(smem + OFF_M);
     const double* __restrict__ Uxi = r.Ux[cur];
     const double* __restrict__ Uyi = r.Uy[cur];
     const unsigned char* __restrict__ msk = r.mask;
     const int sx = nX + 1;  // Ux row length
-
     // ---- stage: issue every load of this thread first, then fill LDS ----
     // The box is read in row-major order, element f = threadIdx.x + NT*q at
     // (f % BX, f / BX), so every lane of every load instruction is used.
-    constexpr int KB = (NBOX + NT - 1) / NT;   // box elements per thread
-    double bux[KB], buy[KB];
-    unsigned int bm[KB];
 #pragma unroll
     for (int q = 0; q < KB; q++) {
       const int f = threadIdx.x + NT * q;
@@ -355,24 +357,49 @@ struct Fused {
           if (gi < nX && gj < nY) m = gld(msk, (unsigned)(gj * nX + gi));
         }
       }
-      bux[q] = ux;
-      buy[q] = uy;
-      bm[q] = m;
+      p.bux[q] = ux;
+      p.buy[q] = uy;
+      p.bm[q] = m;
     }
     // this thread's velocities (consumed in phase 3: the latency overlaps phases 1-2)
     const VT* __restrict__ Vxi = static_cast<const VT*>(r.Vx);
     const VT* __restrict__ Vyi = static_cast<const VT*>(r.Vy);
-    VT vxr[KP], vyr[KP];
 #pragma unroll
     for (int k = 0; k < KP; k++) {
       const int b = wave + NW * k, i = i0 + lane, j = j0 + b;
-      vxr[k] = 0;
-      vyr[k] = 0;
+      p.vxr[k] = 0;
+      p.vyr[k] = 0;
       if (b < TY && lane < TX && (!EDGE || (i < nX && j < nY))) {
-        vxr[k] = gld(Vxi, (unsigned)(j * sx + i));
-        vyr[k] = gld(Vyi, (unsigned)(j * nX + i));
+        p.vxr[k] = gld(Vxi, (unsigned)(j * sx + i));
+        p.vyr[k] = gld(Vyi, (unsigned)(j * nX + i));
       }
     }
+  }
+
+  template <bool EDGE>
+  static __device__ __forceinline__ void run(const RveDev& r, int cur, int i0, int j0, char* smem) {
+    Pre p;
+    fetch<EDGE>(r, cur, i0, j0, p);
+    compute<EDGE>(r, cur, i0, j0, smem, p);
+  }
+
+  template <bool EDGE>
+  static __device__ __forceinline__ void compute(const RveDev& r, int cur, int i0, int j0, char* smem,
+                                                 const Pre& p) {
+    const int nX = r.nX, nY = r.nY;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double* gt = reinterpret_cast<double*>(smem);
+    double* sUx = reinterpret_cast<double*>(smem + OFF_BOX);
+    double* sUy = sUx + NBOX;
+    CT* sP = reinterpret_cast<CT*>(smem + OFF_C);
+    CT* sTxx = sP + CY * S;
+    CT* sTyy = sTxx + CY * S;
+    CT* sTxy = sTyy + CY * S;
+    CT* cTxy = reinterpret_cast<CT*>(smem + OFF_BOX);  // written after phase 1
+    unsigned char* sM = reinterpret_cast<unsigned char*>(smem + OFF_M);
+    const int sx = nX + 1;  // Ux row length
+
+    // ---- stage: LDS fill (the global reads were issued by fetch) ----
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
@@ -381,9 +408,9 @@ struct Fused {
     for (int q = 0; q < KB; q++) {
       const int f = threadIdx.x + NT * q;
       if (f < NBOX) {
-        sUx[f] = bux[q];
-        sUy[f] = buy[q];
-        sM[f] = (unsigned char)bm[q];
+        sUx[f] = p.bux[q];
+        sUy[f] = p.buy[q];
+        sM[f] = (unsigned char)p.bm[q];
       }
     }
     lds_barrier();
@@ -517,8 +544,8 @@ struct Fused {
       const int i = i0 + a, j = j0 + b;
       if (b >= TY || a >= TX) continue;
       if (EDGE && (i >= nX || j >= nY)) continue;
-      CT vx = (CT)vxr[k];
-      CT vy = (CT)vyr[k];
+      CT vx = (CT)p.vxr[k];
+      CT vy = (CT)p.vyr[k];
       if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
         vx = vx * vdampX + dTrho * (dx(-sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a], r) +
                                     dy(Txy(a, b + 1) - Txy(a, b), r));
@@ -733,6 +760,64 @@ __global__ __launch_bounds__(64 * NW) void k_fused_batch(const RveDev* __restric
   }
 }
 
+// Persistent form of the batched kernel: a grid of about one wave of
+// resident blocks, each walking a run of the tile list and fetching the next
+// tile's U box, mask and velocities into registers while it computes the
+// current one, so the global-read latency hides behind arithmetic.  Blocks
+// of XCD x (blockIdx % 8) share the x-th eighth of the list (neighbouring
+// tiles, shared halos in that XCD's L2).  Two register sets alternate, so no
+// copy has to wait for a load.
+template <typename CT, typename VT, int TY, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fused_persist(const RveDev* __restrict__ rves,
+                                                          const int4* __restrict__ tiles,
+                                                          int ntiles, int parity) {
+  using FP = Fused<CT, VT, true, TY, NW>;
+  using FE = Fused<CT, VT, false, TY, NW>;
+  using Pre = typename FP::Pre;
+  static_assert(FP::SMEM == FE::SMEM, "same LDS layout");
+  __shared__ __attribute__((aligned(16))) char smem[FP::SMEM];
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b & 7, nbx = (nb >> 3) + ((b & 7) < (nb & 7) ? 1 : 0);
+  const int lo = (int)((long)ntiles * x / 8), hi = (int)((long)ntiles * (x + 1) / 8);
+  int t = lo + (b >> 3);
+  if (t >= hi) return;
+  auto fetch = [&](int tt, Pre& p) {
+    const int4 q = tiles[tt];
+    const RveDev& r = rves[q.x];
+    const int cur = parity ^ r.cur;
+    const bool interior = q.y >= 2 && q.y + FP::TX + 2 <= r.nX && q.z >= 2 && q.z + TY + 2 <= r.nY;
+    if (interior) FP::template fetch<false>(r, cur, q.y, q.z, p);
+    else FP::template fetch<true>(r, cur, q.y, q.z, p);
+  };
+  auto compute = [&](int tt, const Pre& p) {
+    const int4 q = tiles[tt];
+    const RveDev& r = rves[q.x];
+    const int cur = parity ^ r.cur;
+    const bool interior = q.y >= 2 && q.y + FP::TX + 2 <= r.nX && q.z >= 2 && q.z + TY + 2 <= r.nY;
+    if (r.plastic) {
+      if (interior) FP::template compute<false>(r, cur, q.y, q.z, smem, p);
+      else FP::template compute<true>(r, cur, q.y, q.z, smem, p);
+    } else {
+      if (interior) FE::template compute<false>(r, cur, q.y, q.z, smem, p);
+      else FE::template compute<true>(r, cur, q.y, q.z, smem, p);
+    }
+    lds_barrier();  // every wave is done with this tile's LDS
+  };
+  Pre A, B;
+  fetch(t, A);
+  for (;;) {
+    const int t1 = t + nbx;
+    if (t1 < hi) fetch(t1, B);
+    compute(t, A);
+    if (t1 >= hi) break;
+    const int t2 = t1 + nbx;
+    if (t2 < hi) fetch(t2, A);
+    compute(t1, B);
+    if (t2 >= hi) break;
+    t = t2;
+  }
+}
+
 // Tile heights and their workgroup sizes (waves per block).
 template <int TY> struct TileNW;
 template <> struct TileNW<6> { static constexpr int value = 8; };
@@ -766,6 +851,31 @@ struct BatchFn {
     else
       hipLaunchKernelGGL((k_fused_batch<CT, VT, TY, NW>), dim3(ntiles), dim3(64 * NW), 0, s,
                          rves, tiles, ntiles, parity);
+    return 0;
+  }
+};
+
+template <typename CT, typename VT, bool P, int TY>
+struct PersistFn {
+  static constexpr int NW = TileNW<TY>::value;
+  static int call(const RveDev* rves, const int4* tiles, int ntiles, int parity, hipStream_t s,
+                  hipEvent_t e0, hipEvent_t e1) {
+    static int grid = 0;  // resident blocks on this device: CUs x blocks per CU, a multiple of 8
+    if (grid == 0) {
+      int nb = 0, dev = 0, cus = 0;
+      EP2D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &nb, (const void*)k_fused_persist<CT, VT, TY, NW>, 64 * NW, 0));
+      EP2D_HIP(hipGetDevice(&dev));
+      EP2D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      grid = std::max(8, (std::max(nb, 1) * cus) / 8 * 8);
+    }
+    const int g = std::min(grid, (ntiles + 7) / 8 * 8);
+    if (e0)
+      hipExtLaunchKernelGGL((k_fused_persist<CT, VT, TY, NW>), dim3(g), dim3(64 * NW), 0, s, e0,
+                            e1, 0, rves, tiles, ntiles, parity);
+    else
+      hipLaunchKernelGGL((k_fused_persist<CT, VT, TY, NW>), dim3(g), dim3(64 * NW), 0, s, rves,
+                         tiles, ntiles, parity);
     return 0;
   }
 };
@@ -1241,6 +1351,13 @@ void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int m
                         int parity, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (ntiles <= 0) return;
   dispatch_fused<BatchFn>(mode, true, ty, rves, tiles, ntiles, parity, s, e0, e1);
+  EP2D_HIP(hipGetLastError());
+}
+
+void launch_fused_persist(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,
+                          int parity, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (ntiles <= 0) return;
+  dispatch_fused<PersistFn>(mode, true, ty, rves, tiles, ntiles, parity, s, e0, e1);
   EP2D_HIP(hipGetLastError());
 }
 

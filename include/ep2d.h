@@ -258,7 +258,8 @@ typedef struct ep2d_batch_options {
   long checkInterval;    /* default residual check interval of requests         */
   int tileRows;          /* fixed tile kernel of this height (6, 8, 14, 30)   */
   int marchRows;         /* fixed row-marching kernel, segment rows (> 0)      */
-                         /* both 0: kernel and shape tuned on the running batch */
+  int persistRows;       /* fixed persistent tile kernel of this height        */
+                         /* all 0: kernel and shape tuned on the running batch  */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
@@ -298,12 +299,13 @@ typedef struct ep2d_batch_stats {
   long long timedLaunches;
   int active, queued;
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
-  int kernel;                     /* kernel in use: 1 tiles, 2 row marching   */
+  int kernel;                     /* kernel in use: 1 tiles, 2 row marching,
+                                     3 persistent tiles                       */
   int kernelRows;                 /* its tile height or segment rows          */
   int tunings;                    /* kernel tunings run                       */
-  double candUs[8];               /* last tuning, mean launch us: tiles of 6, 8,
-                                     14, 30 rows; marching segments of 16, 32,
-                                     64, 128 rows                             */
+  double candUs[8];               /* last tuning, mean launch us: tiles of 8, 14,
+                                     30 rows; marching segments of 16, 32 rows;
+                                     persistent tiles of 6, 8, 14 rows        */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;
