@@ -842,7 +842,7 @@ void Solver::jobFinish(ep2d_result* out) {
   res.nRecords = (int)records_.size();
   res.totalIters = totalIters_;
   res.converged = allConverged_ ? 1 : 0;
-  say("\nCalculation time is ", (int)res.seconds, " sec\n");
+  printDuration((int)res.seconds);
   if (out) *out = res;
 }
 
@@ -1084,6 +1084,28 @@ void Solver::logOnly(const A&... a) {
   std::ostringstream os;
   (os << ... << a);
   log_ << os.str();
+}
+
+// printDuration, EffPlast2D.cu:1371-1409: console lines end in one newline,
+// log lines in three.
+void Solver::printDuration(int sec) {
+  std::ostringstream os;
+  os << "\nCalculation time is ";
+  int mn = sec / 60, hr = mn / 60, dy = hr / 24;
+  if (sec < 60) {
+    os << sec << " sec";
+  } else if (mn < 60) {
+    os << mn << " min " << sec % 60 << " sec";
+  } else if (hr < 24) {
+    os << hr << " hours " << mn % 60 << " min " << sec % 60 << " sec";
+  } else if (dy < 7) {
+    os << dy << " days " << hr % 24 << " hours " << mn % 60 << " min " << sec % 60 << " sec";
+  } else {
+    os << dy / 7 << " weeks " << dy % 7 << " days " << hr % 24 << " hours " << mn % 60 << " min "
+       << sec % 60 << " sec";
+  }
+  if (opt_.verbose) std::cout << os.str() << "\n";
+  if (log_.is_open()) log_ << os.str() << "\n\n\n";
 }
 
 void Solver::printStepInfo(int step) {

@@ -99,6 +99,7 @@ class Solver {
 
   // console/log output of the reference (EffPlast2D.cu:1295-1409)
   template <typename... A> void say(const A&... a);
+  void printDuration(int sec);
   template <typename... A> void logOnly(const A&... a);
   void printStepInfo(int step);
   void printCalculationType();

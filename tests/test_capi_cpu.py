@@ -159,3 +159,13 @@ def test_batch_options_and_errors():
     assert L.ep2d_batch_step(None, None, None) == -1
     assert L.ep2d_batch_submit(None, None, None) == -1
     L.ep2d_batch_destroy(None)
+
+
+def test_cli_usage_error():
+    """Missing arguments: the usage message and exit status 1 (main.cu:6-10,
+    39-41), before any device work."""
+    import subprocess
+    cli = os.path.join(ROOT, "effplast2d_amd", "ep2d_cli64")
+    r = subprocess.run([cli, "1", "2"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "ERROR:  missing arguments" in r.stderr and "<time steps>" in r.stderr
