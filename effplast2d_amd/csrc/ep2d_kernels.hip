@@ -331,7 +331,8 @@ struct Fused {
   template <bool EDGE>
   static __device__ __forceinline__ void fetch(const RveDev& r, int cur, int i0, int j0, Pre& p) {
     const int nX = r.nX, nY = r.nY;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform: row loops and row addresses stay scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double* __restrict__ Uxi = r.Ux[cur];
     const double* __restrict__ Uyi = r.Uy[cur];
     const unsigned char* __restrict__ msk = r.mask;
@@ -370,8 +371,8 @@ struct Fused {
       p.vxr[k] = 0;
       p.vyr[k] = 0;
       if (b < TY && lane < TX && (!EDGE || (i < nX && j < nY))) {
-        p.vxr[k] = gld(Vxi, (unsigned)(j * sx + i));
-        p.vyr[k] = gld(Vyi, (unsigned)(j * nX + i));
+        p.vxr[k] = gld(Vxi + (size_t)j * sx, (unsigned)i);
+        p.vyr[k] = gld(Vyi + (size_t)j * nX, (unsigned)i);
       }
     }
   }
@@ -387,8 +388,9 @@ struct Fused {
   static __device__ __forceinline__ void compute(const RveDev& r, int cur, int i0, int j0, char* smem,
                                                  const Pre& p) {
     const int nX = r.nX, nY = r.nY;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double* gt = reinterpret_cast<double*>(smem);
+    // the wave index is uniform: row loops and row addresses stay scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    CT* gt = reinterpret_cast<CT*>(smem);  // vertex G by class, in the arithmetic type
     double* sUx = reinterpret_cast<double*>(smem + OFF_BOX);
     double* sUy = sUx + NBOX;
     CT* sP = reinterpret_cast<CT*>(smem + OFF_C);
@@ -402,7 +404,7 @@ struct Fused {
     // ---- stage: LDS fill (the global reads were issued by fetch) ----
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int c = 0; c < 16; c++) gt[c] = r.gtab[c];
+      for (int c = 0; c < 16; c++) gt[c] = (CT)r.gtab[c];
     }
 #pragma unroll
     for (int q = 0; q < KB; q++) {
@@ -427,6 +429,9 @@ struct Fused {
       uye[k] = EDGE ? sUy[(bb + 3) * BX + a + 2] : 0.0;
     }
 
+    // the two values of K and G (pore or matrix), in the arithmetic type
+    const CT K0 = (CT)r.K0, Kp = (CT)r.Kp, G0 = (CT)r.G0, Gp = (CT)r.Gp;
+
     // ---- phase 1: trial stress ----
     constexpr int K1 = (CY + NW - 1) / NW;
 #pragma unroll
@@ -439,8 +444,8 @@ struct Fused {
         CT pp = 0, txx = 0, tyy = 0;
         if (!EDGE || (ci >= 0 && ci < nX && cj >= 0 && cj < nY)) {
           const unsigned char m = sM[(b + 1) * BX + a + 1];
-          const CT K = (CT)kmat(m, r.K0, r.Kp);
-          const CT G = (CT)kmat(m, r.G0, r.Gp);
+          const CT K = (m & M_CELL_MAT) ? Kp : K0;
+          const CT G = (m & M_CELL_MAT) ? Gp : G0;
           const CT exx = dx(sUx[(b + 1) * BX + a + 2] - sUx[(b + 1) * BX + a + 1], r);
           const CT eyy = dy(sUy[(b + 2) * BX + a + 1] - sUy[(b + 1) * BX + a + 1], r);
           pp = (CT)r.P0 - K * (exx + eyy);
@@ -461,7 +466,7 @@ struct Fused {
         CT txy = 0;
         if (!EDGE || (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1)) {
           const unsigned char m00 = sM[b * BX + a];
-          const CT g = (CT)gt[(m00 >> M_VCLASS_SHIFT) & 15];
+          const CT g = gt[(m00 >> M_VCLASS_SHIFT) & 15];
           txy = g * (dy(sUx[(b + 1) * BX + a + 1] - sUx[b * BX + a + 1], r) +
                      dx(sUy[(b + 1) * BX + a + 1] - sUy[(b + 1) * BX + a], r));
           if (m00 & M_VERT_ZERO) txy = 0;
@@ -549,15 +554,15 @@ struct Fused {
       if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
         vx = vx * vdampX + dTrho * (dx(-sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a], r) +
                                     dy(Txy(a, b + 1) - Txy(a, b), r));
-        gst(Vx, (unsigned)(j * sx + i), (VT)vx);
+        gst(Vx + (size_t)j * sx, (unsigned)i, (VT)vx);
       }
       if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
         vy = vy * vdampY + dTrho * (dy(-sP[(b + 1) * S + a + 1] + sP[b * S + a + 1] + sTyy[(b + 1) * S + a + 1] - sTyy[b * S + a + 1], r) +
                                     dx(Txy(a + 1, b) - Txy(a, b), r));
-        gst(Vy, (unsigned)(j * nX + i), (VT)vy);
+        gst(Vy + (size_t)j * nX, (unsigned)i, (VT)vy);
       }
-      gst(Uxo, (unsigned)(j * sx + i), uxo[k] + (double)vx * r.dT);
-      gst(Uyo, (unsigned)(j * nX + i), uyo[k] + (double)vy * r.dT);
+      gst(Uxo + (size_t)j * sx, (unsigned)i, uxo[k] + (double)vx * r.dT);
+      gst(Uyo + (size_t)j * nX, (unsigned)i, uyo[k] + (double)vy * r.dT);
       if (EDGE) {
         if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxe[k]);
         if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uye[k]);
@@ -997,11 +1002,12 @@ struct March {
     const VT* Vy = static_cast<const VT*>(r.Vy);
     Ld d;
     if (!EDGE) {
-      d.ux = gld(Ux, (unsigned)((j + 1) * sx + i));
-      d.uy = gld(Uy, (unsigned)((j + 2) * nX + i));
-      d.m = gld(r.mask, (unsigned)((j + 1) * nX + i));
-      d.vx = gld(Vx, (unsigned)(j * sx + i));
-      d.vy = gld(Vy, (unsigned)(j * nX + i));
+      // row bases are uniform (scalar arithmetic), the lane offset i is fixed
+      d.ux = gld(Ux + (size_t)(j + 1) * sx, (unsigned)i);
+      d.uy = gld(Uy + (size_t)(j + 2) * nX, (unsigned)i);
+      d.m = gld(r.mask + (size_t)(j + 1) * nX, (unsigned)i);
+      d.vx = gld(Vx + (size_t)j * sx, (unsigned)i);
+      d.vy = gld(Vy + (size_t)j * nX, (unsigned)i);
     } else {
       const bool ci = i >= 0 && i < nX;
       d.ux = (i >= 0 && i <= nX && j + 1 >= 0 && j + 1 < nY) ? gld(Ux, (unsigned)((j + 1) * sx + i)) : 0.0;
@@ -1037,7 +1043,7 @@ struct March {
   }
 
   template <bool EDGE>
-  static __device__ void run(const RveDev& r, int cur, int s, int j0, int j1, const double* gt) {
+  static __device__ void run(const RveDev& r, int cur, int s, int j0, int j1, const CT* gt) {
     const int nX = r.nX, nY = r.nY, sx = nX + 1;
     const int L = threadIdx.x & 63;
     const int i = s + L;
@@ -1116,7 +1122,7 @@ struct March {
       const double uys_n = lane_next(uy_n);
       CT tXY_c;
       {
-        const CT g = (CT)gt[(m_c >> M_VCLASS_SHIFT) & 15];
+        const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
         CT txy = g * (dy(uxs_n - uxs_c, r) + dx(uys_n - uy_n, r));
         const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
         if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
@@ -1172,15 +1178,15 @@ struct March {
         CT vx = (CT)ld.vx, vy = (CT)ld.vy;
         if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
           vx = vx * vdampX + dTrho * (dx(-cP_c + rP + cXX_c - rXX, r) + dy(rXY_c - rXY_p, r));
-          if (outLane) gst(Vx, (unsigned)(j * sx + i), (VT)vx);
+          if (outLane) gst(Vx + (size_t)j * sx, (unsigned)i, (VT)vx);
         }
         if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
           vy = vy * vdampY + dTrho * (dy(-cP_c + cP_p + cYY_c - cYY_p, r) + dx(cXY_p - rXY_p, r));
-          if (outLane) gst(Vy, (unsigned)(j * nX + i), (VT)vy);
+          if (outLane) gst(Vy + (size_t)j * nX, (unsigned)i, (VT)vy);
         }
         if (outLane && (!EDGE || j < nY)) {
-          gst(Uxo, (unsigned)(j * sx + i), ux_c + (double)vx * dT);
-          gst(Uyo, (unsigned)(j * nX + i), uy_c + (double)vy * dT);
+          gst(Uxo + (size_t)j * sx, (unsigned)i, ux_c + (double)vx * dT);
+          gst(Uyo + (size_t)j * nX, (unsigned)i, uy_c + (double)vy * dT);
           if (EDGE) {
             if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxs_c);
             if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uy_n);
@@ -1231,7 +1237,7 @@ template <typename CT, typename VT>
 __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
                                                const int4* __restrict__ segs, int nsegs,
                                                int parity) {
-  __shared__ double gt[4][16];
+  __shared__ CT gt[4][16];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int b = tile_of_block(blockIdx.x, gridDim.x);
@@ -1239,7 +1245,7 @@ __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
   if (k >= nsegs) return;
   const int4 sg = segs[k];
   const RveDev& r = rves[sg.x];
-  if (lane < 16) gt[wave][lane] = r.gtab[lane];
+  if (lane < 16) gt[wave][lane] = (CT)r.gtab[lane];
   __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
   __builtin_amdgcn_wave_barrier();
   const int cur = parity ^ r.cur;
