@@ -156,6 +156,8 @@ def batch_stats_summary(st, mode):
             "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]
             else float("nan"),
             "achieved_GBs": gbs, "launches": st["launches"], "chunks": st["chunks"],
+            "graph_avg_us_per_launch": 1e3 * st["graphMs"] / st["graphLaunches"]
+            if st["graphLaunches"] else None,
             "kernel_shape": shape_name(st["kernel"], st["kernelRows"]),
             "tuning_us": dict(zip(CAND_NAMES, st["candUs"])),
             "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"]}

@@ -120,6 +120,7 @@ class BatchStats(C.Structure):
                                             "finished", "failed", "cellIterations")] + \
                [("bytesAlgorithmic", C.c_double), ("timedMs", C.c_double),
                 ("timedBytes", C.c_double), ("timedLaunches", C.c_longlong),
+                ("graphMs", C.c_double), ("graphLaunches", C.c_longlong),
                 ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t),
                 ("kernel", C.c_int), ("kernelRows", C.c_int), ("tunings", C.c_int),
                 ("candUs", C.c_double * 8)]

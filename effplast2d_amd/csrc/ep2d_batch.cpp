@@ -340,7 +340,21 @@ void Batch::launchChunk(long c) {
     // graph of the chunk; with timing on, its last launch runs outside the
     // graph between dispatch events
     const long glen = timed ? rest - 1 : rest;
-    if (glen > 0) EP2D_HIP(hipGraphLaunch(graphFor(glen), stream_));
+    if (glen > 0) {
+      const hipGraphExec_t ge = graphFor(glen);
+      Pending gv{};
+      if (timed) {
+        gv = takeEvents();
+        gv.graph = true;
+        gv.launches = glen;
+        EP2D_HIP(hipEventRecord(gv.a, stream_));
+      }
+      EP2D_HIP(hipGraphLaunch(ge, stream_));
+      if (timed) {
+        EP2D_HIP(hipEventRecord(gv.b, stream_));
+        evPending_.push_back(gv);
+      }
+    }
     if (timed) {
       Pending ev = takeEvents();
       ev.bytes = bytes;
@@ -389,12 +403,16 @@ void Batch::resolveEvents() {
       tuneMs_[e.cand] += ms;
       tuneN_[e.cand]++;
       tuned = true;
+    } else if (e.graph) {
+      st_.graphMs += ms;
+      st_.graphLaunches += e.launches;
     } else {
       st_.timedMs += ms;
       st_.timedBytes += e.bytes;
       st_.timedLaunches += e.launches;
     }
     e.cand = -1;
+    e.graph = false;
     evPool_.push_back(e);
   }
   evPending_.clear();

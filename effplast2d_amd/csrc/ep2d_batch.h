@@ -42,7 +42,8 @@ class Batch {
     hipEvent_t a = nullptr, b = nullptr;
     double bytes = 0;  // algorithmic bytes of the timed launches
     long launches = 0;
-    int cand = -1;     // tile-height candidate under tuning, -1 for statistics
+    int cand = -1;     // shape candidate under tuning, -1 for statistics
+    bool graph = false;  // a whole graph replay (launches counted in graphLaunches)
   };
   static constexpr int kTiles = 1, kMarch = 2, kPersist = 3;
   struct Shape {

@@ -297,6 +297,8 @@ typedef struct ep2d_batch_stats {
   double bytesAlgorithmic;        /* U and V read and written, all launches   */
   double timedMs, timedBytes;     /* sampled launches: duration and bytes     */
   long long timedLaunches;
+  double graphMs;                 /* timed graph replays: total duration      */
+  long long graphLaunches;        /* launches inside those replays            */
   int active, queued;
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
   int kernel;                     /* kernel in use: 1 tiles, 2 row marching,
