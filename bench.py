@@ -136,11 +136,11 @@ def percentile(xs, q):
 
 
 CAND_NAMES = ["tiles8", "tiles14", "tiles30", "march16", "march32", "persist6", "persist8",
-              "persist14"]
+              "persist14", "march2x32", "march2x64"]
 
 
 def shape_name(kind, rows):
-    return f"{ {1: 'tiles', 2: 'march', 3: 'persist'}[kind]}{rows}"
+    return f"{ {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x'}[kind]}{rows}"
 
 
 def batch_stats_summary(st, mode):
@@ -150,7 +150,7 @@ def batch_stats_summary(st, mode):
         else float("nan")
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
-            3: f"k_fused_persist<{ct}, {ct}, TY>"}[st["kernel"]]
+            3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>"}[st["kernel"]]
     return {"kernel": kern,
             "kernel_avg_us": kms * 1e3, "kernel_samples": st["timedLaunches"],
             "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]

@@ -42,7 +42,8 @@ class Options(C.Structure):
                 ("checkInterval", C.c_long), ("mode", C.c_int), ("device", C.c_int),
                 ("nTimeSteps", C.c_int), ("useShearLoadType", C.c_int),
                 ("shearLoadType", C.c_double * 3), ("verbose", C.c_int),
-                ("logPath", C.c_char_p), ("tileRows", C.c_int), ("marchRows", C.c_int)]
+                ("logPath", C.c_char_p), ("tileRows", C.c_int), ("marchRows", C.c_int),
+                ("itersPerLaunch", C.c_int)]
 
 
 class Moduli(C.Structure):
@@ -89,7 +90,7 @@ class Result(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("checkInterval", C.c_long), ("tileRows", C.c_int),
-                ("marchRows", C.c_int), ("persistRows", C.c_int),
+                ("marchRows", C.c_int), ("persistRows", C.c_int), ("march2Rows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
                 ("useGraphs", C.c_int), ("timingEvery", C.c_long)]
 
@@ -123,7 +124,7 @@ class BatchStats(C.Structure):
                 ("graphMs", C.c_double), ("graphLaunches", C.c_longlong),
                 ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t),
                 ("kernel", C.c_int), ("kernelRows", C.c_int), ("tunings", C.c_int),
-                ("candUs", C.c_double * 8)]
+                ("candUs", C.c_double * 10)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

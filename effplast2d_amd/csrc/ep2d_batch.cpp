@@ -23,11 +23,12 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
 // Kernel shapes the tuner times: the fused tile kernel at three tile
-// heights, the row-marching kernel at two segment lengths and the persistent
-// tile kernel at three tile heights.
+// heights, the row-marching kernel at two segment lengths, the persistent
+// tile kernel at three tile heights and the two-iteration marching kernel at
+// two segment lengths.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8}, {kTiles, 14}, {kTiles, 30}, {kMarch, 16}, {kMarch, 32},
-    {kPersist, 6}, {kPersist, 8}, {kPersist, 14}};
+    {kTiles, 8},   {kTiles, 14},  {kTiles, 30},   {kMarch, 16},  {kMarch, 32},
+    {kPersist, 6}, {kPersist, 8}, {kPersist, 14}, {kMarch2, 32}, {kMarch2, 64}};
 
 namespace {
 
@@ -42,12 +43,16 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
   if (opt_.persistRows != 0 && !valid_tile_rows(opt_.persistRows))
     throw ArgError("persistRows must be 0, 6, 8, 14 or 30");
-  const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0);
-  if (nfixed > 1) throw ArgError("give at most one of tileRows, marchRows, persistRows");
+  if (opt_.march2Rows < 0) throw ArgError("march2Rows must be >= 0");
+  const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +
+                     (opt_.march2Rows != 0);
+  if (nfixed > 1)
+    throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows");
   fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
   else if (opt_.persistRows) shape_ = {kPersist, opt_.persistRows};
+  else if (opt_.march2Rows) shape_ = {kMarch2, opt_.march2Rows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -213,17 +218,21 @@ void Batch::dropGraphs() {
 }
 
 // Work list of every admitted RVE for a kernel shape.  Tiles (fused tile
-// kernel): ceil(nX/62) columns of 62 outputs by ceil(nY/rows) rows, entries
-// (RVE index, i0, j0, unused).  Segments (row-marching kernel): strips of 60
-// output columns by runs of `rows` rows, entries (RVE index, s, j0, j1).  The
-// RVE's current displacement buffer is in its table entry, so the list stays
-// valid while the RVEs iterate.
+// kernels): ceil(nX/62) columns of 62 outputs by ceil(nY/rows) rows, entries
+// (RVE index, i0, j0, unused).  Segments (marching kernels): strips of 60
+// (one iteration) or 56 (two iterations) output columns by runs of `rows`
+// rows, entries (RVE index, s, j0, j1).  The RVE's current buffers are in its
+// table entry, so the list stays valid while the RVEs iterate.
 int Batch::buildWork(Shape sh, int4* out) const {
   int t = 0;
   for (size_t k = 0; k < active_.size(); k++) {
     const Solver& s = *active_[k].s;
     if (sh.kind == kMarch) {
       t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
+      continue;
+    }
+    if (sh.kind == kMarch2) {
+      t += march2_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
     const int ntx = (s.nX() + kTileX - 1) / kTileX;
@@ -237,38 +246,69 @@ int Batch::buildWork(Shape sh, int4* out) const {
   return t;
 }
 
-void Batch::launchWork(Shape sh, const int4* work, int n, int parity, hipEvent_t e0,
-                       hipEvent_t e1) {
-  if (sh.kind == kMarch)
-    launch_march(dRves_, work, n, opt_.mode, parity, stream_, e0, e1);
+// One launch of shape sh as the l-th launch of the chunk after d two-iteration
+// launches: U parity l, V parity d.  Two-iteration shapes launch k_march2
+// when dbl, else one marching iteration over the auxiliary list (V in place
+// in the current buffer).  The other kernels update V in place in the table's
+// buffer, so they are only enqueued when d is even.
+void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d, hipEvent_t e0,
+                      hipEvent_t e1) {
+  if (sh.kind == kMarch2 && dbl)
+    launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1);
+  else if (sh.kind == kMarch2)
+    launch_march(dRves_, dTiles_ + auxOff_, auxCnt_, opt_.mode, (int)(l & 1), (int)(d & 1),
+                 stream_, e0, e1);
+  else if (sh.kind == kMarch)
+    launch_march(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1);
   else if (sh.kind == kPersist)
-    launch_fused_persist(dRves_, work, n, opt_.mode, sh.rows, parity, stream_, e0, e1);
+    launch_fused_persist(dRves_, work, n, opt_.mode, sh.rows, (int)(l & 1), stream_, e0, e1);
   else
-    launch_fused_batch(dRves_, work, n, opt_.mode, sh.rows, parity, stream_, e0, e1);
+    launch_fused_batch(dRves_, work, n, opt_.mode, sh.rows, (int)(l & 1), stream_, e0, e1);
 }
 
-// Graph of `len` launches over the current tables (parities 0, 1, 0, ...),
-// captured once per length while the tables stay the same.
+// Split of `iters` iterations for the two-iteration shape: pairs first, then
+// one or two single iterations, so the chunk's last launch reads U^{n-1}
+// (the stress fields are recomputed from that buffer after a load step).
+static void split_two(long iters, long& pairs, long& singles) {
+  singles = iters <= 0 ? 0 : ((iters & 1) ? 1 : (iters >= 2 ? 2 : 0));
+  pairs = (iters - singles) / 2;
+}
+
+// `n` launches of the current shape from launch index l and double index d
+// (pairs when dbl); advances l and d.
+void Batch::enqueue(long n, bool dbl, long& l, long& d) {
+  for (long k = 0; k < n; k++) {
+    launchOne(shape_, dTiles_, nTiles_, dbl, l, d);
+    l++;
+    if (dbl) d++;
+  }
+}
+
+// Graph of `len` launches of the current shape (pairs for the two-iteration
+// shape) from launch and double index 0 over the current tables, captured
+// once per length while the tables stay the same.
 hipGraphExec_t Batch::graphFor(long len) {
   for (auto& g : graphs_)
     if (g.len == len) return g.exec;
   if (graphs_.size() >= 4) dropGraphs();
   GraphEntry g;
   g.len = len;
+  long l = 0, d = 0;
   EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  for (long q = 0; q < len; q++) launchWork(shape_, dTiles_, nTiles_, (int)(q & 1));
+  enqueue(len, shape_.kind == kMarch2, l, d);
   EP2D_HIP(hipStreamEndCapture(stream_, &g.graph));
   EP2D_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
   graphs_.push_back(g);
   return g.exec;
 }
 
-// c launches of the batched kernel over every admitted RVE.  Launch q uses
-// parity q&1 and RVE k starts from its buffer cur_k (in its table entry), so
-// launch q reads buffer cur_k ^ (q&1) of RVE k.  The kernel shape does not
-// change any result, so a tuning chunk spends its first launches timing each
-// candidate shape.
-void Batch::launchChunk(long c) {
+// c iterations of every admitted RVE.  Launch l reads U buffer cur_k ^ (l&1)
+// of RVE k (cur_k in its table entry); two-iteration launch number d reads V
+// buffer d&1 (the table's Vx/Vy are the current ones) and writes the other.
+// The shape does not change any result, so a tuning chunk spends its first
+// launches timing each candidate (two-iteration candidates in pairs, so V is
+// back in its buffer before the next candidate).  Returns the launch counts.
+Batch::Seq Batch::launchChunk(long c) {
   std::vector<long long> ids;
   ids.reserve(active_.size() + 3);
   double bytes = 0.0, cells = 0.0;
@@ -280,19 +320,29 @@ void Batch::launchChunk(long c) {
   }
   if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
-  constexpr long kTuneEach = 4;  // timed launches per candidate
-  const bool tuning = !fixed_ && tuneDue_ && c >= (kTuneEach + 1) * kNCand + 2;
+  constexpr long kTuneEach = 4;  // timed rounds
+  long roundIters = 0;
+  for (int k = 0; k < kNCand; k++) roundIters += kCand[k].kind == kMarch2 ? 4 : 1;
+  const bool tuning = !fixed_ && tuneDue_ && c >= (kTuneEach + 1) * roundIters + 2;
   ids.push_back(tuning ? 1 : 0);
   ids.push_back(shape_.kind);
   ids.push_back(shape_.rows);
   if (ids != key_) {
     dropGraphs();
+    const Shape aux{kMarch, shape_.rows};
     size_t need = (size_t)buildWork(shape_, nullptr);
+    if (shape_.kind == kMarch2) need += (size_t)buildWork(aux, nullptr);
     if (tuning)
       for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
     ensureCapacity(active_.size(), need);
     nTiles_ = buildWork(shape_, hTiles_);
     int off = nTiles_;
+    auxOff_ = off;
+    auxCnt_ = 0;
+    if (shape_.kind == kMarch2) {
+      auxCnt_ = buildWork(aux, hTiles_ + off);
+      off += auxCnt_;
+    }
     if (tuning)
       for (int k = 0; k < kNCand; k++) {
         candOff_[k] = off;
@@ -304,49 +354,63 @@ void Batch::launchChunk(long c) {
     key_ = ids;
     curs_.clear();
   }
-  // RVE table: re-uploaded when an RVE's current buffer changed (odd chunks)
+  // RVE table: re-uploaded when an RVE's current buffers changed
   std::vector<int> curs(active_.size());
-  for (size_t k = 0; k < active_.size(); k++) curs[k] = active_[k].s->cur();
+  for (size_t k = 0; k < active_.size(); k++)
+    curs[k] = active_[k].s->cur() | (active_[k].s->vBuffer() << 1);
   if (curs != curs_) {
     for (size_t k = 0; k < active_.size(); k++) {
       hRves_[k] = active_[k].s->dev();
-      hRves_[k].cur = curs[k];
+      hRves_[k].cur = active_[k].s->cur();
     }
     EP2D_HIP(hipMemcpyAsync(dRves_, hRves_, active_.size() * sizeof(RveDev),
                             hipMemcpyHostToDevice, stream_));
     curs_ = curs;
   }
-  long q = 0;
+  long l = 0, d = 0, it = 0;
   if (tuning) {
-    // one untimed launch per candidate first (code and tables warm), then
-    // kTuneEach timed launches of each, interleaved
-    for (int k = 0; k < kNCand; k++, q++)
-      launchWork(kCand[k], dTiles_ + candOff_[k], candCnt_[k], (int)(q & 1));
-    for (long r = 0; r < kTuneEach; r++)
-      for (int k = 0; k < kNCand; k++, q++) {
-        Pending ev = takeEvents();
-        ev.cand = k;
-        launchWork(kCand[k], dTiles_ + candOff_[k], candCnt_[k], (int)(q & 1), ev.a, ev.b);
-        evPending_.push_back(ev);
+    // a warm-up round, then kTuneEach timed rounds, candidates interleaved
+    for (long round = 0; round <= kTuneEach; round++)
+      for (int k = 0; k < kNCand; k++) {
+        const Shape sh = kCand[k];
+        const int4* w = dTiles_ + candOff_[k];
+        const bool dbl = sh.kind == kMarch2;
+        for (int rep = 0; rep < (dbl ? 2 : 1); rep++) {
+          Pending ev{};
+          if (round > 0) {
+            ev = takeEvents();
+            ev.cand = k;
+            ev.launches = dbl ? 2 : 1;
+          }
+          launchOne(sh, w, candCnt_[k], dbl, l, d, ev.a, ev.b);
+          l++;
+          if (dbl) d++;
+          if (round > 0) evPending_.push_back(ev);
+          it += dbl ? 2 : 1;
+        }
       }
     for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
     tunedCells_ = cells;
     tuneDue_ = false;
     st_.tunings++;
   }
-  const long rest = c - q;
+  const long rest = c - it;
   const bool timed = opt_.timingEvery > 0;
+  const bool two = shape_.kind == kMarch2;
+  long pairs = 0, singles = rest;
+  if (two) split_two(rest, pairs, singles);
+  const long main = two ? pairs : singles;   // launches of the main kernel
   if (opt_.useGraphs && !tuning) {
-    // graph of the chunk; with timing on, its last launch runs outside the
-    // graph between dispatch events
-    const long glen = timed ? rest - 1 : rest;
+    // graph of the main launches; with timing on, the last of them runs
+    // outside the graph between dispatch events
+    const long glen = timed && main > 0 ? main - 1 : main;
     if (glen > 0) {
       const hipGraphExec_t ge = graphFor(glen);
       Pending gv{};
       if (timed) {
         gv = takeEvents();
         gv.graph = true;
-        gv.launches = glen;
+        gv.launches = two ? 2 * glen : glen;
         EP2D_HIP(hipEventRecord(gv.a, stream_));
       }
       EP2D_HIP(hipGraphLaunch(ge, stream_));
@@ -354,32 +418,43 @@ void Batch::launchChunk(long c) {
         EP2D_HIP(hipEventRecord(gv.b, stream_));
         evPending_.push_back(gv);
       }
+      l += glen;
+      if (two) d += glen;
     }
-    if (timed) {
+    if (timed && main > 0) {
       Pending ev = takeEvents();
-      ev.bytes = bytes;
-      ev.launches = 1;
-      launchWork(shape_, dTiles_, nTiles_, (int)((q + glen) & 1), ev.a, ev.b);
+      ev.bytes = (two ? 2 : 1) * bytes;
+      ev.launches = two ? 2 : 1;
+      launchOne(shape_, dTiles_, nTiles_, two, l, d, ev.a, ev.b);
       evPending_.push_back(ev);
+      l++;
+      if (two) d++;
     }
+    if (two) enqueue(singles, false, l, d);
   } else {
-    for (long r = 0; r < rest; r++, q++) {
-      const bool t = timed && (st_.launches + q) % opt_.timingEvery == 0;
+    auto one = [&](bool dbl) {
+      const bool t = timed && (st_.launches + l) % opt_.timingEvery == 0;
       Pending ev{};
       if (t) {
         ev = takeEvents();
-        ev.bytes = bytes;
-        ev.launches = 1;
+        ev.bytes = (dbl ? 2 : 1) * bytes;
+        ev.launches = dbl ? 2 : 1;
       }
-      launchWork(shape_, dTiles_, nTiles_, (int)(q & 1), ev.a, ev.b);
+      launchOne(shape_, dTiles_, nTiles_, dbl, l, d, ev.a, ev.b);
+      l++;
+      if (dbl) d++;
       if (t) evPending_.push_back(ev);
-    }
+    };
+    for (long k = 0; k < main; k++) one(two);
+    if (two)
+      for (long k = 0; k < singles; k++) one(false);
   }
-  st_.launches += c;
+  st_.launches += l;
   st_.chunks++;
-  st_.tilesLaunched += (long long)c * nTiles_;
+  st_.tilesLaunched += (long long)l * nTiles_;
   st_.cellIterations += (long long)(c * cells);
   st_.bytesAlgorithmic += c * bytes;
+  return Seq{l, d};
 }
 
 Batch::Pending Batch::takeEvents() {
@@ -401,7 +476,7 @@ void Batch::resolveEvents() {
     EP2D_HIP(hipEventElapsedTime(&ms, e.a, e.b));
     if (e.cand >= 0) {
       tuneMs_[e.cand] += ms;
-      tuneN_[e.cand]++;
+      tuneN_[e.cand] += e.launches;   // iterations
       tuned = true;
     } else if (e.graph) {
       st_.graphMs += ms;
@@ -433,9 +508,9 @@ void Batch::step(int* nActive, int* nQueued) {
     long c = active_.front().s->tsChunk();
     for (auto& sl : active_) c = std::min(c, sl.s->tsChunk());
     c = std::max<long>(c, 1);
-    launchChunk(c);
+    const Seq q = launchChunk(c);
     for (auto& sl : active_) {
-      sl.s->noteBatchIterations(c);
+      sl.s->noteBatchIterations(c, q.launches, q.doubles);
       if (sl.s->tsAtCheck()) sl.s->tsCheckLaunch();
     }
     EP2D_HIP(hipStreamSynchronize(stream_));

@@ -41,16 +41,16 @@ class Batch {
   struct Pending {
     hipEvent_t a = nullptr, b = nullptr;
     double bytes = 0;  // algorithmic bytes of the timed launches
-    long launches = 0;
+    long launches = 0;   // iterations covered by the timed launch(es)
     int cand = -1;     // shape candidate under tuning, -1 for statistics
     bool graph = false;  // a whole graph replay (launches counted in graphLaunches)
   };
-  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3;
+  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4;
   struct Shape {
-    int kind = kTiles;  // fused tile, row-marching or persistent tile kernel
+    int kind = kTiles;  // fused tile, row-marching, persistent tile or two-iteration marching
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 8;
+  static constexpr int kNCand = 10;
   static const Shape kCand[kNCand];
   struct GraphEntry {
     long len = 0;
@@ -62,10 +62,14 @@ class Batch {
   void admit();
   void fail(Slot& sl, int status, const char* what);
   void finish(Slot& sl);
-  void launchChunk(long c);
+  struct Seq {
+    long launches, doubles;
+  };
+  Seq launchChunk(long c);
   int buildWork(Shape sh, int4* out) const;  // work list of all admitted RVEs
-  void launchWork(Shape sh, const int4* work, int n, int parity, hipEvent_t e0 = nullptr,
-                  hipEvent_t e1 = nullptr);
+  void launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d,
+                 hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+  void enqueue(long n, bool dbl, long& l, long& d);
   void ensureCapacity(size_t nRves, size_t nTiles);
   void dropGraphs();
   hipGraphExec_t graphFor(long len);
@@ -96,6 +100,7 @@ class Batch {
   std::vector<long long> key_;    // RVE ids, tuning flag and shape of the work list
   std::vector<int> curs_;         // current buffers in the uploaded RVE table
   int candOff_[kNCand] = {}, candCnt_[kNCand] = {};
+  int auxOff_ = 0, auxCnt_ = 0;   // single-iteration segments of a two-iteration shape
 
   // kernel-shape tuning on real iterations
   bool tuneDue_ = true;

@@ -39,6 +39,8 @@ struct RveDev {
   double* Uy[2];
   void* Vx;                 // velocity: double (plain, FP64) or float (split)
   void* Vy;
+  void* Vx2;                // second velocity buffer of the fused schemes (two-
+  void* Vy2;                // iteration launches read one and write the other)
   const unsigned char* mask;
   int nX, nY;
   double dX, dY, dT;
@@ -117,8 +119,16 @@ constexpr int kMarchCols = 60;
 // cache lines neighbouring strips share are read while both run); returns the
 // count and fills out when it is not null.
 int march_segments(int nX, int nY, int rows, int rve, int4* out);
+// V is updated in place in Vx/Vy (vpar = 0) or Vx2/Vy2 (vpar = 1).
 void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                  hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                  int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Two iterations per launch: strips of 56 output columns; U read from U[cur]
+// (cur = parity ^ rve.cur) and written to U[cur^1]; V read from Vx/Vy when
+// vpar = 0 (else Vx2/Vy2) and written to the other pair.
+constexpr int kMarch2Cols = 56;
+int march2_segments(int nX, int nY, int rows, int rve, int4* out);
+void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
+                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // batched fused iteration over many RVEs: tiles[k] = (rve, i0, j0, parity
 // offset); RVE k reads U[parity ^ offset].  All RVEs share mode and tile height.
 void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,

@@ -996,10 +996,9 @@ struct March {
 
   template <bool EDGE>
   static __device__ __forceinline__ Ld load(const RveDev& r, const double* __restrict__ Ux,
-                                            const double* __restrict__ Uy, int i, int j) {
+                                            const double* __restrict__ Uy, const VT* Vx,
+                                            const VT* Vy, int i, int j) {
     const int nX = r.nX, nY = r.nY, sx = nX + 1;
-    const VT* Vx = static_cast<const VT*>(r.Vx);
-    const VT* Vy = static_cast<const VT*>(r.Vy);
     Ld d;
     if (!EDGE) {
       // row bases are uniform (scalar arithmetic), the lane offset i is fixed
@@ -1043,7 +1042,8 @@ struct March {
   }
 
   template <bool EDGE>
-  static __device__ void run(const RveDev& r, int cur, int s, int j0, int j1, const CT* gt) {
+  static __device__ void run(const RveDev& r, int cur, int vpar, int s, int j0, int j1,
+                             const CT* gt) {
     const int nX = r.nX, nY = r.nY, sx = nX + 1;
     const int L = threadIdx.x & 63;
     const int i = s + L;
@@ -1051,8 +1051,9 @@ struct March {
     const double* __restrict__ Uyi = r.Uy[cur];
     double* __restrict__ Uxo = r.Ux[cur ^ 1];
     double* __restrict__ Uyo = r.Uy[cur ^ 1];
-    VT* Vx = static_cast<VT*>(r.Vx);
-    VT* Vy = static_cast<VT*>(r.Vy);
+    // V updated in place in the current buffer (Vx2/Vy2 when vpar)
+    VT* Vx = static_cast<VT*>(vpar ? r.Vx2 : r.Vx);
+    VT* Vy = static_cast<VT*>(vpar ? r.Vy2 : r.Vy);
     const bool plastic = r.plastic != 0;
     const CT coh = (CT)r.coh;
     CT c2lo;
@@ -1089,7 +1090,7 @@ struct March {
     // range: interior segments end at least one row above nY-1, edge loads
     // are guarded), so no branch joins in front of a wait
     auto fetch = [&](int j) {
-      return load<EDGE>(r, Uxi, Uyi, i, EDGE ? j : min(j, j1 - 1));
+      return load<EDGE>(r, Uxi, Uyi, Vx, Vy, i, EDGE ? j : min(j, j1 - 1));
     };
     auto step = [&](const int j, const Ld& ld) {
 
@@ -1236,7 +1237,7 @@ struct March {
 template <typename CT, typename VT>
 __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
                                                const int4* __restrict__ segs, int nsegs,
-                                               int parity) {
+                                               int parity, int vpar) {
   __shared__ CT gt[4][16];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
@@ -1251,8 +1252,317 @@ __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
   const int cur = parity ^ r.cur;
   const int s = sg.y, j0 = sg.z, j1 = sg.w;
   const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 2 && j1 <= r.nY - 1;
-  if (interior) March<CT, VT>::template run<false>(r, cur, s, j0, j1, gt[wave]);
-  else March<CT, VT>::template run<true>(r, cur, s, j0, j1, gt[wave]);
+  if (interior) March<CT, VT>::template run<false>(r, cur, vpar, s, j0, j1, gt[wave]);
+  else March<CT, VT>::template run<true>(r, cur, vpar, s, j0, j1, gt[wave]);
+}
+
+// ---------------------------------------------------------------------------
+// Two iterations per launch (temporal blocking of the row-marching kernel).
+// One iteration is written as a row stream (MarchStage): step j takes row j+1
+// of Ux and the mask, row j+2 of Uy and row j of V of its input state and
+// gives row j of its output state -- V, and U at every node of the row
+// (updated where ComputeDisp updates it, unchanged where it does not, zero
+// outside the grid) -- so a second stage can take it as input.  In k_march2
+// stage A reads the state from HBM and stage B, two rows behind, reads A's
+// rows from registers; only B's rows are written.  U and V are read from one
+// buffer and written to the other (a neighbouring segment's halo needs the
+// old velocities).  Valid outputs shrink by two lanes per stage: lanes 4..59,
+// strips of 56 columns.  The arithmetic of each stage is the single-iteration
+// kernels', so two stages give the same bits as two launches.
+template <typename CT, typename VT, bool EDGE>
+struct MarchStage {
+  using M = March<CT, VT>;
+  const RveDev* r;
+  const CT* gt;
+  int nX, nY, i;
+  bool colC, colV, leftEdge, rightEdge, inX, plastic;
+  CT coh, c2lo, K0, Kp, G0, Gp, P0, vdampX, vdampY, dTrho;
+  double dT;
+  // carried rows: U rows j and j+1, mask row j, trial cells j (and j-1),
+  // trial vertices j-1 and j-2, corrected row j-1
+  double ux_c, uxs_c, uy_c, uy_n;
+  unsigned m_c;
+  CT tP_c, tXX_c, tYY_c, sXX_c, sYY_c, tP_p, tXX_p, tYY_p, tXY_p, tXY_pp, cP_p, cYY_p, cXY_p;
+
+  struct Out {
+    double ux, uy;   // output U of row j at this lane's column
+    CT vx, vy;       // output V of row j (rounded to the storage type)
+    bool updX, updY; // V was updated here (else it is the input)
+  };
+
+  __device__ __forceinline__ void setup(const RveDev& rr, const CT* g, int col) {
+    r = &rr;
+    gt = g;
+    nX = rr.nX;
+    nY = rr.nY;
+    i = col;
+    colC = !EDGE || (i >= 0 && i < nX);
+    colV = !EDGE || (i >= 0 && i < nX - 1);
+    inX = colC;
+    leftEdge = EDGE && i == 0;
+    rightEdge = EDGE && i == nX - 1;
+    plastic = rr.plastic != 0;
+    coh = (CT)rr.coh;
+    if constexpr (sizeof(CT) == 8) c2lo = rr.coh2lo;
+    else c2lo = rr.coh2lof;
+    K0 = (CT)rr.K0;
+    Kp = (CT)rr.Kp;
+    G0 = (CT)rr.G0;
+    Gp = (CT)rr.Gp;
+    P0 = (CT)rr.P0;
+    vdampX = (CT)rr.vdampX;
+    vdampY = (CT)rr.vdampY;
+    dTrho = (CT)rr.dTrho;
+    dT = rr.dT;
+  }
+
+  // Ux row jb, Uy row jb+1, mask row jb (jb = first step)
+  __device__ __forceinline__ void start(double ux, double uy1, unsigned m) {
+    ux_c = ux;
+    uxs_c = lane_next(ux_c);
+    uy_c = 0.0;
+    uy_n = uy1;
+    m_c = m;
+    tP_c = tXX_c = tYY_c = sXX_c = sYY_c = 0;
+    tP_p = tXX_p = tYY_p = 0;
+    tXY_p = tXY_pp = 0;
+    cP_p = cYY_p = cXY_p = 0;
+  }
+
+  __device__ __forceinline__ Out step(const int j, const double ux_n, const double uy_nn,
+                                      const unsigned m_n, const CT vx_in, const CT vy_in) {
+    const RveDev& rr = *r;
+    // trial cell row j+1
+    const double uxs_n = lane_next(ux_n);
+    CT tP_n, tXX_n, tYY_n;
+    {
+      const CT K = (m_n & M_CELL_MAT) ? Kp : K0;
+      const CT G = (m_n & M_CELL_MAT) ? Gp : G0;
+      const CT exx = M::dx(uxs_n - ux_n, rr);
+      const CT eyy = M::dy(uy_nn - uy_n, rr);
+      CT pp = P0 - K * (exx + eyy);
+      CT txx = (CT)2.0 * G * (exx - M::third(exx + eyy));
+      CT tyy = (CT)2.0 * G * (eyy - M::third(exx + eyy));
+      const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
+      if ((m_n & M_CELL_ZERO) || !ok) {
+        pp = 0;
+        txx = 0;
+        tyy = 0;
+      }
+      tP_n = pp;
+      tXX_n = txx;
+      tYY_n = tyy;
+    }
+    const CT sXX_n = lane_next(tXX_n), sYY_n = lane_next(tYY_n);
+    // trial vertex row j
+    const double uys_n = lane_next(uy_n);
+    CT tXY_c;
+    {
+      const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
+      CT txy = g * (M::dy(uxs_n - uxs_c, rr) + M::dx(uys_n - uy_n, rr));
+      const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
+      if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
+      tXY_c = txy;
+    }
+    // return mapping
+    CT cXY_c = tXY_c, cP_c = tP_c, cXX_c = tXX_c, cYY_c = tYY_c;
+    if (plastic) {
+      const bool vok = !EDGE || (colV && j >= 0 && j < nY - 1);
+      const CT sxx = tXX_c + sXX_c + tXX_n + sXX_n;
+      const CT syy = tYY_c + sYY_c + tYY_n + sYY_n;
+      if (vok) cXY_c = M::vertex_rm(tXY_c, sxx, syy, coh, c2lo);
+      const CT pr_p = lane_prev(tXY_p), pr_c = lane_prev(tXY_c);
+      if (!EDGE) {
+        const CT av = (CT)0.25 * (pr_p + tXY_p + pr_c + tXY_c);
+        M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+      } else {
+        const CT nx_p = lane_next(tXY_p), nx_c = lane_next(tXY_c), nx_pp = lane_next(tXY_pp);
+        const CT pr_pp = lane_prev(tXY_pp);
+        const CT pp_p = lane_prev(pr_p), pp_c = lane_prev(pr_c), pp_pp = lane_prev(pr_pp);
+        auto first = [&](CT t, CT pr, CT pp) { return leftEdge ? t : (rightEdge ? pp : pr); };
+        auto second = [&](CT t, CT pr, CT nxv) { return leftEdge ? nxv : (rightEdge ? pr : t); };
+        if (colC && j >= 1 && j < nY - 1) {
+          const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
+                                    first(tXY_c, pr_c, pp_c) + second(tXY_c, pr_c, nx_c));
+          M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+        } else if (colC && j == nY - 1 && j >= 1) {
+          const CT av = (CT)0.25 * (first(tXY_pp, pr_pp, pp_pp) + second(tXY_pp, pr_pp, nx_pp) +
+                                    first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p));
+          M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
+        }
+        if (j == 1) {
+          CT xx = tXX_p, yy = tYY_p;
+          if (colC) {
+            const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
+                                      first(tXY_c, pr_c, pp_c) + second(tXY_c, pr_c, nx_c));
+            M::cell_rm(xx, yy, av, coh, c2lo);
+          }
+          cP_p = tP_p;
+          cYY_p = yy;
+        }
+      }
+    }
+    // velocity and displacement of row j
+    const CT rP = lane_prev(cP_c), rXX = lane_prev(cXX_c);
+    const CT rXY_c = lane_prev(cXY_c), rXY_p = lane_prev(cXY_p);
+    Out o;
+    CT vx = vx_in, vy = vy_in;
+    o.updX = !EDGE || (i > 0 && i < nX && j > 0 && j < nY - 1);
+    o.updY = !EDGE || (i > 0 && i < nX - 1 && j > 0 && j < nY);
+    if (o.updX)
+      vx = (CT)(VT)(vx * vdampX + dTrho * (M::dx(-cP_c + rP + cXX_c - rXX, rr) + M::dy(rXY_c - rXY_p, rr)));
+    if (o.updY)
+      vy = (CT)(VT)(vy * vdampY + dTrho * (M::dy(-cP_c + cP_p + cYY_c - cYY_p, rr) + M::dx(cXY_p - rXY_p, rr)));
+    const bool in = !EDGE || (inX && j >= 0 && j < nY);
+    o.vx = vx;
+    o.vy = vy;
+    o.ux = in ? ux_c + (double)vx * dT : ux_c;
+    o.uy = in ? uy_c + (double)vy * dT : uy_c;
+    // rotate
+    ux_c = ux_n;
+    uxs_c = uxs_n;
+    uy_c = uy_n;
+    uy_n = uy_nn;
+    m_c = m_n;
+    tP_p = tP_c;
+    tXX_p = tXX_c;
+    tYY_p = tYY_c;
+    tP_c = tP_n;
+    tXX_c = tXX_n;
+    tYY_c = tYY_n;
+    sXX_c = sXX_n;
+    sYY_c = sYY_n;
+    tXY_pp = tXY_p;
+    tXY_p = tXY_c;
+    cP_p = cP_c;
+    cYY_p = cYY_c;
+    cXY_p = cXY_c;
+    return o;
+  }
+};
+
+template <typename CT, typename VT, bool EDGE>
+__device__ __forceinline__ void march2_run(const RveDev& r, int cur, int vpar, int s, int j0,
+                                           int j1, const CT* gt) {
+  using M = March<CT, VT>;
+  using Ld = typename M::Ld;
+  const int nX = r.nX, nY = r.nY, sx = nX + 1;
+  const int L = threadIdx.x & 63;
+  const int i = s + L;
+  const double* __restrict__ Uxi = r.Ux[cur];
+  const double* __restrict__ Uyi = r.Uy[cur];
+  double* __restrict__ Uxo = r.Ux[cur ^ 1];
+  double* __restrict__ Uyo = r.Uy[cur ^ 1];
+  // velocities are read from one buffer and written to the other
+  const VT* Vxi = static_cast<const VT*>(vpar ? r.Vx2 : r.Vx);
+  const VT* Vyi = static_cast<const VT*>(vpar ? r.Vy2 : r.Vy);
+  VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
+  VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
+  const bool outLane = L >= 4 && L < 60 && (!EDGE || i < nX);
+
+  MarchStage<CT, VT, EDGE> A, B;
+  A.setup(r, gt, i);
+  B.setup(r, gt, i);
+  const int jbA = j0 - 4, jbB = j0 - 2;
+  {
+    double ux = 0.0, uy = 0.0;
+    unsigned m = 0;
+    const bool colC = !EDGE || (i >= 0 && i < nX);
+    if (!EDGE) {
+      ux = gld(Uxi, (unsigned)(jbA * sx + i));
+      uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
+      m = gld(r.mask, (unsigned)(jbA * nX + i));
+    } else {
+      if (i >= 0 && i <= nX && jbA >= 0 && jbA < nY) ux = gld(Uxi, (unsigned)(jbA * sx + i));
+      if (colC && jbA + 1 >= 0 && jbA + 1 <= nY) uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
+      if (colC && jbA >= 0 && jbA < nY) m = gld(r.mask, (unsigned)(jbA * nX + i));
+    }
+    A.start(ux, uy, m);
+  }
+  // A's outputs of rows t-1 (Ux) and t-1, t-2 (V); A's input mask rows t, t-1
+  double h1ux = 0.0;
+  CT h1vx = 0, h1vy = 0, h2vx = 0, h2vy = 0;
+  unsigned mh1 = 0, mh2 = 0;
+  const int tEnd = j1 + 1;  // A's last step: B's last step j1-1 needs A's row j1+1
+  // step t reads Ux row t+1, Uy row t+2, mask row t+1, V row t (unconditional:
+  // interior segments clamp to rows in range, edge loads are guarded)
+  auto fetch = [&](int t0) {
+    const int t = EDGE ? t0 : min(t0, tEnd);
+    Ld d;
+    if (!EDGE) {
+      d.ux = gld(Uxi + (size_t)(t + 1) * sx, (unsigned)i);
+      d.uy = gld(Uyi + (size_t)(t + 2) * nX, (unsigned)i);
+      d.m = gld(r.mask + (size_t)(t + 1) * nX, (unsigned)i);
+      d.vx = gld(Vxi + (size_t)t * sx, (unsigned)i);
+      d.vy = gld(Vyi + (size_t)t * nX, (unsigned)i);
+    } else {
+      const bool ci = i >= 0 && i < nX;
+      d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
+      d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
+      d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(r.mask, (unsigned)((t + 1) * nX + i)) : 0u;
+      const bool vr = ci && t >= 0 && t < nY;
+      d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
+      d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
+    }
+    return d;
+  };
+  auto body = [&](const int t, const Ld& ld) {
+    const auto oA = A.step(t, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
+    if (t == jbB + 1) B.start(h1ux, oA.uy, mh2);  // Ux' row jbB, Uy' row jbB+1, mask row jbB
+    if (t >= jbB + 2) {
+      const int j = t - 2;
+      const double uxsB = B.uxs_c;  // Ux(i+1, j) of B's input, before the step
+      const auto oB = B.step(j, h1ux, oA.uy, mh2, h2vx, h2vy);
+      if (j >= j0 && outLane) {
+        if (oB.updX) gst(Vxo + (size_t)j * sx, (unsigned)i, (VT)oB.vx);
+        if (oB.updY) gst(Vyo + (size_t)j * nX, (unsigned)i, (VT)oB.vy);
+        gst(Uxo + (size_t)j * sx, (unsigned)i, oB.ux);
+        gst(Uyo + (size_t)j * nX, (unsigned)i, oB.uy);
+        if (EDGE) {
+          // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+          if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsB);
+          if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), B.uy_c);  // row j+1 after the step
+        }
+      }
+    }
+    h2vx = h1vx;
+    h2vy = h1vy;
+    h1ux = oA.ux;
+    h1vx = oA.vx;
+    h1vy = oA.vy;
+    mh2 = mh1;
+    mh1 = ld.m;
+  };
+  Ld P = fetch(jbA);
+  for (int t = jbA; t <= tEnd; t += 2) {
+    const Ld Q = fetch(t + 1);
+    body(t, P);
+    if (t + 1 > tEnd) break;
+    P = fetch(t + 2);
+    body(t + 1, Q);
+  }
+}
+
+template <typename CT, typename VT>
+__global__ __launch_bounds__(256) void k_march2(const RveDev* __restrict__ rves,
+                                                const int4* __restrict__ segs, int nsegs,
+                                                int parity, int vpar) {
+  __shared__ CT gt[4][16];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = tile_of_block(blockIdx.x, gridDim.x);
+  const int k = b * 4 + wave;
+  if (k >= nsegs) return;
+  const int4 sg = segs[k];
+  const RveDev& r = rves[sg.x];
+  if (lane < 16) gt[wave][lane] = (CT)r.gtab[lane];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const int cur = parity ^ r.cur;
+  const int s = sg.y, j0 = sg.z, j1 = sg.w;
+  const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 4 && j1 <= r.nY - 3;
+  if (interior) march2_run<CT, VT, false>(r, cur, vpar, s, j0, j1, gt[wave]);
+  else march2_run<CT, VT, true>(r, cur, vpar, s, j0, j1, gt[wave]);
 }
 
 }  // namespace
@@ -1380,17 +1690,44 @@ int march_segments(int nX, int nY, int rows, int rve, int4* out) {
 }
 
 void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                  hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                  int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (nsegs <= 0) return;
   const dim3 grid((nsegs + 3) / 4), block(256);
   auto go = [&](auto kern) {
     if (e0)
-      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity);
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
     else
-      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity);
+      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
   };
   if (mode == 2) go(k_march<float, float>);  // split: FP32 stress and velocity
   else go(k_march<double, double>);
+  EP2D_HIP(hipGetLastError());
+}
+
+int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
+  if (rows < 1) rows = 1;
+  const int ns = (nX + kMarch2Cols - 1) / kMarch2Cols;
+  int n = 0;
+  for (int j0 = 0; j0 < nY; j0 += rows)
+    for (int k = 0; k < ns; k++) {
+      if (out) out[n] = make_int4(rve, kMarch2Cols * k - 4, j0, std::min(j0 + rows, nY));
+      n++;
+    }
+  return n;
+}
+
+void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
+                   int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (nsegs <= 0) return;
+  const dim3 grid((nsegs + 3) / 4), block(256);
+  auto go = [&](auto kern) {
+    if (e0)
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
+    else
+      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
+  };
+  if (mode == 2) go(k_march2<float, float>);  // split: FP32 stress and velocity
+  else go(k_march2<double, double>);
   EP2D_HIP(hipGetLastError());
 }
 

@@ -150,6 +150,10 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   r_.Uy[1] = fused ? (double*)alloc(nUy * 8) : r_.Uy[0];
   r_.Vx = alloc(nUx * vbytes);
   r_.Vy = alloc(nUy * vbytes);
+  // second velocity buffers: two-iteration launches read one and write the
+  // other; boundary velocities (never updated) are kept equal in both
+  r_.Vx2 = fused ? alloc(nUx * vbytes) : r_.Vx;
+  r_.Vy2 = fused ? alloc(nUy * vbytes) : r_.Vy;
   r_.mask = (unsigned char*)alloc(nC);
   f_.P = (double*)alloc(nC * 8);
   f_.tauXX = (double*)alloc(nC * 8);
@@ -201,12 +205,16 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     launch_fill(r_.Ux[k], nUx, 0.0, stream_);
     launch_fill(r_.Uy[k], nUy, 0.0, stream_);
   }
-  if (mode_ == EP2D_MODE_SPLIT) {
-    launch_fill_f32((float*)r_.Vx, nUx, 0.0f, stream_);
-    launch_fill_f32((float*)r_.Vy, nUy, 0.0f, stream_);
-  } else {
-    launch_fill((double*)r_.Vx, nUx, 0.0, stream_);
-    launch_fill((double*)r_.Vy, nUy, 0.0, stream_);
+  for (int k = 0; k < (fused ? 2 : 1); k++) {
+    void* vx = k ? r_.Vx2 : r_.Vx;
+    void* vy = k ? r_.Vy2 : r_.Vy;
+    if (mode_ == EP2D_MODE_SPLIT) {
+      launch_fill_f32((float*)vx, nUx, 0.0f, stream_);
+      launch_fill_f32((float*)vy, nUy, 0.0f, stream_);
+    } else {
+      launch_fill((double*)vx, nUx, 0.0, stream_);
+      launch_fill((double*)vy, nUy, 0.0, stream_);
+    }
   }
   for (double* p : {f_.P, f_.tauXX, f_.tauYY, f_.tauXYav, f_.J2}) launch_fill(p, nC, 0.0, stream_);
   for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
@@ -217,6 +225,11 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (ty == 0) ty = default_tile_rows(mode_, nX);
     fplan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
     if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
+    if (opt_.itersPerLaunch < 0 || opt_.itersPerLaunch > 2)
+      throw ArgError("itersPerLaunch must be 1 or 2");
+    if (opt_.itersPerLaunch == 2 && opt_.marchRows <= 0)
+      throw ArgError("itersPerLaunch 2 needs the row-marching kernel (marchRows > 0)");
+    twoIter_ = opt_.itersPerLaunch == 2;
     if (opt_.marchRows > 0) {
       // row-marching kernel: the parameter table and the segment list live
       // on the device (the kernel reads the current buffer as parity ^ cur)
@@ -231,6 +244,15 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       EP2D_HIP(hipMemcpyAsync(dRve_, &rd, sizeof(RveDev), hipMemcpyHostToDevice, stream_));
       EP2D_HIP(hipMemcpyAsync(dSegs_, segs.data(), segs.size() * sizeof(int4),
                               hipMemcpyHostToDevice, stream_));
+      if (twoIter_) {
+        nSegs2_ = march2_segments(nX, nY, marchRows_, 0, nullptr);
+        std::vector<int4> s2((size_t)nSegs2_);
+        march2_segments(nX, nY, marchRows_, 0, s2.data());
+        dSegs2_ = (int4*)alloc(s2.size() * sizeof(int4));
+        EP2D_HIP(hipMemcpyAsync(dSegs2_, s2.data(), s2.size() * sizeof(int4),
+                                hipMemcpyHostToDevice, stream_));
+        EP2D_HIP(hipStreamSynchronize(stream_));
+      }
       EP2D_HIP(hipStreamSynchronize(stream_));
     }
   }
@@ -254,6 +276,7 @@ void Solver::release() {
     (void)hipEventDestroy(e.second);
   }
   pending_.clear();
+  pendingIters_.clear();
   freeEvents_.clear();
   for (void* p : allocs_) (void)hipFree(p);
   allocs_.clear();
@@ -363,8 +386,8 @@ void Solver::launchIteration() {
                       timed ? ev.second : nullptr);
   } else {
     if (marchRows_ > 0)
-      launch_march(dRve_, dSegs_, nSegs_, mode_, cur_, stream_, timed ? ev.first : nullptr,
-                   timed ? ev.second : nullptr);
+      launch_march(dRve_, dSegs_, nSegs_, mode_, cur_, vIdx_, stream_,
+                   timed ? ev.first : nullptr, timed ? ev.second : nullptr);
     else
       launch_fused(r_, cur_, mode_, plastic, fplan_, stream_, timed ? ev.first : nullptr,
                    timed ? ev.second : nullptr);
@@ -373,26 +396,77 @@ void Solver::launchIteration() {
   }
   if (timed) {
     pending_.push_back(ev);
+    pendingIters_.push_back(1);
     if (pending_.size() >= 256) resolveTiming(false);
   }
   launchCount_++;
 }
 
+std::pair<hipEvent_t, hipEvent_t> Solver::takeTimingEvents() {
+  if (freeEvents_.empty()) {
+    hipEvent_t a, b;
+    EP2D_HIP(hipEventCreate(&a));
+    EP2D_HIP(hipEventCreate(&b));
+    freeEvents_.push_back({a, b});
+  }
+  auto ev = freeEvents_.back();
+  freeEvents_.pop_back();
+  return ev;
+}
+
+// Two iterations in one launch; V is read from buffer vIdx_ ^ (q & 1) and
+// written to the other, so after an odd number of double launches the other
+// buffer is current (flipV).
+void Solver::launchDouble(int q) {
+  const bool timed = sampleEvery_ > 0 && (launchCount_ % sampleEvery_) == 0;
+  std::pair<hipEvent_t, hipEvent_t> ev{};
+  if (timed) ev = takeTimingEvents();
+  launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
+                timed ? ev.first : nullptr, timed ? ev.second : nullptr);
+  cur_ ^= 1;
+  stressStale_ = true;
+  if (timed) {
+    pending_.push_back(ev);
+    pendingIters_.push_back(2);
+    if (pending_.size() >= 256) resolveTiming(false);
+  }
+  launchCount_++;
+}
+
+void Solver::flipV() {
+  vIdx_ ^= 1;
+  std::swap(r_.Vx, r_.Vx2);
+  std::swap(r_.Vy, r_.Vy2);
+}
+
 void Solver::iterate(long n) {
   EP2D_HIP(hipSetDevice(device_));
-  for (long k = 0; k < n; k++) launchIteration();
+  if (!twoIter_) {
+    for (long k = 0; k < n; k++) launchIteration();
+    return;
+  }
+  // pairs first, then one or two single iterations (in place), so the last
+  // launch reads U^{n-1}, the state the stress fields are recomputed from
+  if (n <= 0) return;
+  const long singles = (n & 1) ? 1 : 2;
+  const long d = (n - singles) / 2;
+  for (long q = 0; q < d; q++) launchDouble((int)(q & 1));
+  if (d & 1) flipV();
+  for (long q = 0; q < singles; q++) launchIteration();
 }
 
 void Solver::resolveTiming(bool all) {
   const size_t keep = all ? 0 : 64;
   while (pending_.size() > keep) {
     auto ev = pending_.front();
+    const int its = pendingIters_.front();
     pending_.erase(pending_.begin());
+    pendingIters_.erase(pendingIters_.begin());
     EP2D_HIP(hipEventSynchronize(ev.second));
     float ms = 0.f;
     EP2D_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
-    timedMs_ += ms;
-    timedSamples_++;
+    timedMs_ += ms;          // averages are per iteration
+    timedSamples_ += its;
     freeEvents_.push_back(ev);
   }
 }
@@ -412,7 +486,11 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   if (name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
-    if (marchRows_ > 0)
+    if (marchRows_ > 0 && twoIter_)
+      nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2<float, float>" : "k_march2<double, double>") +
+           " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs2_) +
+           " (per iteration)";
+    else if (marchRows_ > 0)
       nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march<float, float>" : "k_march<double, double>") +
            " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs_);
     else if (mode_ != EP2D_MODE_PLAIN) nm += " TY=" + std::to_string(fplan_.ty) + " grid=" +
@@ -878,9 +956,10 @@ void Solver::computeEffDamping(double initLoadValue, const double loadType[3], d
 }
 
 // Iterations executed by a batched launch on this RVE's buffers.
-void Solver::noteBatchIterations(long n) {
+void Solver::noteBatchIterations(long n, long launches, long doubles) {
   if (mode_ != EP2D_MODE_PLAIN) {
-    if (n & 1) cur_ ^= 1;
+    if (launches & 1) cur_ ^= 1;
+    if (doubles & 1) flipV();
     if (n) stressStale_ = true;
   }
   tsDone_ += n;
@@ -1048,12 +1127,16 @@ void Solver::setField(int field, const double* host, size_t count) {
     case EP2D_FIELD_UX: h2d(r_.Ux[cur_], host, N); return;
     case EP2D_FIELD_UY: h2d(r_.Uy[cur_], host, N); return;
     case EP2D_FIELD_VX: case EP2D_FIELD_VY: {
-      void* dst = field == EP2D_FIELD_VX ? r_.Vx : r_.Vy;
+      // both buffers, so the velocities two-iteration launches never write
+      // (the boundary) agree
+      void* dst[2] = {field == EP2D_FIELD_VX ? r_.Vx : r_.Vy,
+                      field == EP2D_FIELD_VX ? r_.Vx2 : r_.Vy2};
+      const int nb = dst[0] == dst[1] ? 1 : 2;
       if (mode_ == EP2D_MODE_SPLIT) {
         std::vector<float> tmp(host, host + N);
-        h2d((float*)dst, tmp.data(), N);
+        for (int k = 0; k < nb; k++) h2d((float*)dst[k], tmp.data(), N);
       } else {
-        h2d((double*)dst, host, N);
+        for (int k = 0; k < nb; k++) h2d((double*)dst[k], host, N);
       }
       return;
     }

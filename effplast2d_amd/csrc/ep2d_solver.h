@@ -51,10 +51,13 @@ class Solver {
   void tsFinish();                // averages, boundary, hole area, analytic moduli
   void jobFinish(ep2d_result* out);
   void runJob(ep2d_result* out);
-  void noteBatchIterations(long n);
+  // after the batched engine ran n iterations of this RVE in `launches`
+  // launches, `doubles` of them two-iteration launches
+  void noteBatchIterations(long n, long launches, long doubles);
   const RveDev& dev() const { return r_; }
   const FusedPlan& fusedPlan() const { return fplan_; }
   int cur() const { return cur_; }
+  int vBuffer() const { return vIdx_; }
   int mode() const { return mode_; }
   bool plastic() const { return opt_.nTasks > 1; }
   int nX() const { return nX_; }
@@ -124,8 +127,15 @@ class Solver {
   StressDev f_{};
   FusedPlan fplan_{};
   int marchRows_ = 0, nSegs_ = 0;   // row-marching kernel when marchRows_ > 0
-  RveDev* dRve_ = nullptr;
+  RveDev* dRve_ = nullptr;          // device copy of r_ with Vx/Vy = vbuf 0, Vx2/Vy2 = vbuf 1
   int4* dSegs_ = nullptr;
+  bool twoIter_ = false;            // two iterations per launch (k_march2)
+  int nSegs2_ = 0;
+  int4* dSegs2_ = nullptr;
+  int vIdx_ = 0;                    // velocity buffer holding the current V
+  void flipV();                     // the other velocity buffer becomes current
+  std::pair<hipEvent_t, hipEvent_t> takeTimingEvents();
+  void launchDouble(int q);         // two iterations, q-th double launch of a run
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;
@@ -152,6 +162,7 @@ class Solver {
   long sampleEvery_ = 0;
   long launchCount_ = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pending_, freeEvents_;
+  std::vector<int> pendingIters_;   // iterations in each pending timed launch
   double timedMs_ = 0.0;
   long timedSamples_ = 0;
 

@@ -103,6 +103,7 @@ typedef struct ep2d_options {
                            (EffPlast2D.log, :1607) */
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
   int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
+  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch (same results) */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
@@ -259,6 +260,7 @@ typedef struct ep2d_batch_options {
   int tileRows;          /* fixed tile kernel of this height (6, 8, 14, 30)   */
   int marchRows;         /* fixed row-marching kernel, segment rows (> 0)      */
   int persistRows;       /* fixed persistent tile kernel of this height        */
+  int march2Rows;        /* fixed two-iteration marching kernel, segment rows  */
                          /* all 0: kernel and shape tuned on the running batch  */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
@@ -296,18 +298,21 @@ typedef struct ep2d_batch_stats {
   long long cellIterations;       /* sum over launches of RVE cells advanced  */
   double bytesAlgorithmic;        /* U and V read and written, all launches   */
   double timedMs, timedBytes;     /* sampled launches: duration and bytes     */
-  long long timedLaunches;
+  long long timedLaunches;        /* iterations in the sampled launches       */
   double graphMs;                 /* timed graph replays: total duration      */
   long long graphLaunches;        /* launches inside those replays            */
   int active, queued;
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
   int kernel;                     /* kernel in use: 1 tiles, 2 row marching,
-                                     3 persistent tiles                       */
+                                     3 persistent tiles, 4 row marching with
+                                     two iterations per launch                */
   int kernelRows;                 /* its tile height or segment rows          */
   int tunings;                    /* kernel tunings run                       */
-  double candUs[8];               /* last tuning, mean launch us: tiles of 8, 14,
-                                     30 rows; marching segments of 16, 32 rows;
-                                     persistent tiles of 6, 8, 14 rows        */
+  double candUs[10];              /* last tuning, mean us per iteration: tiles
+                                     of 8, 14, 30 rows; marching segments of
+                                     16, 32 rows; persistent tiles of 6, 8, 14
+                                     rows; two-iteration marching segments of
+                                     32, 64 rows                              */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;

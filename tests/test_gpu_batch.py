@@ -117,10 +117,10 @@ def test_tuned_kernel_equals_fixed():
     results."""
     _note("tuned vs fixed kernel shapes")
     tuned, st = _batch(REQS, "split")
-    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3)
+    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4)
     assert all(u > 0 for u in st["candUs"])
     for kw in ({"tileRows": 6}, {"tileRows": 30}, {"marchRows": 16}, {"marchRows": 128},
-               {"persistRows": 6}, {"persistRows": 14}):
+               {"persistRows": 6}, {"persistRows": 14}, {"march2Rows": 8}, {"march2Rows": 48}):
         fixed, st2 = _batch(REQS, "split", **kw)
         assert st2["tunings"] == 0
         assert st2["kernelRows"] == list(kw.values())[0]
@@ -172,3 +172,26 @@ def test_persistent_kernel_tiny_batch():
     b, _ = _batch(reqs, "fp64", tileRows=14)
     for t in range(len(reqs)):
         _same(a[t]["result"], b[t]["result"])
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_march2_batch_equals_single(mode, graphs):
+    """Two-iteration launches in the engine (pairs, then one or two single
+    iterations per chunk; odd check intervals included) give the single-RVE
+    results."""
+    _note(f"march2 batch {mode} graphs={graphs}")
+    got, st = _batch(REQS, mode, march2Rows=32, useGraphs=graphs, timingEvery=1)
+    assert st["kernel"] == 4
+    for t, rq in enumerate(REQS):
+        _same(got[t]["result"], _single(rq, mode))
+    nX, nP, phi, Y, nl, load = REQS[0]
+    pa = make_pa(nX, nPores=nP, porosity=phi, Y=Y)
+    with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=199,
+                    mode=mode) as g:
+        ref = g.ComputeEffModuli(load, LT)
+    with Batch(mode=mode, checkInterval=199, march2Rows=16, useGraphs=graphs) as b:
+        b.submit(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, initLoad=load, loadType=LT, tag=0)
+        b.run()
+        r = b.poll()[0]
+    _same(r["result"], ref)

@@ -122,6 +122,47 @@ def test_march_matches_oracle(rows, nX, nPores):
             assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
 
 
+@pytest.mark.parametrize("rows,iters", [(1, 7), (3, 12), (16, 1), (16, 2), (64, 9)])
+@pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (125, 3), (8, 1)])
+def test_march2_matches_oracle(rows, iters, nX, nPores):
+    """Two iterations per launch (temporal blocking), FP64: odd and even
+    iteration counts, every segment length, bit for bit against the oracle;
+    the stress fields come from U^{n-1}, which the last (single) launch read."""
+    nl = 3
+    _note(f"march2 fp64 rows={rows} iters={iters} nX={nX}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.03)
+    init = random_state(nX, 500 + rows + nX, pa=pa, boundary_v=True)
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode="fp64", marchRows=rows, itersPerLaunch=2) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS:
+            got = g.get(k)
+            e = rel_err(got, s[k])
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
+
+
+@pytest.mark.parametrize("nl", [1, 3])
+def test_march2_split_equals_tiles(nl):
+    nX, iters = 190, 21
+    _note(f"march2 split nl={nl}")
+    pa = make_pa(nX, nPores=2, porosity=0.04)
+    init = random_state(nX, 700 + nl, pa=pa, boundary_v=True)
+    out = []
+    for kw in ({"marchRows": 24, "itersPerLaunch": 2}, {"tileRows": 8}):
+        with EffPlast2D(pa, nX, nTasks=nl, mode="split", **kw) as g:
+            for k in STATE_FIELDS:
+                g.set(k, init[k])
+            g.iterate(iters)
+            g.iterate(iters + 1)
+            out.append({k: g.get(k) for k in STATE_FIELDS + STRESS_FIELDS})
+    for k in STATE_FIELDS + STRESS_FIELDS:
+        assert np.array_equal(out[0][k], out[1][k]), \
+            f"{k}: {np.count_nonzero(out[0][k] != out[1][k])} differ"
+
+
 @pytest.mark.parametrize("rows", [2, 32])
 @pytest.mark.parametrize("nl", [1, 3])
 def test_march_split_equals_tiles(rows, nl):
