@@ -148,13 +148,14 @@ def batch_stats_summary(st, mode):
     kms = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else float("nan")
     gbs = st["timedBytes"] / st["timedLaunches"] / (kms * 1e-3) / 1e9 if st["timedLaunches"] \
         else float("nan")
+    ipl = 2 if st["kernel"] == 4 else 1   # iterations per launch
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
             3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>"}[st["kernel"]]
-    return {"kernel": kern,
-            "kernel_avg_us": kms * 1e3, "kernel_samples": st["timedLaunches"],
-            "bytes_per_launch": st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"]
-            else float("nan"),
+    return {"kernel": kern, "iterations_per_launch": ipl,
+            "kernel_avg_us": ipl * kms * 1e3, "kernel_samples": st["timedLaunches"] // ipl,
+            "bytes_per_launch": ipl * st["timedBytes"] / st["timedLaunches"]
+            if st["timedLaunches"] else float("nan"),
             "achieved_GBs": gbs, "launches": st["launches"], "chunks": st["chunks"],
             "graph_avg_us_per_launch": 1e3 * st["graphMs"] / st["graphLaunches"]
             if st["graphLaunches"] else None,
