@@ -8,8 +8,8 @@ initLoadValue=-3e-3 along loadType [1,1,0], then the 0.5% hydrostatic
 increment and a pure xy shear increment (shear loadType [0,0,1]),
 nTimeSteps=1, eIter=1e-8, nIter=100000, residual checked every 500
 iterations.  The timed region submits K such requests to the batched engine
-in the FP32/FP64 split scheme (graph-captured chunks, tile height tuned on the
-running batch) and runs them to convergence; value = K / elapsed.  Inputs are
+in the FP32/FP64 split scheme (graph-captured chunks, kernel shape tuned on
+the running batch) and runs them to convergence; value = K / elapsed.  Inputs are
 generated on the device; nothing crosses PCIe but scalars, tables and sampled
 boundary values.
 
@@ -25,7 +25,8 @@ Beside it, in the same run:
   * the CPU oracle on this host's cores on a bounded sample.
 
 Printed: ONE JSON line; value and roofline are the headline's, the roofline's
-kernel time is the live dispatch-event timing of the batched kernel.
+kernel time is the live HIP-event timing of the batched kernel's graph
+replays on the engine's stream (the sampled single-dispatch figure beside it).
 
 Other entry points:
   --probe      a fixed 200-iteration run of one scheme plus a copy kernel of
@@ -135,32 +136,44 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-CAND_NAMES = ["tiles8", "tiles14", "tiles30", "march16", "march32", "persist6", "persist8",
-              "persist14", "march2x32", "march2x64"]
-
-
 def shape_name(kind, rows):
-    return f"{ {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x'}[kind]}{rows}"
+    return f"{ {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair'}[kind]}{rows}"
+
+
+def cand_us(st):
+    """Last tuning's mean us per iteration, by candidate shape name."""
+    return {shape_name(k, r): u for k, r, u in zip(st["candKind"], st["candRows"], st["candUs"])}
 
 
 def batch_stats_summary(st, mode):
-    """Kernel figures of a batched run from its sampled dispatch events."""
-    kms = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else float("nan")
-    gbs = st["timedBytes"] / st["timedLaunches"] / (kms * 1e-3) / 1e9 if st["timedLaunches"] \
-        else float("nan")
-    ipl = 2 if st["kernel"] == 4 else 1   # iterations per launch
+    """Kernel figures of a batched run.  The launch duration is the average
+    over the timed graph replays (HIP events on the engine's stream around
+    each replay of back-to-back launches of the chosen kernel) when graphs
+    ran, else over the sampled single launches between dispatch events; a
+    single bracketed launch also carries the dispatch latency of the launch
+    after a replay, so it reads longer than the kernel itself."""
+    nan = float("nan")
+    ipl = 2 if st["kernel"] in (4, 5) else 1   # iterations per launch
+    it_disp = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else nan
+    it_graph = st["graphMs"] / st["graphLaunches"] if st["graphLaunches"] else None
+    bpl = ipl * st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"] else nan
+    kms = ipl * (it_graph if it_graph is not None else it_disp)   # ms per launch
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
-            3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>"}[st["kernel"]]
+            3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>",
+            5: f"k_march2p<{ct}, {ct}>"}[st["kernel"]]
     return {"kernel": kern, "iterations_per_launch": ipl,
-            "kernel_avg_us": ipl * kms * 1e3, "kernel_samples": st["timedLaunches"] // ipl,
-            "bytes_per_launch": ipl * st["timedBytes"] / st["timedLaunches"]
-            if st["timedLaunches"] else float("nan"),
-            "achieved_GBs": gbs, "launches": st["launches"], "chunks": st["chunks"],
-            "graph_avg_us_per_launch": 1e3 * st["graphMs"] / st["graphLaunches"]
-            if st["graphLaunches"] else None,
+            "kernel_avg_us": kms * 1e3,
+            "kernel_timing": "graph replays" if it_graph is not None else "dispatch events",
+            "kernel_samples": (st["graphLaunches"] if it_graph is not None
+                               else st["timedLaunches"]) // ipl,
+            "kernel_avg_us_dispatch": ipl * it_disp * 1e3,
+            "bytes_per_launch": bpl,
+            "achieved_GBs": bpl / (kms * 1e-3) / 1e9,
+            "launches": st["launches"], "chunks": st["chunks"],
+            "graph_avg_us_per_launch": ipl * 1e3 * it_graph if it_graph is not None else None,
             "kernel_shape": shape_name(st["kernel"], st["kernelRows"]),
-            "tuning_us": dict(zip(CAND_NAMES, st["candUs"])),
+            "tuning_us": cand_us(st),
             "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"]}
 
 
@@ -198,7 +211,7 @@ def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
     r0 = res[0]["result"]
     log(f"[bench] batched {mode} x{steps}: {dt:.2f} s, {steps / dt:.2f} solves/s, "
         f"kernel {shape_name(st['kernel'], st['kernelRows'])} "
-        f"{ {k: round(v, 1) for k, v in zip(CAND_NAMES, st['candUs'])} }")
+        f"{ {k: round(v, 1) for k, v in cand_us(st).items()} }")
     out = {"mode": mode, "batched": steps, "solves_per_s": steps / dt, "elapsed_s": dt,
            "ms_per_solve": 1e3 * dt / steps, "p50_ms": 1e3 * percentile(lat, 0.5),
            "p99_ms": 1e3 * percentile(lat, 0.99), "iters_per_solve": tot_iters / steps,
@@ -370,10 +383,12 @@ def run_config5(rate, duration, seed, check, budget_gib):
     return out
 
 
-def read_traffic(mode, nx):
-    """Measured HBM bytes per launch of the dominant kernel, from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)."""
-    path = os.path.join(ROOT, "profiles", f"pmc_fused_{mode}_{nx}.txt")
+def read_traffic(shape, mode, nx):
+    """Measured HBM bytes per launch of the dominant kernel in the given shape
+    (e.g. "march2x32"), from the committed rocprofv3 PMC summary of that
+    kernel (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated); None when no
+    summary of that shape is committed."""
+    path = os.path.join(ROOT, "profiles", f"pmc_batch_{shape}_{mode}_{nx}.txt")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -408,7 +423,7 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0):
+def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
     preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
     over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
@@ -421,7 +436,8 @@ def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0):
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
-        shape = {"marchRows": march} if march else {"tileRows": tile or 8}
+        shape = ({"march2Rows": march2} if march2 else
+                 {"marchRows": march} if march else {"tileRows": tile or 8})
         run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False, **shape)
     else:
         with EffPlast2D(pa, nx, nTasks=3, mode=mode, tileRows=tile, marchRows=march) as g:
@@ -543,18 +559,21 @@ def pmc_traffic(fetch_dir, write_dir, kernel_key, copy_mib, dest, command):
     print(text)
 
 
-def run_config3(check):
+def run_config3(check, nIter=100_000):
     """4096^2, 16 pores (nPores=4), porosity 0.05, Y=2.5e-3, nTasks=3: the
     split scheme's moduli (including the periodic internal-cell ones) against
-    an FP64 run of the same case."""
+    an FP64 run of the same case.  Each load step runs to eIter=1e-8 or to
+    nIter iterations (the reference's NITER, EffPlast2D.h:29-31), whichever
+    comes first; both schemes get the same budget."""
     from effplast2d_amd import EffPlast2D, derive_params
     pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=4096, nPores=4, porosity=0.05, Y=2.5e-3, K0=1.0,
                            G0=0.5, rho0=1.0, CFL=0.5)
     out = {}
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
-        with EffPlast2D(pa, nx, nTasks=3, nIter=200_000, eIter=1e-8, checkInterval=check,
-                        mode=mode) as g:
+        with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
+                        mode=mode, shearLoadType=LOAD["shearLoadType"],
+                        itersPerLaunch=2) as g:
             g.set_timing(64)
             res = g.ComputeEffModuli(LOAD["initLoad"], LOAD["loadType"])
             ms, n, name = g.timing()
@@ -562,6 +581,8 @@ def run_config3(check):
         dt = time.perf_counter() - t0
         out[mode] = {"seconds": dt, "kernel_avg_us": ms * 1e3, "kernel": name,
                      "iters": [int(r["iters"]) for r in recs],
+                     "errors": [float(r["error"]) for r in recs],
+                     "converged": bool(res["converged"]),
                      "num": res["num"], "num_per": res["num_per"], "Gxy": res["Gxy"],
                      "GxyPer": res["GxyPer"]}
         log(f"[bench] config3 {mode}: {dt:.1f} s, iters {out[mode]['iters']}")
@@ -607,6 +628,8 @@ def main():
                     help="probe the batched kernel with this many headline RVEs")
     ap.add_argument("--probe-tile", type=int, default=0, help="probe: tile height")
     ap.add_argument("--probe-march", type=int, default=0, help="probe: marching segment rows")
+    ap.add_argument("--probe-march2", type=int, default=0,
+                    help="probe: two-iteration marching segment rows (batched only)")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
@@ -633,7 +656,7 @@ def main():
         return 2
     if a.probe:
         probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
-              a.probe_march)
+              a.probe_march, a.probe_march2)
         return 0
     if a.sweep:
         shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]
@@ -670,7 +693,7 @@ def main():
         legs["config5_serving"] = run_config5(a.serve_rate, a.serve_seconds, a.serve_seed,
                                               a.check_interval, a.serve_budget_gib)
     frac = head["achieved_GBs"] / HBM_PEAK_GBS
-    traffic = read_traffic(f"batch_{a.mode}", nx)
+    traffic = read_traffic(head["kernel_shape"], a.mode, nx)
     out = {
         "metric": METRIC,
         "value": head["solves_per_s"],
@@ -695,6 +718,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": frac, "traffic": traffic,
                      "kernel": head["kernel"], "kernel_avg_us": head["kernel_avg_us"],
+                     "kernel_timing": head["kernel_timing"],
+                     "kernel_avg_us_dispatch": head["kernel_avg_us_dispatch"],
                      "bytes_per_launch": head["bytes_per_launch"],
                      "kernel_samples": head["kernel_samples"],
                      "kernel_shape": head["kernel_shape"]},

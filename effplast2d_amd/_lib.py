@@ -91,6 +91,7 @@ class Result(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("checkInterval", C.c_long), ("tileRows", C.c_int),
                 ("marchRows", C.c_int), ("persistRows", C.c_int), ("march2Rows", C.c_int),
+                ("pairRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
                 ("useGraphs", C.c_int), ("timingEvery", C.c_long)]
 
@@ -116,6 +117,9 @@ class BatchResult(C.Structure):
                 "message": self.message.decode(errors="replace")}
 
 
+MAX_CAND = 24  # EP2D_MAX_CAND
+
+
 class BatchStats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in ("launches", "chunks", "tilesLaunched", "submitted",
                                             "finished", "failed", "cellIterations")] + \
@@ -124,11 +128,14 @@ class BatchStats(C.Structure):
                 ("graphMs", C.c_double), ("graphLaunches", C.c_longlong),
                 ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t),
                 ("kernel", C.c_int), ("kernelRows", C.c_int), ("tunings", C.c_int),
-                ("candUs", C.c_double * 10)]
+                ("nCand", C.c_int), ("candKind", C.c_int * MAX_CAND),
+                ("candRows", C.c_int * MAX_CAND), ("candUs", C.c_double * MAX_CAND)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
-        d["candUs"] = list(self.candUs)
+        n = self.nCand
+        for k in ("candKind", "candRows", "candUs"):
+            d[k] = list(getattr(self, k))[:n]
         return d
 
 

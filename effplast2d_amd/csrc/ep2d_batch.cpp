@@ -27,8 +27,10 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 // tile kernel at three tile heights and the two-iteration marching kernel at
 // two segment lengths.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8},   {kTiles, 14},  {kTiles, 30},   {kMarch, 16},  {kMarch, 32},
-    {kPersist, 6}, {kPersist, 8}, {kPersist, 14}, {kMarch2, 32}, {kMarch2, 64}};
+    {kTiles, 8},    {kTiles, 14},   {kTiles, 30},   {kMarch, 16},   {kMarch, 32},
+    {kPersist, 6},  {kPersist, 8},  {kPersist, 14}, {kMarch2, 24},  {kMarch2, 32},
+    {kMarch2, 40},  {kMarch2, 48},  {kMarch2, 64},  {kPair, 24},    {kPair, 32},
+    {kPair, 48},    {kPair, 64}};
 
 namespace {
 
@@ -44,15 +46,17 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.persistRows != 0 && !valid_tile_rows(opt_.persistRows))
     throw ArgError("persistRows must be 0, 6, 8, 14 or 30");
   if (opt_.march2Rows < 0) throw ArgError("march2Rows must be >= 0");
+  if (opt_.pairRows < 0) throw ArgError("pairRows must be >= 0");
   const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +
-                     (opt_.march2Rows != 0);
+                     (opt_.march2Rows != 0) + (opt_.pairRows != 0);
   if (nfixed > 1)
-    throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows");
+    throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows, pairRows");
   fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
   else if (opt_.persistRows) shape_ = {kPersist, opt_.persistRows};
   else if (opt_.march2Rows) shape_ = {kMarch2, opt_.march2Rows};
+  else if (opt_.pairRows) shape_ = {kPair, opt_.pairRows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -231,7 +235,7 @@ int Batch::buildWork(Shape sh, int4* out) const {
       t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
-    if (sh.kind == kMarch2) {
+    if (twoIter(sh.kind)) {
       t += march2_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
@@ -253,9 +257,10 @@ int Batch::buildWork(Shape sh, int4* out) const {
 // buffer, so they are only enqueued when d is even.
 void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d, hipEvent_t e0,
                       hipEvent_t e1) {
-  if (sh.kind == kMarch2 && dbl)
-    launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1);
-  else if (sh.kind == kMarch2)
+  if (twoIter(sh.kind) && dbl)
+    launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1,
+                  sh.kind == kPair);
+  else if (twoIter(sh.kind))
     launch_march(dRves_, dTiles_ + auxOff_, auxCnt_, opt_.mode, (int)(l & 1), (int)(d & 1),
                  stream_, e0, e1);
   else if (sh.kind == kMarch)
@@ -295,7 +300,7 @@ hipGraphExec_t Batch::graphFor(long len) {
   g.len = len;
   long l = 0, d = 0;
   EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  enqueue(len, shape_.kind == kMarch2, l, d);
+  enqueue(len, twoIter(shape_.kind), l, d);
   EP2D_HIP(hipStreamEndCapture(stream_, &g.graph));
   EP2D_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
   graphs_.push_back(g);
@@ -320,10 +325,13 @@ Batch::Seq Batch::launchChunk(long c) {
   }
   if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
-  constexpr long kTuneEach = 4;  // timed rounds
+  // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
+  // every candidate, as many as the chunk holds (at least one timed round)
+  constexpr long kTuneEach = 4;
   long roundIters = 0;
-  for (int k = 0; k < kNCand; k++) roundIters += kCand[k].kind == kMarch2 ? 4 : 1;
-  const bool tuning = !fixed_ && tuneDue_ && c >= (kTuneEach + 1) * roundIters + 2;
+  for (int k = 0; k < kNCand; k++) roundIters += twoIter(kCand[k].kind) ? 4 : 1;
+  const long rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
+  const bool tuning = !fixed_ && tuneDue_ && rounds >= 2;
   ids.push_back(tuning ? 1 : 0);
   ids.push_back(shape_.kind);
   ids.push_back(shape_.rows);
@@ -331,7 +339,7 @@ Batch::Seq Batch::launchChunk(long c) {
     dropGraphs();
     const Shape aux{kMarch, shape_.rows};
     size_t need = (size_t)buildWork(shape_, nullptr);
-    if (shape_.kind == kMarch2) need += (size_t)buildWork(aux, nullptr);
+    if (twoIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
     if (tuning)
       for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
     ensureCapacity(active_.size(), need);
@@ -339,7 +347,7 @@ Batch::Seq Batch::launchChunk(long c) {
     int off = nTiles_;
     auxOff_ = off;
     auxCnt_ = 0;
-    if (shape_.kind == kMarch2) {
+    if (twoIter(shape_.kind)) {
       auxCnt_ = buildWork(aux, hTiles_ + off);
       off += auxCnt_;
     }
@@ -369,12 +377,12 @@ Batch::Seq Batch::launchChunk(long c) {
   }
   long l = 0, d = 0, it = 0;
   if (tuning) {
-    // a warm-up round, then kTuneEach timed rounds, candidates interleaved
-    for (long round = 0; round <= kTuneEach; round++)
+    // a warm-up round, then the timed rounds, candidates interleaved
+    for (long round = 0; round < rounds; round++)
       for (int k = 0; k < kNCand; k++) {
         const Shape sh = kCand[k];
         const int4* w = dTiles_ + candOff_[k];
-        const bool dbl = sh.kind == kMarch2;
+        const bool dbl = twoIter(sh.kind);
         for (int rep = 0; rep < (dbl ? 2 : 1); rep++) {
           Pending ev{};
           if (round > 0) {
@@ -396,7 +404,7 @@ Batch::Seq Batch::launchChunk(long c) {
   }
   const long rest = c - it;
   const bool timed = opt_.timingEvery > 0;
-  const bool two = shape_.kind == kMarch2;
+  const bool two = twoIter(shape_.kind);
   long pairs = 0, singles = rest;
   if (two) split_two(rest, pairs, singles);
   const long main = two ? pairs : singles;   // launches of the main kernel
@@ -575,6 +583,11 @@ void Batch::stats(ep2d_batch_stats* s) const {
   s->deviceBytes = used_;
   s->kernel = shape_.kind;
   s->kernelRows = shape_.rows;
+  s->nCand = kNCand;
+  for (int k = 0; k < kNCand; k++) {
+    s->candKind[k] = kCand[k].kind;
+    s->candRows[k] = kCand[k].rows;
+  }
 }
 
 }  // namespace ep2d

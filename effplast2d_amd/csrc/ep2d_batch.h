@@ -45,13 +45,16 @@ class Batch {
     int cand = -1;     // shape candidate under tuning, -1 for statistics
     bool graph = false;  // a whole graph replay (launches counted in graphLaunches)
   };
-  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4;
+  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4, kPair = 5;
+  static bool twoIter(int kind) { return kind == kMarch2 || kind == kPair; }
   struct Shape {
-    int kind = kTiles;  // fused tile, row-marching, persistent tile or two-iteration marching
+    int kind = kTiles;  // fused tile, row-marching, persistent tile, two-iteration
+                        // marching on one wave or on a pair of waves
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 10;
+  static constexpr int kNCand = 17;
   static const Shape kCand[kNCand];
+  static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
     long len = 0;
     hipGraph_t graph = nullptr;

@@ -124,11 +124,13 @@ void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int
                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Two iterations per launch: strips of 56 output columns; U read from U[cur]
 // (cur = parity ^ rve.cur) and written to U[cur^1]; V read from Vx/Vy when
-// vpar = 0 (else Vx2/Vy2) and written to the other pair.
+// vpar = 0 (else Vx2/Vy2) and written to the other pair.  pairs: each segment
+// on two waves (one per iteration, rows handed over in LDS), else on one.
 constexpr int kMarch2Cols = 56;
 int march2_segments(int nX, int nY, int rows, int rve, int4* out);
 void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                   bool pairs = false);
 // batched fused iteration over many RVEs: tiles[k] = (rve, i0, j0, parity
 // offset); RVE k reads U[parity ^ offset].  All RVEs share mode and tile height.
 void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,

@@ -252,6 +252,35 @@ __device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
   *(GT*)((GC*)base + idx * (unsigned)sizeof(T)) = v;
 }
 
+// Raw buffer access for row marches: a resource (base in SGPRs; no range
+// limit, every access is in bounds by construction), a per-lane byte offset
+// fixed along the march and the row's byte offset, uniform, in an SGPR, so
+// stepping to the next row costs no vector instruction.  readfirstlane states
+// the row offset's uniformity (a value the compiler kept in a VGPR then costs
+// one read, not a loop over lanes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+}
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  soff = __builtin_amdgcn_readfirstlane(soff);
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0));
+}
+template <typename T>
+__device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, T v) {
+  soff = __builtin_amdgcn_readfirstlane(soff);
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
 // Workgroup barrier that waits for LDS traffic only: global loads issued
 // before it (the next tile's prefetch) stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -1276,8 +1305,12 @@ struct MarchStage {
   const CT* gt;
   int nX, nY, i;
   bool colC, colV, leftEdge, rightEdge, inX, plastic;
-  CT coh, c2lo, K0, Kp, G0, Gp, P0, vdampX, vdampY, dTrho;
+  CT coh, c2lo, K0, Kp, G2_0, G2_p, P0, vdampX, vdampY, dTrho;
   double dT;
+  // grid steps held in registers: the stores of a step may alias the RVE
+  // table, so values read from it would be read again after every store
+  double dXd, dYd, rdXd, rdYd;
+  CT rdXc, rdYc;
   // carried rows: U rows j and j+1, mask row j, trial cells j (and j-1),
   // trial vertices j-1 and j-2, corrected row j-1
   double ux_c, uxs_c, uy_c, uy_n;
@@ -1307,13 +1340,29 @@ struct MarchStage {
     else c2lo = rr.coh2lof;
     K0 = (CT)rr.K0;
     Kp = (CT)rr.Kp;
-    G0 = (CT)rr.G0;
-    Gp = (CT)rr.Gp;
+    G2_0 = (CT)2.0 * (CT)rr.G0;  // 2G, exact
+    G2_p = (CT)2.0 * (CT)rr.Gp;
     P0 = (CT)rr.P0;
+    dXd = rr.dX;
+    dYd = rr.dY;
+    rdXd = rr.rdX;
+    rdYd = rr.rdY;
+    rdXc = (CT)rr.rdX;
+    rdYc = (CT)rr.rdY;
     vdampX = (CT)rr.vdampX;
     vdampY = (CT)rr.vdampY;
     dTrho = (CT)rr.dTrho;
     dT = rr.dT;
+  }
+
+  // difference quotients: March::dx / March::dy with the steps in registers
+  __device__ __forceinline__ CT dxs(double a) const {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, dXd, rdXd);
+    else return (CT)a * rdXc;
+  }
+  __device__ __forceinline__ CT dys(double a) const {
+    if constexpr (sizeof(CT) == 8) return qdiv(a, dYd, rdYd);
+    else return (CT)a * rdYc;
   }
 
   // Ux row jb, Uy row jb+1, mask row jb (jb = first step)
@@ -1331,18 +1380,17 @@ struct MarchStage {
 
   __device__ __forceinline__ Out step(const int j, const double ux_n, const double uy_nn,
                                       const unsigned m_n, const CT vx_in, const CT vy_in) {
-    const RveDev& rr = *r;
     // trial cell row j+1
     const double uxs_n = lane_next(ux_n);
     CT tP_n, tXX_n, tYY_n;
     {
       const CT K = (m_n & M_CELL_MAT) ? Kp : K0;
-      const CT G = (m_n & M_CELL_MAT) ? Gp : G0;
-      const CT exx = M::dx(uxs_n - ux_n, rr);
-      const CT eyy = M::dy(uy_nn - uy_n, rr);
+      const CT G2 = (m_n & M_CELL_MAT) ? G2_p : G2_0;
+      const CT exx = dxs(uxs_n - ux_n);
+      const CT eyy = dys(uy_nn - uy_n);
       CT pp = P0 - K * (exx + eyy);
-      CT txx = (CT)2.0 * G * (exx - M::third(exx + eyy));
-      CT tyy = (CT)2.0 * G * (eyy - M::third(exx + eyy));
+      CT txx = G2 * (exx - M::third(exx + eyy));
+      CT tyy = G2 * (eyy - M::third(exx + eyy));
       const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
       if ((m_n & M_CELL_ZERO) || !ok) {
         pp = 0;
@@ -1359,7 +1407,7 @@ struct MarchStage {
     CT tXY_c;
     {
       const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
-      CT txy = g * (M::dy(uxs_n - uxs_c, rr) + M::dx(uys_n - uy_n, rr));
+      CT txy = g * (dys(uxs_n - uxs_c) + dxs(uys_n - uy_n));
       const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
       if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
       tXY_c = txy;
@@ -1410,9 +1458,9 @@ struct MarchStage {
     o.updX = !EDGE || (i > 0 && i < nX && j > 0 && j < nY - 1);
     o.updY = !EDGE || (i > 0 && i < nX - 1 && j > 0 && j < nY);
     if (o.updX)
-      vx = (CT)(VT)(vx * vdampX + dTrho * (M::dx(-cP_c + rP + cXX_c - rXX, rr) + M::dy(rXY_c - rXY_p, rr)));
+      vx = (CT)(VT)(vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p)));
     if (o.updY)
-      vy = (CT)(VT)(vy * vdampY + dTrho * (M::dy(-cP_c + cP_p + cYY_c - cYY_p, rr) + M::dx(cXY_p - rXY_p, rr)));
+      vy = (CT)(VT)(vy * vdampY + dTrho * (dys(-cP_c + cP_p + cYY_c - cYY_p) + dxs(cXY_p - rXY_p)));
     const bool in = !EDGE || (inX && j >= 0 && j < nY);
     o.vx = vx;
     o.vy = vy;
@@ -1563,6 +1611,202 @@ __global__ __launch_bounds__(256) void k_march2(const RveDev* __restrict__ rves,
   const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 4 && j1 <= r.nY - 3;
   if (interior) march2_run<CT, VT, false>(r, cur, vpar, s, j0, j1, gt[wave]);
   else march2_run<CT, VT, true>(r, cur, vpar, s, j0, j1, gt[wave]);
+}
+
+// ---------------------------------------------------------------------------
+// Two iterations per launch on a pair of waves.  Of each pair of waves the
+// first runs stage A (iteration n+1 from the state in HBM) and the second
+// stage B (iteration n+2) three rows behind; A hands every row's outputs (Ux,
+// Uy, V, and the mask row it read) to B through a four-row LDS ring, and the
+// workgroup advances one row per barrier.  A wave holds one stage's state
+// instead of two, so about twice as many waves fit on a SIMD to hide the
+// memory latency.  Segments, arithmetic and outputs are those of k_march2.
+template <typename CT>
+struct PairRing {
+  double ux[4][64];
+  double uy[4][64];
+  CT vx[4][64];
+  CT vy[4][64];
+  unsigned m[4][64];
+};
+
+// Lockstep step q (0 <= q < nSteps) is row t = j0-4+q of stage A and row
+// t-3 of stage B.  A at row t writes ring slot q&3; B at row j reads Ux' of
+// row j+1 (slot (q-2)&3), Uy' of row j+2 (slot (q-1)&3), and V' of row j and
+// mask row j+1 (slot (q-3)&3), so reads and the write of one step never share
+// a slot and the barrier after each step orders a write before its reads.
+template <typename CT, typename VT, bool EDGE>
+__device__ __forceinline__ void march2p_run(const RveDev& r, int cur, int vpar, int s, int j0,
+                                            int j1, const CT* gt, bool stageB,
+                                            PairRing<CT>& ring, int nSteps) {
+  using M = March<CT, VT>;
+  using Ld = typename M::Ld;
+  const int nX = r.nX, nY = r.nY, sx = nX + 1;
+  const int L = threadIdx.x & 63;
+  const int i = s + L;
+  const int jbA = j0 - 4, jbB = j0 - 2, tEnd = j1 + 1;
+  MarchStage<CT, VT, EDGE> st;
+  st.setup(r, gt, i);
+  if (!stageB) {
+    const double* __restrict__ Uxi = r.Ux[cur];
+    const double* __restrict__ Uyi = r.Uy[cur];
+    const VT* Vxi = static_cast<const VT*>(vpar ? r.Vx2 : r.Vx);
+    const VT* Vyi = static_cast<const VT*>(vpar ? r.Vy2 : r.Vy);
+    {
+      double ux = 0.0, uy = 0.0;
+      unsigned m = 0;
+      const bool colC = !EDGE || (i >= 0 && i < nX);
+      if (!EDGE) {
+        ux = gld(Uxi, (unsigned)(jbA * sx + i));
+        uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
+        m = gld(r.mask, (unsigned)(jbA * nX + i));
+      } else {
+        if (i >= 0 && i <= nX && jbA >= 0 && jbA < nY) ux = gld(Uxi, (unsigned)(jbA * sx + i));
+        if (colC && jbA + 1 >= 0 && jbA + 1 <= nY) uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
+        if (colC && jbA >= 0 && jbA < nY) m = gld(r.mask, (unsigned)(jbA * nX + i));
+      }
+      st.start(ux, uy, m);
+    }
+    // step t reads Ux row t+1, Uy row t+2, mask row t+1, V row t (interior
+    // segments clamp to rows in range, edge loads are guarded)
+    const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxi), bUy = brsrc(Uyi), bVx = brsrc(Vxi),
+                                 bVy = brsrc(Vyi), bM = brsrc(r.mask);
+    auto fetch = [&](int t0) {
+      const int t = EDGE ? t0 : min(t0, tEnd);
+      Ld d;
+      if (!EDGE) {
+        d.ux = bld<double>(bUx, o8, (unsigned)(t + 1) * (unsigned)sx * 8u);
+        d.uy = bld<double>(bUy, o8, (unsigned)(t + 2) * (unsigned)nX * 8u);
+        d.m = bld<unsigned char>(bM, (unsigned)i, (unsigned)(t + 1) * (unsigned)nX);
+        d.vx = bld<VT>(bVx, oV, (unsigned)t * (unsigned)sx * (unsigned)sizeof(VT));
+        d.vy = bld<VT>(bVy, oV, (unsigned)t * (unsigned)nX * (unsigned)sizeof(VT));
+      } else {
+        const bool ci = i >= 0 && i < nX;
+        d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
+        d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
+        d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(r.mask, (unsigned)((t + 1) * nX + i)) : 0u;
+        const bool vr = ci && t >= 0 && t < nY;
+        d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
+        d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
+      }
+      return d;
+    };
+    auto stepA = [&](int q, const Ld& ld, int sl) {
+      const int t = jbA + q;
+      if (t <= tEnd) {
+        const auto o = st.step(t, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
+        ring.ux[sl][L] = o.ux;
+        ring.uy[sl][L] = o.uy;
+        ring.vx[sl][L] = o.vx;
+        ring.vy[sl][L] = o.vy;
+        ring.m[sl][L] = ld.m;
+      }
+      lds_barrier();
+    };
+    // two row buffers (an unroll by four, for constant ring slots, would
+    // keep more registers live than the immediate offsets save)
+    Ld P = fetch(jbA);
+    for (int q = 0; q < nSteps; q += 2) {
+      const Ld Q = fetch(jbA + q + 1);
+      stepA(q, P, q & 3);
+      if (q + 1 >= nSteps) break;
+      P = fetch(jbA + q + 2);
+      stepA(q + 1, Q, (q + 1) & 3);
+    }
+  } else {
+    double* __restrict__ Uxo = r.Ux[cur ^ 1];
+    double* __restrict__ Uyo = r.Uy[cur ^ 1];
+    VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
+    VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
+    const bool outLane = L >= 4 && L < 60 && (!EDGE || i < nX);
+    const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo), bVx = brsrc(Vxo),
+                                 bVy = brsrc(Vyo);
+    // slot phase ph = q & 3 known at compile time (the loop is unrolled by
+    // four): ring addresses are immediate offsets
+    auto stepB = [&](int q, auto ph) {
+      constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;  // q-1, q-2, q-3
+      const int t = jbA + q;
+      if (t == jbB + 2) {
+        st.start(ring.ux[s2][L], ring.uy[s1][L], ring.m[s3][L]);
+      } else if (t >= jbB + 3 && t <= tEnd + 1) {
+        const int j = t - 3;
+        const double uxn = ring.ux[s2][L], uynn = ring.uy[s1][L];
+        const unsigned mn = ring.m[s3][L];
+        const CT vxi = ring.vx[s3][L], vyi = ring.vy[s3][L];
+        const double uxsB = st.uxs_c;  // Ux(i+1, j) of B's input, before the step
+        const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
+        if (j >= j0 && outLane) {
+          if (!EDGE) {
+            // interior: every lane's V is updated
+            bst(bVx, oV, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
+            bst(bVy, oV, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
+            bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
+            bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
+          } else {
+            if (o.updX) gst(Vxo + (size_t)j * sx, (unsigned)i, (VT)o.vx);
+            if (o.updY) gst(Vyo + (size_t)j * nX, (unsigned)i, (VT)o.vy);
+            gst(Uxo + (size_t)j * sx, (unsigned)i, o.ux);
+            gst(Uyo + (size_t)j * nX, (unsigned)i, o.uy);
+          }
+          if (EDGE) {
+            // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+            if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsB);
+            if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
+          }
+        }
+      }
+      lds_barrier();
+    };
+    for (int q = 0; q < nSteps; q += 4) {
+      stepB(q, std::integral_constant<int, 0>{});
+      if (q + 1 >= nSteps) break;
+      stepB(q + 1, std::integral_constant<int, 1>{});
+      if (q + 2 >= nSteps) break;
+      stepB(q + 2, std::integral_constant<int, 2>{});
+      if (q + 3 >= nSteps) break;
+      stepB(q + 3, std::integral_constant<int, 3>{});
+    }
+  }
+}
+
+template <typename CT, typename VT>
+__global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves,
+                                                 const int4* __restrict__ segs, int nsegs,
+                                                 int parity, int vpar) {
+  __shared__ CT gt[4][16];
+  __shared__ PairRing<CT> ring[2];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b = tile_of_block(blockIdx.x, gridDim.x);
+  const int k0 = 2 * b;
+  if (k0 >= nsegs) return;  // the whole workgroup
+  // the workgroup marches the longer of its two segments, one barrier a step
+  const int4 s0 = segs[k0];
+  int nSteps = s0.w - s0.z + 7;
+  if (k0 + 1 < nsegs) {
+    const int4 s1 = segs[k0 + 1];
+    nSteps = max(nSteps, s1.w - s1.z + 7);
+  }
+  const int k = k0 + (wave >> 1);
+  if (k >= nsegs) {  // a pair without a segment keeps the barrier count
+    for (int q = 0; q < nSteps; q++) lds_barrier();
+    return;
+  }
+  const int4 sg = segs[k];
+  const RveDev& r = rves[sg.x];
+  if (lane < 16) gt[wave][lane] = (CT)r.gtab[lane];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const int cur = parity ^ r.cur;
+  const int s = sg.y, j0 = sg.z, j1 = sg.w;
+  const bool stageB = (wave & 1) != 0;
+  const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 4 && j1 <= r.nY - 3;
+  if (interior)
+    march2p_run<CT, VT, false>(r, cur, vpar, s, j0, j1, gt[wave], stageB, ring[wave >> 1], nSteps);
+  else
+    march2p_run<CT, VT, true>(r, cur, vpar, s, j0, j1, gt[wave], stageB, ring[wave >> 1], nSteps);
 }
 
 }  // namespace
@@ -1717,17 +1961,23 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
 }
 
 void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                   int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                   int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1, bool pairs) {
   if (nsegs <= 0) return;
-  const dim3 grid((nsegs + 3) / 4), block(256);
+  const dim3 grid(pairs ? (nsegs + 1) / 2 : (nsegs + 3) / 4), block(256);
   auto go = [&](auto kern) {
     if (e0)
       hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
     else
       hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
   };
-  if (mode == 2) go(k_march2<float, float>);  // split: FP32 stress and velocity
-  else go(k_march2<double, double>);
+  if (pairs) {
+    if (mode == 2) go(k_march2p<float, float>);
+    else go(k_march2p<double, double>);
+  } else if (mode == 2) {
+    go(k_march2<float, float>);  // split: FP32 stress and velocity
+  } else {
+    go(k_march2<double, double>);
+  }
   EP2D_HIP(hipGetLastError());
 }
 

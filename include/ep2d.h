@@ -261,6 +261,7 @@ typedef struct ep2d_batch_options {
   int marchRows;         /* fixed row-marching kernel, segment rows (> 0)      */
   int persistRows;       /* fixed persistent tile kernel of this height        */
   int march2Rows;        /* fixed two-iteration marching kernel, segment rows  */
+  int pairRows;          /* fixed two-iteration marching on wave pairs, rows   */
                          /* all 0: kernel and shape tuned on the running batch  */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
@@ -293,6 +294,8 @@ typedef struct ep2d_batch_result {
   char message[160];     /* error text when status != EP2D_OK                  */
 } ep2d_batch_result;
 
+#define EP2D_MAX_CAND 24
+
 typedef struct ep2d_batch_stats {
   long long launches, chunks, tilesLaunched, submitted, finished, failed;
   long long cellIterations;       /* sum over launches of RVE cells advanced  */
@@ -305,14 +308,15 @@ typedef struct ep2d_batch_stats {
   size_t deviceBytes;             /* device memory held by admitted RVEs      */
   int kernel;                     /* kernel in use: 1 tiles, 2 row marching,
                                      3 persistent tiles, 4 row marching with
-                                     two iterations per launch                */
+                                     two iterations per launch, 5 the same on
+                                     pairs of waves (one iteration each)       */
   int kernelRows;                 /* its tile height or segment rows          */
   int tunings;                    /* kernel tunings run                       */
-  double candUs[10];              /* last tuning, mean us per iteration: tiles
-                                     of 8, 14, 30 rows; marching segments of
-                                     16, 32 rows; persistent tiles of 6, 8, 14
-                                     rows; two-iteration marching segments of
-                                     32, 64 rows                              */
+  int nCand;                      /* kernel shapes the tuner compares        */
+  int candKind[EP2D_MAX_CAND];    /* candidate k: kernel (as `kernel`)       */
+  int candRows[EP2D_MAX_CAND];    /* candidate k: tile height / segment rows */
+  double candUs[EP2D_MAX_CAND];   /* last tuning, mean us per iteration of
+                                     candidate k (0 if not timed)             */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;

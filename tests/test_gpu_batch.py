@@ -117,10 +117,11 @@ def test_tuned_kernel_equals_fixed():
     results."""
     _note("tuned vs fixed kernel shapes")
     tuned, st = _batch(REQS, "split")
-    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4)
+    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4, 5)
     assert all(u > 0 for u in st["candUs"])
     for kw in ({"tileRows": 6}, {"tileRows": 30}, {"marchRows": 16}, {"marchRows": 128},
-               {"persistRows": 6}, {"persistRows": 14}, {"march2Rows": 8}, {"march2Rows": 48}):
+               {"persistRows": 6}, {"persistRows": 14}, {"march2Rows": 8}, {"march2Rows": 48},
+               {"pairRows": 8}, {"pairRows": 40}):
         fixed, st2 = _batch(REQS, "split", **kw)
         assert st2["tunings"] == 0
         assert st2["kernelRows"] == list(kw.values())[0]
@@ -176,13 +177,14 @@ def test_persistent_kernel_tiny_batch():
 
 @pytest.mark.parametrize("mode", ["fp64", "split"])
 @pytest.mark.parametrize("graphs", [True, False])
-def test_march2_batch_equals_single(mode, graphs):
+@pytest.mark.parametrize("shape", ["march2Rows", "pairRows"])
+def test_march2_batch_equals_single(mode, graphs, shape):
     """Two-iteration launches in the engine (pairs, then one or two single
-    iterations per chunk; odd check intervals included) give the single-RVE
-    results."""
-    _note(f"march2 batch {mode} graphs={graphs}")
-    got, st = _batch(REQS, mode, march2Rows=32, useGraphs=graphs, timingEvery=1)
-    assert st["kernel"] == 4
+    iterations per chunk; odd check intervals included), on one wave or on a
+    pair of waves per segment, give the single-RVE results."""
+    _note(f"{shape} batch {mode} graphs={graphs}")
+    got, st = _batch(REQS, mode, useGraphs=graphs, timingEvery=1, **{shape: 32})
+    assert st["kernel"] == (4 if shape == "march2Rows" else 5)
     for t, rq in enumerate(REQS):
         _same(got[t]["result"], _single(rq, mode))
     nX, nP, phi, Y, nl, load = REQS[0]
@@ -190,7 +192,7 @@ def test_march2_batch_equals_single(mode, graphs):
     with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=199,
                     mode=mode) as g:
         ref = g.ComputeEffModuli(load, LT)
-    with Batch(mode=mode, checkInterval=199, march2Rows=16, useGraphs=graphs) as b:
+    with Batch(mode=mode, checkInterval=199, useGraphs=graphs, **{shape: 16}) as b:
         b.submit(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, initLoad=load, loadType=LT, tag=0)
         b.run()
         r = b.poll()[0]
