@@ -53,7 +53,7 @@ struct RveDev {
   int plastic;              // 1: return mapping on (nTasks > 1, EffPlast2D.cu:392)
   int cur;      // batched path: buffer index of the current displacements
   double coh2lo;            // below coh^2: a smaller squared J2 cannot exceed coh
-  float coh2lof;            // the same bound for the FP32 arithmetic of the split scheme
+  float coh2lof;            // split scheme: coh^2 in FP32, the yield test of its FP32 stress
   double gtab[16];          // vertex shear modulus by vertex class (mask bits 3..6)
 };
 

@@ -281,6 +281,34 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned voff, uns
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 
+// Return-mapping scale (ComputeJ2 / ComputePlasticity, EffPlast2D.cu:109-176):
+// a stress whose squared invariant s2 lies outside the yield radius coh is
+// scaled by sc = coh / sqrt(s2); returns whether it is.  FP64 keeps the
+// reference's test and quotient (sqrt, compare, divide), skipped when s2 <= c2
+// = coh2lo, below which sqrt(s2) <= coh exactly.  The FP32 split scheme tests
+// s2 against c2 = coh^2 (rounded to FP32) and scales by coh * rsq(s2), the
+// hardware reciprocal square root (about one ulp): the scaled stress lies on
+// the yield surface to FP32 accuracy either way.
+template <typename CT>
+__device__ __forceinline__ bool rm_scale(CT s2, CT coh, CT c2, CT& sc) {
+  if constexpr (sizeof(CT) == 8) {
+    if (s2 > c2) {
+      const CT J = sqrt(s2);
+      if (J > coh) {
+        sc = coh / J;
+        return true;
+      }
+    }
+    return false;
+  } else {
+    if (s2 > c2) {
+      sc = coh * __builtin_amdgcn_rsqf(s2);
+      return true;
+    }
+    return false;
+  }
+}
+
 // Workgroup barrier that waits for LDS traffic only: global loads issued
 // before it (the next tile's prefetch) stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -307,9 +335,8 @@ __device__ __forceinline__ void lds_barrier() {
 // copies row nY of Uy: ComputeDisp never updates those nodes.
 //
 // Tiles whose whole dependency box lies inside the grid run without bounds
-// checks, edge clamps or boundary conditions (EDGE = false).  A return
-// mapping is skipped when the squared invariant is below coh2lo: then
-// sqrt(s) <= coh exactly, as the reference's test would find.
+// checks, edge clamps or boundary conditions (EDGE = false).  Return mappings
+// go through rm_scale (the reference's test and quotient in FP64).
 //
 // CT is the arithmetic type of stress and velocity, VT the storage type of V.
 // With CT = double every expression is the plain kernels' expression, in the
@@ -524,10 +551,8 @@ struct Fused {
             const CT sxx = sTxx[b * S + a] + sTxx[b * S + a + 1] + sTxx[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1];
             const CT syy = sTyy[b * S + a] + sTyy[b * S + a + 1] + sTyy[(b + 1) * S + a] + sTyy[(b + 1) * S + a + 1];
             const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
-            if (s2 > c2lo) {
-              const CT J2XY = sqrt(s2);
-              if (J2XY > coh) txy *= coh / J2XY;
-            }
+            CT sc;
+            if (rm_scale(s2, coh, c2lo, sc)) txy *= sc;
           }
           cTxy[b * S + a] = txy;
         }
@@ -548,13 +573,10 @@ struct Fused {
                                       sTxy[(vb + 1) * S + va] + sTxy[(vb + 1) * S + va + 1]);
             const CT txx = sTxx[b * S + a], tyy = sTyy[b * S + a];
             const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
-            if (s2 > c2lo) {
-              const CT J2 = sqrt(s2);
-              if (J2 > coh) {
-                const CT sc = coh / J2;
-                sTxx[b * S + a] = txx * sc;
-                sTyy[b * S + a] = tyy * sc;
-              }
+            CT sc;
+            if (rm_scale(s2, coh, c2lo, sc)) {
+              sTxx[b * S + a] = txx * sc;
+              sTyy[b * S + a] = tyy * sc;
             }
           }
         }
@@ -1052,21 +1074,16 @@ struct March {
   // cells around it, summed in the reference's order.
   static __device__ __forceinline__ CT vertex_rm(CT txy, CT sxx, CT syy, CT coh, CT c2lo) {
     const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
-    if (s2 > c2lo) {
-      const CT J2XY = sqrt(s2);
-      if (J2XY > coh) txy *= coh / J2XY;
-    }
+    CT sc;
+    if (rm_scale(s2, coh, c2lo, sc)) txy *= sc;
     return txy;
   }
   static __device__ __forceinline__ void cell_rm(CT& txx, CT& tyy, CT av, CT coh, CT c2lo) {
     const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
-    if (s2 > c2lo) {
-      const CT J2 = sqrt(s2);
-      if (J2 > coh) {
-        const CT sc = coh / J2;
-        txx = txx * sc;
-        tyy = tyy * sc;
-      }
+    CT sc;
+    if (rm_scale(s2, coh, c2lo, sc)) {
+      txx = txx * sc;
+      tyy = tyy * sc;
     }
   }
 

@@ -188,9 +188,10 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   r_.dTrho = dT_ / rho_;               // :19
   // coh2lo <= coh^2 (real): s <= coh2lo implies RN(sqrt(s)) <= coh, i.e. no yield
   r_.coh2lo = (coh_ * coh_) * (1.0 - 0x1p-50);
+  // split scheme: yield when the FP32 squared invariant exceeds coh^2 in FP32
   {
-    const double cf = (double)(float)coh_;
-    r_.coh2lof = (float)((cf * cf) * (1.0 - 0x1p-20));
+    const float cf = (float)coh_;
+    r_.coh2lof = cf * cf;
   }
   // vertex shear modulus 0.25*(G[i,j] + G[i+1,j] + G[i,j+1] + G[i+1,j+1]) (:72)
   // for every in-pore pattern of the four cells, summed in the reference order
