@@ -423,7 +423,7 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0):
+def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
     preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
     over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
@@ -436,7 +436,7 @@ def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0):
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
-        shape = ({"march2Rows": march2} if march2 else
+        shape = ({"pairRows": pair} if pair else {"march2Rows": march2} if march2 else
                  {"marchRows": march} if march else {"tileRows": tile or 8})
         run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False, **shape)
     else:
@@ -572,7 +572,7 @@ def run_config3(check, nIter=100_000):
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
         with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
-                        mode=mode, shearLoadType=LOAD["shearLoadType"],
+                        mode=mode, shearLoadType=LOAD["shearLoadType"], marchRows=24,
                         itersPerLaunch=2) as g:
             g.set_timing(64)
             res = g.ComputeEffModuli(LOAD["initLoad"], LOAD["loadType"])
@@ -630,6 +630,8 @@ def main():
     ap.add_argument("--probe-march", type=int, default=0, help="probe: marching segment rows")
     ap.add_argument("--probe-march2", type=int, default=0,
                     help="probe: two-iteration marching segment rows (batched only)")
+    ap.add_argument("--probe-pair", type=int, default=0,
+                    help="probe: two-iteration marching on wave pairs, segment rows (batched only)")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
@@ -656,7 +658,7 @@ def main():
         return 2
     if a.probe:
         probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
-              a.probe_march, a.probe_march2)
+              a.probe_march, a.probe_march2, a.probe_pair)
         return 0
     if a.sweep:
         shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]

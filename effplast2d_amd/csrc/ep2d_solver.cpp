@@ -415,15 +415,15 @@ std::pair<hipEvent_t, hipEvent_t> Solver::takeTimingEvents() {
   return ev;
 }
 
-// Two iterations in one launch; V is read from buffer vIdx_ ^ (q & 1) and
-// written to the other, so after an odd number of double launches the other
-// buffer is current (flipV).
+// Two iterations in one launch (each segment on a pair of waves, one iteration
+// each); V is read from buffer vIdx_ ^ (q & 1) and written to the other, so
+// after an odd number of double launches the other buffer is current (flipV).
 void Solver::launchDouble(int q) {
   const bool timed = sampleEvery_ > 0 && (launchCount_ % sampleEvery_) == 0;
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) ev = takeTimingEvents();
   launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
-                timed ? ev.first : nullptr, timed ? ev.second : nullptr);
+                timed ? ev.first : nullptr, timed ? ev.second : nullptr, true);
   cur_ ^= 1;
   stressStale_ = true;
   if (timed) {
@@ -488,7 +488,7 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
     if (marchRows_ > 0 && twoIter_)
-      nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2<float, float>" : "k_march2<double, double>") +
+      nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>" : "k_march2p<double, double>") +
            " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs2_) +
            " (per iteration)";
     else if (marchRows_ > 0)

@@ -125,9 +125,10 @@ def test_march_matches_oracle(rows, nX, nPores):
 @pytest.mark.parametrize("rows,iters", [(1, 7), (3, 12), (16, 1), (16, 2), (64, 9)])
 @pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (125, 3), (8, 1)])
 def test_march2_matches_oracle(rows, iters, nX, nPores):
-    """Two iterations per launch (temporal blocking), FP64: odd and even
-    iteration counts, every segment length, bit for bit against the oracle;
-    the stress fields come from U^{n-1}, which the last (single) launch read."""
+    """Two iterations per launch (temporal blocking, each segment on a pair of
+    waves), FP64: odd and even iteration counts, every segment length, bit for
+    bit against the oracle; the stress fields come from U^{n-1}, which the
+    last (single) launch read."""
     nl = 3
     _note(f"march2 fp64 rows={rows} iters={iters} nX={nX}")
     pa = make_pa(nX, nPores=nPores, porosity=0.03)
