@@ -27,10 +27,9 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 // tile kernel at three tile heights and the two-iteration marching kernel at
 // two segment lengths.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8},    {kTiles, 14},   {kTiles, 30},   {kMarch, 16},   {kMarch, 32},
-    {kPersist, 6},  {kPersist, 8},  {kPersist, 14}, {kMarch2, 24},  {kMarch2, 32},
-    {kMarch2, 40},  {kMarch2, 48},  {kMarch2, 64},  {kPair, 24},    {kPair, 32},
-    {kPair, 48},    {kPair, 64}};
+    {kTiles, 8},  {kTiles, 14}, {kTiles, 30}, {kMarch, 32}, {kPersist, 14},
+    {kMarch2, 40}, {kPair, 20},  {kPair, 24},  {kPair, 26},  {kPair, 28},
+    {kPair, 32},  {kPair, 40},  {kPair, 52}};
 
 namespace {
 
@@ -335,33 +334,7 @@ Batch::Seq Batch::launchChunk(long c) {
   ids.push_back(tuning ? 1 : 0);
   ids.push_back(shape_.kind);
   ids.push_back(shape_.rows);
-  if (ids != key_) {
-    dropGraphs();
-    const Shape aux{kMarch, shape_.rows};
-    size_t need = (size_t)buildWork(shape_, nullptr);
-    if (twoIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
-    if (tuning)
-      for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
-    ensureCapacity(active_.size(), need);
-    nTiles_ = buildWork(shape_, hTiles_);
-    int off = nTiles_;
-    auxOff_ = off;
-    auxCnt_ = 0;
-    if (twoIter(shape_.kind)) {
-      auxCnt_ = buildWork(aux, hTiles_ + off);
-      off += auxCnt_;
-    }
-    if (tuning)
-      for (int k = 0; k < kNCand; k++) {
-        candOff_[k] = off;
-        candCnt_[k] = buildWork(kCand[k], hTiles_ + off);
-        off += candCnt_[k];
-      }
-    EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, (size_t)off * sizeof(int4), hipMemcpyHostToDevice,
-                            stream_));
-    key_ = ids;
-    curs_.clear();
-  }
+  prepareWork(ids, tuning);
   // RVE table: re-uploaded when an RVE's current buffers changed
   std::vector<int> curs(active_.size());
   for (size_t k = 0; k < active_.size(); k++)
@@ -401,7 +374,57 @@ Batch::Seq Batch::launchChunk(long c) {
     tunedCells_ = cells;
     tuneDue_ = false;
     st_.tunings++;
+    // pick the shape now, so the rest of this chunk already runs it (the
+    // tuning launches left d even, as the one-iteration kernels need)
+    EP2D_HIP(hipStreamSynchronize(stream_));
+    resolveEvents();
+    ids.resize(ids.size() - 3);
+    ids.push_back(0);
+    ids.push_back(shape_.kind);
+    ids.push_back(shape_.rows);
+    prepareWork(ids, false);
   }
+  return runRest(c, it, l, d, bytes, cells, tuning);
+}
+
+// Work lists of the admitted RVEs for the current shape (and its single-
+// iteration list for a two-iteration shape, and every candidate's list when
+// tuning), uploaded when the RVE set, the tuning state or the shape changed.
+void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
+  if (ids != key_) {
+    dropGraphs();
+    const Shape aux{kMarch, shape_.rows};
+    size_t need = (size_t)buildWork(shape_, nullptr);
+    if (twoIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
+    if (tuning)
+      for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
+    ensureCapacity(active_.size(), need);
+    nTiles_ = buildWork(shape_, hTiles_);
+    int off = nTiles_;
+    auxOff_ = off;
+    auxCnt_ = 0;
+    if (twoIter(shape_.kind)) {
+      auxCnt_ = buildWork(aux, hTiles_ + off);
+      off += auxCnt_;
+    }
+    if (tuning)
+      for (int k = 0; k < kNCand; k++) {
+        candOff_[k] = off;
+        candCnt_[k] = buildWork(kCand[k], hTiles_ + off);
+        off += candCnt_[k];
+      }
+    EP2D_HIP(hipMemcpyAsync(dTiles_, hTiles_, (size_t)off * sizeof(int4), hipMemcpyHostToDevice,
+                            stream_));
+    key_ = ids;
+    curs_.clear();
+  }
+}
+
+// The chunk's iterations after the `it` tuning ones, from launch index l and
+// double index d: the main launches (a graph replay when allowed), then for a
+// two-iteration shape its one or two single iterations.
+Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double cells,
+                          bool tuning) {
   const long rest = c - it;
   const bool timed = opt_.timingEvery > 0;
   const bool two = twoIter(shape_.kind);

@@ -52,7 +52,7 @@ class Batch {
                         // marching on one wave or on a pair of waves
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 17;
+  static constexpr int kNCand = 13;
   static const Shape kCand[kNCand];
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
@@ -69,6 +69,8 @@ class Batch {
     long launches, doubles;
   };
   Seq launchChunk(long c);
+  void prepareWork(const std::vector<long long>& ids, bool tuning);
+  Seq runRest(long c, long it, long l, long d, double bytes, double cells, bool tuning);
   int buildWork(Shape sh, int4* out) const;  // work list of all admitted RVEs
   void launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
