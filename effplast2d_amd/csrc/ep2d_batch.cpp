@@ -328,7 +328,7 @@ Batch::Seq Batch::launchChunk(long c) {
   // every candidate, as many as the chunk holds (at least one timed round)
   constexpr long kTuneEach = 4;
   long roundIters = 0;
-  for (int k = 0; k < kNCand; k++) roundIters += twoIter(kCand[k].kind) ? 4 : 1;
+  for (int k = 0; k < kNCand; k++) roundIters += twoIter(kCand[k].kind) ? 4 : 2;
   const long rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
   const bool tuning = !fixed_ && tuneDue_ && rounds >= 2;
   ids.push_back(tuning ? 1 : 0);
@@ -350,24 +350,31 @@ Batch::Seq Batch::launchChunk(long c) {
   }
   long l = 0, d = 0, it = 0;
   if (tuning) {
-    // a warm-up round, then the timed rounds, candidates interleaved
+    // a warm-up round, then the timed rounds, candidates interleaved; a
+    // candidate's two launches of a round run back to back between one pair
+    // of events (two-iteration candidates in pairs, so V is back in its
+    // buffer before the next candidate)
     for (long round = 0; round < rounds; round++)
       for (int k = 0; k < kNCand; k++) {
         const Shape sh = kCand[k];
         const int4* w = dTiles_ + candOff_[k];
         const bool dbl = twoIter(sh.kind);
-        for (int rep = 0; rep < (dbl ? 2 : 1); rep++) {
-          Pending ev{};
-          if (round > 0) {
-            ev = takeEvents();
-            ev.cand = k;
-            ev.launches = dbl ? 2 : 1;
-          }
-          launchOne(sh, w, candCnt_[k], dbl, l, d, ev.a, ev.b);
+        Pending ev{};
+        if (round > 0) {
+          ev = takeEvents();
+          ev.cand = k;
+          ev.launches = dbl ? 4 : 2;   // iterations
+          EP2D_HIP(hipEventRecord(ev.a, stream_));
+        }
+        for (int rep = 0; rep < 2; rep++) {
+          launchOne(sh, w, candCnt_[k], dbl, l, d);
           l++;
           if (dbl) d++;
-          if (round > 0) evPending_.push_back(ev);
           it += dbl ? 2 : 1;
+        }
+        if (round > 0) {
+          EP2D_HIP(hipEventRecord(ev.b, stream_));
+          evPending_.push_back(ev);
         }
       }
     for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;

@@ -655,11 +655,15 @@ __global__ __launch_bounds__(256) void k_maxabs(const VT* __restrict__ Vx, size_
   double mx = 0.0, my = 0.0;
   unsigned long long bad = 0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  // a few blocks loop over many elements (one atomic per block); the
+  // unrolled loops keep several loads in flight per thread
+#pragma unroll 4
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nx; k += stride) {
     const double v = fabs((double)Vx[k]);
     if (v > mx) mx = v;
     if (!(v <= 1.7976931348623157e308)) bad++;
   }
+#pragma unroll 4
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < ny; k += stride) {
     const double v = fabs((double)Vy[k]);
     if (v > my) my = v;
@@ -2009,7 +2013,9 @@ const char* fused_kernel_name(int mode) {
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s) {
   const size_t nx = (size_t)(r.nX + 1) * r.nY, ny = (size_t)r.nX * (r.nY + 1);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 3);
-  const int blocks = (int)std::min<size_t>((std::max(nx, ny) + 255) / 256, 2048);
+  // about 16 elements of each array per thread, at most 1024 blocks: the
+  // blocks' atomics on the three result words serialise, so fewer is faster
+  const int blocks = (int)std::max<size_t>(1, std::min<size_t>((std::max(nx, ny) + 4095) / 4096, 1024));
   if (f32)
     hipLaunchKernelGGL((k_maxabs<float>), dim3(blocks), dim3(256), 0, s,
                        static_cast<const float*>(r.Vx), nx, static_cast<const float*>(r.Vy), ny, out);
