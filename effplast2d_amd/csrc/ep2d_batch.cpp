@@ -270,6 +270,12 @@ void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long 
     launch_fused_batch(dRves_, work, n, opt_.mode, sh.rows, (int)(l & 1), stream_, e0, e1);
 }
 
+int Batch::candIndex(Shape sh) {
+  for (int k = 0; k < kNCand; k++)
+    if (kCand[k].kind == sh.kind && kCand[k].rows == sh.rows) return k;
+  return 0;
+}
+
 // Split of `iters` iterations for the two-iteration shape: pairs first, then
 // one or two single iterations, so the chunk's last launch reads U^{n-1}
 // (the stress fields are recomputed from that buffer after a load step).
@@ -385,6 +391,41 @@ Batch::Seq Batch::launchChunk(long c) {
     // tuning launches left d even, as the one-iteration kernels need)
     EP2D_HIP(hipStreamSynchronize(stream_));
     resolveEvents();
+    // refine among the candidates within 3% of the best: longer back-to-back
+    // runs, so differences of a percent decide reliably
+    {
+      int near[kNCand], nn = 0;
+      const double best = st_.candUs[candIndex(shape_)];
+      for (int k = 0; k < kNCand && nn < 3; k++)
+        if (st_.candUs[k] > 0 && st_.candUs[k] <= 1.03 * best) near[nn++] = k;
+      constexpr int kRefRounds = 4, kRefLaunches = 4;
+      long need = 0;
+      for (int q = 0; q < nn; q++)
+        need += kRefRounds * kRefLaunches * (twoIter(kCand[near[q]].kind) ? 2 : 1);
+      if (nn > 1 && c - it >= need + 2) {
+        for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
+        for (int round = 0; round < kRefRounds; round++)
+          for (int q = 0; q < nn; q++) {
+            const int k = near[q];
+            const Shape sh = kCand[k];
+            const bool dbl = twoIter(sh.kind);
+            Pending ev = takeEvents();
+            ev.cand = k;
+            ev.launches = kRefLaunches * (dbl ? 2 : 1);
+            EP2D_HIP(hipEventRecord(ev.a, stream_));
+            for (int rep = 0; rep < kRefLaunches; rep++) {
+              launchOne(sh, dTiles_ + candOff_[k], candCnt_[k], dbl, l, d);
+              l++;
+              if (dbl) d++;
+              it += dbl ? 2 : 1;
+            }
+            EP2D_HIP(hipEventRecord(ev.b, stream_));
+            evPending_.push_back(ev);
+          }
+        EP2D_HIP(hipStreamSynchronize(stream_));
+        resolveEvents();
+      }
+    }
     ids.resize(ids.size() - 3);
     ids.push_back(0);
     ids.push_back(shape_.kind);
@@ -530,10 +571,15 @@ void Batch::resolveEvents() {
   }
   evPending_.clear();
   if (tuned) {
+    // the fastest of the candidates timed in this pass (a refinement pass
+    // times a few; the others keep their last figures)
     int best = -1;
+    double bestUs = 0.0;
     for (int k = 0; k < kNCand; k++) {
-      st_.candUs[k] = tuneN_[k] ? 1e3 * tuneMs_[k] / tuneN_[k] : 0.0;
-      if (tuneN_[k] && (best < 0 || st_.candUs[k] < st_.candUs[best])) best = k;
+      if (!tuneN_[k]) continue;
+      const double us = 1e3 * tuneMs_[k] / tuneN_[k];
+      st_.candUs[k] = us;
+      if (best < 0 || us < bestUs) best = k, bestUs = us;
     }
     if (best >= 0) shape_ = kCand[best];
   }

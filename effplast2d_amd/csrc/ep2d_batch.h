@@ -70,6 +70,7 @@ class Batch {
   };
   Seq launchChunk(long c);
   void prepareWork(const std::vector<long long>& ids, bool tuning);
+  static int candIndex(Shape sh);
   Seq runRest(long c, long it, long l, long d, double bytes, double cells, bool tuning);
   int buildWork(Shape sh, int4* out) const;  // work list of all admitted RVEs
   void launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d,

@@ -313,3 +313,49 @@ def test_split_moduli_close_to_fp64_oracle(case):
     assert res_g["num"]["Kphi"] == pytest.approx(b, rel=1e-3)
     for k in ["P", "tauXX", "Ux", "Uy"]:
         assert l2_rel(fields[k], s[k]) <= 1e-3, k
+
+
+@pytest.mark.parametrize("kw", [{"marchRows": 26, "itersPerLaunch": 2}, {"tileRows": 8}],
+                         ids=["pairs26", "tiles8"])
+def test_full_size_fp64_matches_oracle(kw):
+    """The headline's RVE at full size (1024^2, one pore, porosity 0.01,
+    Y = 2.5e-3, G0 = 0.5, three tasks): from a noisy affine compression, 101
+    plastic iterations (an odd count: two-iteration launches then one single),
+    FP64, every field against the oracle (OpenMP) at <= 1e-12 relative, bit
+    for bit in practice."""
+    import os
+    nX, iters = 1024, 101
+    _note(f"full size fp64 {kw} iters={iters}")
+    pa = make_pa(nX, nPores=1, porosity=0.01)
+    init = random_state(nX, 4242, pa=pa, boundary_v=True)
+    s = oracle_state(nX, pa, 3, init, nThreads=min(16, os.cpu_count() or 1))
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=3, mode="fp64", **kw) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS:
+            got = g.get(k)
+            e = rel_err(got, s[k])
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
+    # the run must have exercised the return mapping
+    assert np.count_nonzero(s["J2"] > pa[ora.PA_COH] * (1 - 1e-12)) > 0
+
+
+def test_full_size_split_pairs_equal_tiles():
+    """Split scheme at 1024^2: the two-iteration kernel on wave pairs gives the
+    tile kernel's fields bit for bit (201 iterations)."""
+    nX, iters = 1024, 201
+    _note(f"full size split pairs vs tiles iters={iters}")
+    pa = make_pa(nX, nPores=1, porosity=0.01)
+    init = random_state(nX, 4243, pa=pa, boundary_v=True)
+    out = []
+    for kw in ({"marchRows": 26, "itersPerLaunch": 2}, {"tileRows": 8}):
+        with EffPlast2D(pa, nX, nTasks=3, mode="split", **kw) as g:
+            for k in STATE_FIELDS:
+                g.set(k, init[k])
+            g.iterate(iters)
+            out.append({k: g.get(k) for k in STATE_FIELDS + STRESS_FIELDS})
+    for k in STATE_FIELDS + STRESS_FIELDS:
+        assert np.array_equal(out[0][k], out[1][k]), \
+            f"{k}: {np.count_nonzero(out[0][k] != out[1][k])} differ"
