@@ -117,6 +117,8 @@ void Batch::admit() {
     Job& jb = queue_.front();
     const size_t need = estimateBytes(jb.rq);
     if (used_ + need > budget_ && !active_.empty()) return;  // wait for retirements
+    // blocks held for reuse count against the budget too
+    if (used_ + need + pool_.heldBytes() > budget_) pool_.trim();
     Slot sl;
     sl.id = jb.id;
     sl.tag = jb.rq.tag;
@@ -135,7 +137,7 @@ void Batch::admit() {
     std::memcpy(o.shearLoadType, jb.rq.shearLoadType, sizeof(o.shearLoadType));
     o.tileRows = 0;
     try {
-      sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_);
+      sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_, &pool_);
       sl.bytes = sl.s->deviceBytes();
       sl.s->jobBegin(jb.rq.nTasks > 3 ? 1 : 0, jb.rq.initLoadValue, jb.rq.loadType,
                      jb.rq.loadValue);

@@ -84,6 +84,7 @@ class Batch {
 
   ep2d_batch_options opt_;
   int device_ = 0;
+  MemPool pool_;   // buffers of retired RVEs, reused by the next admissions
   hipStream_t stream_ = nullptr;
   std::chrono::steady_clock::time_point t0_;
   std::deque<Job> queue_;

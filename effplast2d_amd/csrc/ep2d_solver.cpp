@@ -54,16 +54,58 @@ void derive_params(const ep2d_physics& ph, double pa[EP2D_NPARS], int* nXout) {
 }
 
 // ---------------------------------------------------------------------------
+void* MemPool::take(size_t bytes, bool host) {
+  auto& m = host ? host_ : dev_;
+  const auto it = m.find(bytes);
+  if (it == m.end()) return nullptr;
+  void* p = it->second;
+  m.erase(it);
+  held_ -= bytes;
+  return p;
+}
+
+void MemPool::give(void* p, size_t bytes, bool host) {
+  (host ? host_ : dev_).emplace(bytes, p);
+  held_ += bytes;
+}
+
+void MemPool::trim() {
+  for (auto& kv : dev_) (void)hipFree(kv.second);
+  for (auto& kv : host_) (void)hipHostFree(kv.second);
+  dev_.clear();
+  host_.clear();
+  held_ = 0;
+}
+
+// Device buffer of `bytes` (at least 256): from the pool when it holds one of
+// that size, else hipMalloc (after trimming the pool if memory runs short).
+// Buffers are not cleared: every user initialises what it reads.
 void* Solver::alloc(size_t bytes) {
-  void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 256));
-  if (e != hipSuccess) {
-    (void)hipGetLastError();
-    throw StatusError(EP2D_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(bytes) +
-                                           " bytes failed: " + hipGetErrorString(e));
+  bytes = std::max<size_t>(bytes, 256);
+  void* p = pool_ ? pool_->take(bytes, false) : nullptr;
+  if (!p) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess && pool_ && pool_->heldBytes() > 0) {
+      (void)hipGetLastError();
+      pool_->trim();
+      e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      throw StatusError(EP2D_ERR_MEMORY, std::string("hipMalloc of ") + std::to_string(bytes) +
+                                             " bytes failed: " + hipGetErrorString(e));
+    }
   }
-  allocs_.push_back(p);
+  allocs_.push_back({p, bytes});
   devBytes_ += bytes;
+  return p;
+}
+
+// Pinned host buffer, pooled like the device buffers.
+void* Solver::hostAlloc(size_t bytes) {
+  void* p = pool_ ? pool_->take(bytes, true) : nullptr;
+  if (!p) EP2D_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+  hostAllocs_.push_back({p, bytes});
   return p;
 }
 
@@ -81,8 +123,8 @@ void Solver::d2h(T* dst, const T* src, size_t n) {
 
 // EffPlast2D::EffPlast2D, EffPlast2D.cu:1541-1612.
 Solver::Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt,
-               hipStream_t stream)
-    : nX_(nX), nY_(nY), opt_(opt) {
+               hipStream_t stream, MemPool* pool)
+    : nX_(nX), nY_(nY), opt_(opt), pool_(pool) {
   try {
     init(pa, stream);
   } catch (...) {
@@ -165,8 +207,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   dMax_ = (unsigned long long*)alloc(4 * sizeof(unsigned long long));
   dAvgPartial_ = (double*)alloc((size_t)avg_partial_blocks(nX, nY) * NAVG * 8);
   dAvg_ = (double*)alloc(NAVG * 8);
-  EP2D_HIP(hipHostMalloc((void**)&hMax_, 4 * sizeof(unsigned long long), hipHostMallocDefault));
-  EP2D_HIP(hipHostMalloc((void**)&hAvg_, NAVG * sizeof(double), hipHostMallocDefault));
+  hMax_ = (unsigned long long*)hostAlloc(4 * sizeof(unsigned long long));
+  hAvg_ = (double*)hostAlloc(NAVG * sizeof(double));
 
   r_.nX = nX;
   r_.nY = nY;
@@ -279,11 +321,17 @@ void Solver::release() {
   pending_.clear();
   pendingIters_.clear();
   freeEvents_.clear();
-  for (void* p : allocs_) (void)hipFree(p);
+  // the stream is idle: the blocks can go back to the pool
+  for (auto& a : allocs_) {
+    if (pool_) pool_->give(a.first, a.second, false);
+    else (void)hipFree(a.first);
+  }
   allocs_.clear();
-  if (hMax_) (void)hipHostFree(hMax_);
-  if (hAvg_) (void)hipHostFree(hAvg_);
-  if (hGather_) (void)hipHostFree(hGather_);
+  for (auto& a : hostAllocs_) {
+    if (pool_) pool_->give(a.first, a.second, true);
+    else (void)hipHostFree(a.first);
+  }
+  hostAllocs_.clear();
   hMax_ = nullptr;
   hAvg_ = nullptr;
   hGather_ = nullptr;
@@ -351,8 +399,7 @@ void Solver::buildGatherTable() {
   dTable_ = (int2*)alloc(tab.size() * sizeof(int2));
   dGather_ = (double*)alloc(tab.size() * sizeof(double));
   dBases_ = (const double**)alloc(8 * sizeof(double*));
-  EP2D_HIP(hipHostMalloc((void**)&hGather_, std::max<size_t>(tab.size(), 1) * sizeof(double),
-                         hipHostMallocDefault));
+  hGather_ = (double*)hostAlloc(std::max<size_t>(tab.size(), 1) * sizeof(double));
   const double* bases[8] = {f_.P, f_.tauXX, f_.tauYY, r_.Ux[0], r_.Uy[0], r_.Ux[1], r_.Uy[1], nullptr};
   h2d(dTable_, tab.data(), tab.size());
   h2d(dBases_, bases, 8);

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <fstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/ep2d.h"
@@ -22,11 +23,33 @@ struct StatusError : std::runtime_error {
 // Derivation of pa[] from the physical inputs (get_sigma_2D.m:3-32).
 void derive_params(const ep2d_physics& ph, double pa[EP2D_NPARS], int* nX);
 
+// Device and pinned host blocks kept for reuse, by exact size.  A batch
+// engine's solvers take their buffers from it and give them back when they
+// retire, so admitting the next RVE of a size seen before allocates nothing
+// and retiring one frees nothing (hipMalloc / hipFree cost tens to hundreds of
+// microseconds each and hipFree waits for the device).  The owner trims it.
+class MemPool {
+ public:
+  MemPool() = default;
+  MemPool(const MemPool&) = delete;
+  MemPool& operator=(const MemPool&) = delete;
+  ~MemPool() { trim(); }
+  void* take(size_t bytes, bool host);           // nullptr when none is held
+  void give(void* p, size_t bytes, bool host);
+  void trim();                                    // free every held block
+  size_t heldBytes() const { return held_; }
+
+ private:
+  std::unordered_multimap<size_t, void*> dev_, host_;
+  size_t held_ = 0;
+};
+
 class Solver {
  public:
-  // stream: run on this stream (owned by the caller) instead of a new one
+  // stream: run on this stream (owned by the caller) instead of a new one;
+  // pool: take buffers from it and give them back on destruction
   Solver(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options& opt,
-         hipStream_t stream = nullptr);
+         hipStream_t stream = nullptr, MemPool* pool = nullptr);
   ~Solver();
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
@@ -96,6 +119,7 @@ class Solver {
   void launchIteration();
   void resolveTiming(bool all);
   void* alloc(size_t bytes);
+  void* hostAlloc(size_t bytes);
   void buildGatherTable();
   template <typename T> void h2d(T* dst, const T* src, size_t n);
   template <typename T> void d2h(T* dst, const T* src, size_t n);
@@ -148,7 +172,9 @@ class Solver {
   double* dGather_ = nullptr;
   double* hGather_ = nullptr;
   int nGather_ = 0;
-  std::vector<void*> allocs_;
+  std::vector<std::pair<void*, size_t>> allocs_;   // device blocks and their sizes
+  std::vector<std::pair<void*, size_t>> hostAllocs_;  // pinned host blocks
+  MemPool* pool_ = nullptr;
   size_t devBytes_ = 0;
 
   // gather table layout (host side)
