@@ -136,6 +136,7 @@ void Batch::admit() {
     o.useShearLoadType = jb.rq.useShearLoadType;
     std::memcpy(o.shearLoadType, jb.rq.shearLoadType, sizeof(o.shearLoadType));
     o.tileRows = 0;
+    o.itersPerLaunch = 1;   // the engine launches its own kernels
     try {
       sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_, &pool_);
       sl.bytes = sl.s->deviceBytes();

@@ -110,7 +110,7 @@ void ep2d_default_options(ep2d_options* o) {
   o->mode = EP2D_MODE_FP64;
   o->device = 0;            // DEVICE_IDX, EffPlast2D.h:37-39
   o->nTimeSteps = 1;        // EffPlast2D(const unsigned int nTimeSteps = 1)
-  o->itersPerLaunch = 1;
+  o->itersPerLaunch = 0;
   o->verbose = 0;
   o->logPath = nullptr;
 }

@@ -262,6 +262,16 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   for (double* p : {f_.P, f_.tauXX, f_.tauYY, f_.tauXYav, f_.J2}) launch_fill(p, nC, 0.0, stream_);
   for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
   buildGatherTable();
+  if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
+    // automatic shape for one RVE: two iterations per launch on wave pairs
+    // where that measured faster than the tile kernel (split from 1024^2,
+    // FP64 from 2048^2), with segments short enough to give the GPU waves
+    const int n = std::max(nX, nY);
+    if ((mode_ == EP2D_MODE_SPLIT && n >= 1024) || (mode_ == EP2D_MODE_FP64 && n >= 2048)) {
+      opt_.marchRows = n <= 1024 ? 8 : (n <= 2048 ? 16 : 26);
+      opt_.itersPerLaunch = 2;
+    }
+  }
   if (fused) {
     int ty = opt_.tileRows;
     if (ty != 0 && !valid_tile_rows(ty)) throw ArgError("tileRows must be 0, 6, 8, 14 or 30");

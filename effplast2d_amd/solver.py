@@ -49,7 +49,7 @@ class EffPlast2D:
 
     def __init__(self, pa, nX, nTimeSteps=1, nTasks=3, nIter=100_000, eIter=1e-10,
                  checkInterval=10_000, mode="fp64", device=0, shearLoadType=None,
-                 verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=1):
+                 verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=0):
         self.pa = np.ascontiguousarray(pa, dtype=np.float64)
         if self.pa.shape != (L.NPARS,):
             raise ValueError("pa must hold 11 values")

@@ -103,7 +103,9 @@ typedef struct ep2d_options {
                            (EffPlast2D.log, :1607) */
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
   int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
-  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch (same results) */
+  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch on wave pairs;
+                           1: one; 0 (default) with tileRows = marchRows = 0: the
+                           shape is chosen by grid size and scheme (same results) */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
