@@ -622,13 +622,14 @@ struct Fused {
   }
 };
 
-// Block -> tile map.  Workgroups are dispatched round-robin over the 8 XCDs;
-// when the tile count divides by 8, XCD k receives the contiguous band of
-// tiles [k*nb/8, (k+1)*nb/8), so a tile and the tile above it (which share
-// halo rows) mostly run on the same XCD and share its L2.
+// Block -> tile map.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b to XCD b % 8), so XCD x receives its blocks as a contiguous run of
+// the tile list: x*q + min(x, r) + b/8 with q = nb/8, r = nb%8 (the first r
+// XCDs take one more).  A tile and its neighbours, which share halo rows,
+// then mostly run on the same XCD and share its L2.
 __device__ __forceinline__ int tile_of_block(int b, int nb) {
-  if ((nb & 7) == 0) return (b & 7) * (nb >> 3) + (b >> 3);
-  return b;
+  const int x = b & 7, q = nb >> 3, r = nb & 7;
+  return x * q + min(x, r) + (b >> 3);
 }
 
 template <typename CT, typename VT, bool PLASTIC, int TY, int NW>
