@@ -28,8 +28,8 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 // two segment lengths.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
     {kTiles, 8},  {kTiles, 14}, {kTiles, 30}, {kMarch, 32}, {kPersist, 14},
-    {kMarch2, 40}, {kPair, 20},  {kPair, 24},  {kPair, 26},  {kPair, 28},
-    {kPair, 32},  {kPair, 40},  {kPair, 52}};
+    {kMarch2, 40}, {kPair, 8},   {kPair, 14},  {kPair, 20},  {kPair, 24},
+    {kPair, 26},  {kPair, 28},  {kPair, 32},  {kPair, 40},  {kPair, 52}};
 
 namespace {
 

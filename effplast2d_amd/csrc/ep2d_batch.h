@@ -52,7 +52,7 @@ class Batch {
                         // marching on one wave or on a pair of waves
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 13;
+  static constexpr int kNCand = 15;
   static const Shape kCand[kNCand];
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
