@@ -1511,6 +1511,70 @@ struct MarchStage {
   }
 };
 
+// Stage A's inputs in a two-iteration segment: the first rows (start) and,
+// for step t, Ux row t+1, Uy row t+2, mask row t+1 and V row t of the state in
+// HBM.  Interior segments read through buffer resources (the lane's byte
+// offset fixed, the row's in an SGPR) and clamp rows to the segment's last
+// step tEnd; edge segments guard every load.
+template <typename CT, typename VT, bool EDGE>
+struct StageInput {
+  using Ld = typename March<CT, VT>::Ld;
+  const double* Uxi;
+  const double* Uyi;
+  const VT* Vxi;
+  const VT* Vyi;
+  const unsigned char* mask;
+  int nX, nY, sx, i, tEnd;
+  unsigned o8, oV;
+  __amdgpu_buffer_rsrc_t bUx, bUy, bVx, bVy, bM;
+
+  __device__ __forceinline__ StageInput(const RveDev& r, int cur, int vpar, int col, int tEnd_)
+      : Uxi(r.Ux[cur]), Uyi(r.Uy[cur]),
+        Vxi(static_cast<const VT*>(vpar ? r.Vx2 : r.Vx)),
+        Vyi(static_cast<const VT*>(vpar ? r.Vy2 : r.Vy)), mask(r.mask), nX(r.nX), nY(r.nY),
+        sx(r.nX + 1), i(col), tEnd(tEnd_), o8((unsigned)col * 8u),
+        oV((unsigned)col * (unsigned)sizeof(VT)), bUx(brsrc(Uxi)), bUy(brsrc(Uyi)),
+        bVx(brsrc(Vxi)), bVy(brsrc(Vyi)), bM(brsrc(mask)) {}
+
+  // Ux row jb, Uy row jb+1, mask row jb into the stage
+  __device__ __forceinline__ void start(MarchStage<CT, VT, EDGE>& st, int jb) const {
+    double ux = 0.0, uy = 0.0;
+    unsigned m = 0;
+    const bool colC = !EDGE || (i >= 0 && i < nX);
+    if (!EDGE) {
+      ux = gld(Uxi, (unsigned)(jb * sx + i));
+      uy = gld(Uyi, (unsigned)((jb + 1) * nX + i));
+      m = gld(mask, (unsigned)(jb * nX + i));
+    } else {
+      if (i >= 0 && i <= nX && jb >= 0 && jb < nY) ux = gld(Uxi, (unsigned)(jb * sx + i));
+      if (colC && jb + 1 >= 0 && jb + 1 <= nY) uy = gld(Uyi, (unsigned)((jb + 1) * nX + i));
+      if (colC && jb >= 0 && jb < nY) m = gld(mask, (unsigned)(jb * nX + i));
+    }
+    st.start(ux, uy, m);
+  }
+
+  __device__ __forceinline__ Ld fetch(int t0) const {
+    const int t = EDGE ? t0 : min(t0, tEnd);
+    Ld d;
+    if (!EDGE) {
+      d.ux = bld<double>(bUx, o8, (unsigned)(t + 1) * (unsigned)sx * 8u);
+      d.uy = bld<double>(bUy, o8, (unsigned)(t + 2) * (unsigned)nX * 8u);
+      d.m = bld<unsigned char>(bM, (unsigned)i, (unsigned)(t + 1) * (unsigned)nX);
+      d.vx = bld<VT>(bVx, oV, (unsigned)t * (unsigned)sx * (unsigned)sizeof(VT));
+      d.vy = bld<VT>(bVy, oV, (unsigned)t * (unsigned)nX * (unsigned)sizeof(VT));
+    } else {
+      const bool ci = i >= 0 && i < nX;
+      d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
+      d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
+      d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(mask, (unsigned)((t + 1) * nX + i)) : 0u;
+      const bool vr = ci && t >= 0 && t < nY;
+      d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
+      d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
+    }
+    return d;
+  }
+};
+
 template <typename CT, typename VT, bool EDGE>
 __device__ __forceinline__ void march2_run(const RveDev& r, int cur, int vpar, int s, int j0,
                                            int j1, const CT* gt) {
@@ -1519,13 +1583,9 @@ __device__ __forceinline__ void march2_run(const RveDev& r, int cur, int vpar, i
   const int nX = r.nX, nY = r.nY, sx = nX + 1;
   const int L = threadIdx.x & 63;
   const int i = s + L;
-  const double* __restrict__ Uxi = r.Ux[cur];
-  const double* __restrict__ Uyi = r.Uy[cur];
   double* __restrict__ Uxo = r.Ux[cur ^ 1];
   double* __restrict__ Uyo = r.Uy[cur ^ 1];
   // velocities are read from one buffer and written to the other
-  const VT* Vxi = static_cast<const VT*>(vpar ? r.Vx2 : r.Vx);
-  const VT* Vyi = static_cast<const VT*>(vpar ? r.Vy2 : r.Vy);
   VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
   VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
   const bool outLane = L >= 4 && L < 60 && (!EDGE || i < nX);
@@ -1534,48 +1594,14 @@ __device__ __forceinline__ void march2_run(const RveDev& r, int cur, int vpar, i
   A.setup(r, gt, i);
   B.setup(r, gt, i);
   const int jbA = j0 - 4, jbB = j0 - 2;
-  {
-    double ux = 0.0, uy = 0.0;
-    unsigned m = 0;
-    const bool colC = !EDGE || (i >= 0 && i < nX);
-    if (!EDGE) {
-      ux = gld(Uxi, (unsigned)(jbA * sx + i));
-      uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
-      m = gld(r.mask, (unsigned)(jbA * nX + i));
-    } else {
-      if (i >= 0 && i <= nX && jbA >= 0 && jbA < nY) ux = gld(Uxi, (unsigned)(jbA * sx + i));
-      if (colC && jbA + 1 >= 0 && jbA + 1 <= nY) uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
-      if (colC && jbA >= 0 && jbA < nY) m = gld(r.mask, (unsigned)(jbA * nX + i));
-    }
-    A.start(ux, uy, m);
-  }
+  const int tEnd = j1 + 1;  // A's last step: B's last step j1-1 needs A's row j1+1
+  const StageInput<CT, VT, EDGE> in(r, cur, vpar, i, tEnd);
+  in.start(A, jbA);
   // A's outputs of rows t-1 (Ux) and t-1, t-2 (V); A's input mask rows t, t-1
   double h1ux = 0.0;
   CT h1vx = 0, h1vy = 0, h2vx = 0, h2vy = 0;
   unsigned mh1 = 0, mh2 = 0;
-  const int tEnd = j1 + 1;  // A's last step: B's last step j1-1 needs A's row j1+1
-  // step t reads Ux row t+1, Uy row t+2, mask row t+1, V row t (unconditional:
-  // interior segments clamp to rows in range, edge loads are guarded)
-  auto fetch = [&](int t0) {
-    const int t = EDGE ? t0 : min(t0, tEnd);
-    Ld d;
-    if (!EDGE) {
-      d.ux = gld(Uxi + (size_t)(t + 1) * sx, (unsigned)i);
-      d.uy = gld(Uyi + (size_t)(t + 2) * nX, (unsigned)i);
-      d.m = gld(r.mask + (size_t)(t + 1) * nX, (unsigned)i);
-      d.vx = gld(Vxi + (size_t)t * sx, (unsigned)i);
-      d.vy = gld(Vyi + (size_t)t * nX, (unsigned)i);
-    } else {
-      const bool ci = i >= 0 && i < nX;
-      d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
-      d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
-      d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(r.mask, (unsigned)((t + 1) * nX + i)) : 0u;
-      const bool vr = ci && t >= 0 && t < nY;
-      d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
-      d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
-    }
-    return d;
-  };
+  auto fetch = [&](int t) { return in.fetch(t); };
   auto body = [&](const int t, const Ld& ld) {
     const auto oA = A.step(t, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
     if (t == jbB + 1) B.start(h1ux, oA.uy, mh2);  // Ux' row jbB, Uy' row jbB+1, mask row jbB
@@ -1670,50 +1696,9 @@ __device__ __forceinline__ void march2p_run(const RveDev& r, int cur, int vpar, 
   MarchStage<CT, VT, EDGE> st;
   st.setup(r, gt, i);
   if (!stageB) {
-    const double* __restrict__ Uxi = r.Ux[cur];
-    const double* __restrict__ Uyi = r.Uy[cur];
-    const VT* Vxi = static_cast<const VT*>(vpar ? r.Vx2 : r.Vx);
-    const VT* Vyi = static_cast<const VT*>(vpar ? r.Vy2 : r.Vy);
-    {
-      double ux = 0.0, uy = 0.0;
-      unsigned m = 0;
-      const bool colC = !EDGE || (i >= 0 && i < nX);
-      if (!EDGE) {
-        ux = gld(Uxi, (unsigned)(jbA * sx + i));
-        uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
-        m = gld(r.mask, (unsigned)(jbA * nX + i));
-      } else {
-        if (i >= 0 && i <= nX && jbA >= 0 && jbA < nY) ux = gld(Uxi, (unsigned)(jbA * sx + i));
-        if (colC && jbA + 1 >= 0 && jbA + 1 <= nY) uy = gld(Uyi, (unsigned)((jbA + 1) * nX + i));
-        if (colC && jbA >= 0 && jbA < nY) m = gld(r.mask, (unsigned)(jbA * nX + i));
-      }
-      st.start(ux, uy, m);
-    }
-    // step t reads Ux row t+1, Uy row t+2, mask row t+1, V row t (interior
-    // segments clamp to rows in range, edge loads are guarded)
-    const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
-    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxi), bUy = brsrc(Uyi), bVx = brsrc(Vxi),
-                                 bVy = brsrc(Vyi), bM = brsrc(r.mask);
-    auto fetch = [&](int t0) {
-      const int t = EDGE ? t0 : min(t0, tEnd);
-      Ld d;
-      if (!EDGE) {
-        d.ux = bld<double>(bUx, o8, (unsigned)(t + 1) * (unsigned)sx * 8u);
-        d.uy = bld<double>(bUy, o8, (unsigned)(t + 2) * (unsigned)nX * 8u);
-        d.m = bld<unsigned char>(bM, (unsigned)i, (unsigned)(t + 1) * (unsigned)nX);
-        d.vx = bld<VT>(bVx, oV, (unsigned)t * (unsigned)sx * (unsigned)sizeof(VT));
-        d.vy = bld<VT>(bVy, oV, (unsigned)t * (unsigned)nX * (unsigned)sizeof(VT));
-      } else {
-        const bool ci = i >= 0 && i < nX;
-        d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
-        d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
-        d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(r.mask, (unsigned)((t + 1) * nX + i)) : 0u;
-        const bool vr = ci && t >= 0 && t < nY;
-        d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
-        d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
-      }
-      return d;
-    };
+    const StageInput<CT, VT, EDGE> in(r, cur, vpar, i, tEnd);
+    in.start(st, jbA);
+    auto fetch = [&](int t) { return in.fetch(t); };
     auto stepA = [&](int q, const Ld& ld, int sl) {
       const int t = jbA + q;
       if (t <= tEnd) {
