@@ -253,6 +253,32 @@ def c4_requests(n, seed):
     return reqs
 
 
+def run_config1(steps=5):
+    """128^2, one central pore (porosity 0.005), elastic (nTasks=1), plain
+    FP64 (the reference's four kernels), preload -1e-5 along [1,1,0],
+    eIter=1e-8: solves/s and the numerical moduli against the analytic
+    elastic ones the reference logs (EffPlast2D.cu:643-654)."""
+    from effplast2d_amd import EffPlast2D, derive_params
+    pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=128, nPores=1, porosity=0.005, Y=2.5e-3, K0=1.0,
+                           G0=0.5, rho0=1.0, CFL=0.5)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        with EffPlast2D(pa, nx, nTasks=1, nIter=200_000, eIter=1e-8, checkInterval=500,
+                        mode="plain") as g:
+            res = g.ComputeEffModuli(-1e-5, [1.0, 1.0, 0.0])
+    dt = time.perf_counter() - t0
+    out = {"workload": "128x128, one central pore, porosity 0.005, elastic (nTasks=1), plain "
+                       "FP64, preload -1e-5 [1,1,0], eIter 1e-8, check 500",
+           "solves_per_s": steps / dt, "iters": int(res["totalIters"]),
+           "Kd": res["num"]["Kd"], "Kphi": res["num"]["Kphi"],
+           "KdExactElast": res["KdExactElast"], "KphiExact": res["an"]["Kphi"],
+           "Kd_rel_vs_exact": abs(res["num"]["Kd"] - res["KdExactElast"]) / res["KdExactElast"],
+           "Kphi_rel_vs_exact": abs(res["num"]["Kphi"] - res["an"]["Kphi"]) / res["an"]["Kphi"]}
+    log(f"[bench] config1: {out['solves_per_s']:.1f} solves/s, Kd {out['Kd']:.5f} (exact "
+        f"{out['KdExactElast']:.5f}), Kphi {out['Kphi']:.4f} (exact {out['KphiExact']:.4f})")
+    return out
+
+
 def run_config4(n, seed, check):
     reqs = c4_requests(n, seed)
     res, st, dt = run_batch(reqs, "split", check)
@@ -691,6 +717,7 @@ def main():
             if isinstance(r, dict) and "moduli" in r}
         legs["speedup_vs_single_plain_fp64"] = head["solves_per_s"] / \
             legs["single_plain"]["solves_per_s"]
+        legs["config1_elastic_plain"] = run_config1()
         legs["config4_batched_sweep"], _ = run_config4(a.c4_rves, a.c4_seed, a.check_interval)
         legs["config5_serving"] = run_config5(a.serve_rate, a.serve_seconds, a.serve_seed,
                                               a.check_interval, a.serve_budget_gib)
