@@ -43,7 +43,7 @@ class Options(C.Structure):
                 ("nTimeSteps", C.c_int), ("useShearLoadType", C.c_int),
                 ("shearLoadType", C.c_double * 3), ("verbose", C.c_int),
                 ("logPath", C.c_char_p), ("tileRows", C.c_int), ("marchRows", C.c_int),
-                ("itersPerLaunch", C.c_int)]
+                ("itersPerLaunch", C.c_int), ("adaptiveCheck", C.c_int)]
 
 
 class Moduli(C.Structure):
@@ -93,7 +93,7 @@ class BatchOptions(C.Structure):
                 ("marchRows", C.c_int), ("persistRows", C.c_int), ("march2Rows", C.c_int),
                 ("pairRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
-                ("useGraphs", C.c_int), ("timingEvery", C.c_long)]
+                ("useGraphs", C.c_int), ("timingEvery", C.c_long), ("adaptiveCheck", C.c_int)]
 
 
 class Request(C.Structure):
@@ -159,6 +159,7 @@ class Ep2dError(RuntimeError):
 # Every symbol include/ep2d.h declares, with its ctypes signature.
 SIGNATURES = {
     "ep2d_last_error": ([], C.c_char_p),
+    "ep2d_struct_size": ([C.c_char_p], C.c_size_t),
     "ep2d_version": ([], C.c_char_p),
     "ep2d_device_count": ([], C.c_int),
     "ep2d_derive_params": ([C.POINTER(Physics), _dp, C.POINTER(C.c_int)], C.c_int),

@@ -20,7 +20,7 @@ from ._lib import check, lib
 class Batch:
     def __init__(self, mode="split", checkInterval=500, tileRows=0, marchRows=0, persistRows=0,
                  march2Rows=0, pairRows=0, device=0, memoryBudget=0, maxActive=0, useGraphs=True,
-                 timingEvery=0):
+                 timingEvery=0, adaptiveCheck=False):
         o = L.BatchOptions()
         lib().ep2d_batch_default_options(C.byref(o))
         o.mode = L.MODES[mode] if isinstance(mode, str) else int(mode)
@@ -35,6 +35,7 @@ class Batch:
         o.maxActive = int(maxActive)
         o.useGraphs = 1 if useGraphs else 0
         o.timingEvery = int(timingEvery)
+        o.adaptiveCheck = 1 if adaptiveCheck else 0
         self._opts = o
         h = C.c_void_p()
         check(lib().ep2d_batch_create(C.byref(o), C.byref(h)))

@@ -137,6 +137,7 @@ void Batch::admit() {
     std::memcpy(o.shearLoadType, jb.rq.shearLoadType, sizeof(o.shearLoadType));
     o.tileRows = 0;
     o.itersPerLaunch = 1;   // the engine launches its own kernels
+    o.adaptiveCheck = opt_.adaptiveCheck;
     try {
       sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_, &pool_);
       sl.bytes = sl.s->deviceBytes();
@@ -302,10 +303,21 @@ void Batch::enqueue(long n, bool dbl, long& l, long& d) {
 // once per length while the tables stay the same.
 hipGraphExec_t Batch::graphFor(long len) {
   for (auto& g : graphs_)
-    if (g.len == len) return g.exec;
-  if (graphs_.size() >= 4) dropGraphs();
+    if (g.len == len) {
+      g.lastUse = st_.chunks;
+      return g.exec;
+    }
+  if ((int)graphs_.size() >= kMaxGraphs) {
+    auto lru = graphs_.begin();
+    for (auto it = graphs_.begin(); it != graphs_.end(); ++it)
+      if (it->lastUse < lru->lastUse) lru = it;
+    (void)hipGraphExecDestroy(lru->exec);
+    (void)hipGraphDestroy(lru->graph);
+    graphs_.erase(lru);
+  }
   GraphEntry g;
   g.len = len;
+  g.lastUse = st_.chunks;
   long l = 0, d = 0;
   EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
   enqueue(len, twoIter(shape_.kind), l, d);
@@ -484,8 +496,13 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
   const long main = two ? pairs : singles;   // launches of the main kernel
   if (opt_.useGraphs && !tuning) {
     // graph of the main launches; with timing on, the last of them runs
-    // outside the graph between dispatch events
-    const long glen = timed && main > 0 ? main - 1 : main;
+    // outside the graph between dispatch events; a short run (an adaptive
+    // check close to the last one) is launched directly, not captured
+    long glen = timed && main > 0 ? main - 1 : main;
+    if (glen < kMinGraphLaunches) {
+      enqueue(glen, two, l, d);
+      glen = 0;
+    }
     if (glen > 0) {
       const hipGraphExec_t ge = graphFor(glen);
       Pending gv{};

@@ -57,9 +57,12 @@ class Batch {
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
     long len = 0;
+    long lastUse = 0;   // chunk counter at the last replay (least recently used goes)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   };
+  static constexpr int kMaxGraphs = 6;
+  static constexpr long kMinGraphLaunches = 16;  // shorter runs launch directly
 
   size_t estimateBytes(const ep2d_request& rq) const;
   void admit();

@@ -100,6 +100,22 @@ int ep2d_write_pa(const char* path, const double pa[EP2D_NPARS]) {
   });
 }
 
+size_t ep2d_struct_size(const char* name) {
+  if (!name) return 0;
+  const std::string n(name);
+  if (n == "ep2d_physics") return sizeof(ep2d_physics);
+  if (n == "ep2d_options") return sizeof(ep2d_options);
+  if (n == "ep2d_moduli") return sizeof(ep2d_moduli);
+  if (n == "ep2d_record") return sizeof(ep2d_record);
+  if (n == "ep2d_result") return sizeof(ep2d_result);
+  if (n == "ep2d_an_static") return sizeof(ep2d_an_static);
+  if (n == "ep2d_batch_options") return sizeof(ep2d_batch_options);
+  if (n == "ep2d_request") return sizeof(ep2d_request);
+  if (n == "ep2d_batch_result") return sizeof(ep2d_batch_result);
+  if (n == "ep2d_batch_stats") return sizeof(ep2d_batch_stats);
+  return 0;
+}
+
 void ep2d_default_options(ep2d_options* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));

@@ -801,11 +801,36 @@ void Solver::tsBegin() {
   tsDone_ = 0;
   tsError_ = 0.0;
   tsConv_ = false;
+  nextCheck_ = opt_.checkInterval;
+  lastCheckAt_ = -1;
+  lastCheckErr_ = 0.0;
 }
 
 long Solver::tsChunk() const {
+  return std::min(nextCheck_ - tsDone_, opt_.nIter - tsDone_);
+}
+
+// Next residual check after the one at tsDone_ with residual `error`.  Fixed
+// schedule: every checkInterval iterations (EffPlast2D.cu:407, output_step).
+// Adaptive: the residual of the pseudo-transient iteration decays about
+// geometrically, so the rate over the last two checks predicts the iteration
+// where it reaches eIter; the next check goes there (at least K/64 and at
+// most K iterations on, so it never checks later than the fixed schedule; K
+// when the residual did not fall).  A missed prediction costs one more short
+// gap, a check costs a few iterations' time.
+void Solver::scheduleNextCheck(double error) {
   const long K = opt_.checkInterval;
-  return std::min(K - (tsDone_ % K), opt_.nIter - tsDone_);
+  long gap = K;
+  if (opt_.adaptiveCheck && lastCheckAt_ >= 0 && error > 0.0 && error < lastCheckErr_) {
+    const double rate = (std::log(error) - std::log(lastCheckErr_)) /
+                        (double)(tsDone_ - lastCheckAt_);
+    const double remaining = std::log(opt_.eIter / error) / rate;
+    const long lo = std::max<long>(1, K / 64), hi = K;
+    gap = remaining < (double)lo ? lo : (remaining > (double)hi ? hi : (long)std::ceil(remaining));
+  }
+  lastCheckAt_ = tsDone_;
+  lastCheckErr_ = error;
+  nextCheck_ = tsDone_ + gap;
 }
 
 bool Solver::tsOver() const { return tsConv_ || tsDone_ >= opt_.nIter; }
@@ -837,6 +862,7 @@ bool Solver::tsCheckFinish() {
   } else if (tsDone_ >= opt_.nIter) {
     say("WARNING: Maximum number of iterations reached!\nError is ", error, "\n\n");
   }
+  scheduleNextCheck(error);
   return tsOver();
 }
 
@@ -983,7 +1009,7 @@ void Solver::jobFinish(ep2d_result* out) {
 }
 
 // One RVE on its own: the iteration loop of EffPlast2D.cu:389-428 with a
-// residual check every checkInterval iterations.
+// residual check every checkInterval iterations (or on the adaptive schedule).
 void Solver::runJob(ep2d_result* out) {
   while (!jobDone()) {
     tsBegin();
@@ -991,7 +1017,7 @@ void Solver::runJob(ep2d_result* out) {
       const long c = tsChunk();
       iterate(c);
       tsDone_ += c;
-      if (tsDone_ % opt_.checkInterval == 0) {
+      if (tsAtCheck()) {
         tsCheckLaunch();
         EP2D_HIP(hipStreamSynchronize(stream_));
         tsCheckFinish();

@@ -65,10 +65,11 @@ class Solver {
   void jobBegin(int kind, double initLoadValue, const double loadType[3], double loadValue);
   bool jobDone() const { return task_ >= plan_.size(); }
   void tsBegin();                 // start a time step (affine increment)
+  void scheduleNextCheck(double error);
   long tsChunk() const;           // iterations to the next check or the cap
   bool tsOver() const;            // converged or iteration cap reached
   long tsDoneIters() const { return tsDone_; }
-  bool tsAtCheck() const { return tsDone_ > 0 && tsDone_ % opt_.checkInterval == 0; }
+  bool tsAtCheck() const { return tsDone_ > 0 && tsDone_ == nextCheck_; }
   void tsCheckLaunch();           // residual reduction + async copy of its result
   bool tsCheckFinish();           // after the stream is synchronised
   void tsFinish();                // averages, boundary, hole area, analytic moduli
@@ -202,6 +203,9 @@ class Solver {
   size_t task_ = 0;
   long tstep_ = 0;
   long tsDone_ = 0;
+  long nextCheck_ = 0;       // iteration of the next residual check in this time step
+  long lastCheckAt_ = -1;    // the previous check and its residual (adaptive schedule)
+  double lastCheckErr_ = 0.0;
   double tsError_ = 0.0;
   bool tsConv_ = false;
   int jobKind_ = 0;

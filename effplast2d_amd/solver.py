@@ -49,7 +49,8 @@ class EffPlast2D:
 
     def __init__(self, pa, nX, nTimeSteps=1, nTasks=3, nIter=100_000, eIter=1e-10,
                  checkInterval=10_000, mode="fp64", device=0, shearLoadType=None,
-                 verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=0):
+                 verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=0,
+                 adaptiveCheck=False):
         self.pa = np.ascontiguousarray(pa, dtype=np.float64)
         if self.pa.shape != (L.NPARS,):
             raise ValueError("pa must hold 11 values")
@@ -61,7 +62,8 @@ class EffPlast2D:
                                      nTimeSteps=nTimeSteps, shearLoadType=shearLoadType,
                                      verbose=1 if verbose else 0, logPath=logPath,
                                      tileRows=tileRows, marchRows=marchRows,
-                                     itersPerLaunch=itersPerLaunch)
+                                     itersPerLaunch=itersPerLaunch,
+                                     adaptiveCheck=1 if adaptiveCheck else 0)
         h = C.c_void_p()
         check(lib().ep2d_create(_dptr(self.pa), self.nX, self.nY, C.byref(self._opts),
                                 C.byref(h)))

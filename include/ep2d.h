@@ -106,6 +106,11 @@ typedef struct ep2d_options {
   int itersPerLaunch;   /* 2: with marchRows, two iterations per launch on wave pairs;
                            1: one; 0 (default) with tileRows = marchRows = 0: the
                            shape is chosen by grid size and scheme (same results) */
+  int adaptiveCheck;    /* extension, 0: check every checkInterval iterations;
+                           1: first check after checkInterval, then where the
+                           residual's decay rate over the last two checks
+                           predicts eIter (between checkInterval/64 and
+                           checkInterval later)                           */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
@@ -140,6 +145,11 @@ typedef struct ep2d_handle ep2d_handle;
 
 /* ---- library ------------------------------------------------------------ */
 const char* ep2d_last_error(void);
+
+/* sizeof of one of this header's structs by its name ("ep2d_options",
+ * "ep2d_result", ...; 0 for an unknown name), so a binding in another
+ * language can check its mirror of the layout.  No reference counterpart. */
+size_t ep2d_struct_size(const char* name);
 const char* ep2d_version(void);
 /* Number of visible HIP devices (0 when none; never an error). */
 int ep2d_device_count(void);
@@ -270,6 +280,8 @@ typedef struct ep2d_batch_options {
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
   int useGraphs;         /* capture iteration chunks in HIP graphs             */
   long timingEvery;      /* time every n-th launch with dispatch events (0: off) */
+  int adaptiveCheck;     /* residual checks scheduled by the decay rate
+                            (ep2d_options.adaptiveCheck) for every request   */
 } ep2d_batch_options;
 
 typedef struct ep2d_request {

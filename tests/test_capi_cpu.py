@@ -169,3 +169,16 @@ def test_cli_usage_error():
     r = subprocess.run([cli, "1", "2"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "ERROR:  missing arguments" in r.stderr and "<time steps>" in r.stderr
+
+
+def test_struct_layouts_match_the_header():
+    """Every ctypes mirror of an include/ep2d.h struct has the C size."""
+    from effplast2d_amd import _lib as L
+    lib = L.lib()
+    pairs = {"ep2d_physics": L.Physics, "ep2d_options": L.Options, "ep2d_moduli": L.Moduli,
+             "ep2d_record": L.Record, "ep2d_result": L.Result, "ep2d_an_static": L.AnStatic,
+             "ep2d_batch_options": L.BatchOptions, "ep2d_request": L.Request,
+             "ep2d_batch_result": L.BatchResult, "ep2d_batch_stats": L.BatchStats}
+    for name, cls in pairs.items():
+        assert lib.ep2d_struct_size(name.encode()) == C.sizeof(cls), name
+    assert lib.ep2d_struct_size(b"no_such_struct") == 0

@@ -197,3 +197,29 @@ def test_march2_batch_equals_single(mode, graphs, shape):
         b.run()
         r = b.poll()[0]
     _same(r["result"], ref)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+def test_adaptive_check_schedule(mode):
+    """Residual checks placed by the decay rate: every load step ends below
+    eIter, the solve takes at most one check interval more than the fixed
+    schedule (in practice fewer), the moduli agree with the fixed schedule's
+    to 1e-4, and the batch gives the single-RVE results with the same
+    schedule."""
+    _note(f"adaptive check {mode}")
+    nX, nP, phi, Y, nl, load = REQS[0]
+    pa = make_pa(nX, nPores=nP, porosity=phi, Y=Y)
+    runs = {}
+    for adaptive in (False, True):
+        with EffPlast2D(pa, nX, nTasks=nl, nIter=200_000, eIter=1e-8, checkInterval=K,
+                        mode=mode, adaptiveCheck=adaptive) as g:
+            runs[adaptive] = (g.ComputeEffModuli(load, LT), g.records())
+    (rf, recf), (ra, reca) = runs[False], runs[True]
+    assert all(r["error"] < 1e-8 for r in reca)
+    assert sum(a["iters"] for a in reca) <= sum(f["iters"] for f in recf) + K
+    # both stop below eIter, at different iterations: the moduli agree to the
+    # solver's own convergence (Kphi, a small difference quotient, to ~1e-5)
+    for k in ("Kphi", "Kd"):
+        assert ra["num"][k] == pytest.approx(rf["num"][k], rel=1e-4), k
+    got, _ = _batch(REQS, mode, adaptiveCheck=True)
+    _same(got[0]["result"], ra)
