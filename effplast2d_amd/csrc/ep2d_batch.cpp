@@ -97,6 +97,8 @@ long long Batch::submit(const ep2d_request& rq) {
   if (rq.nTasks < 1 || rq.nIter < 1 || rq.nTimeSteps < 1)
     throw ArgError("request: nTasks, nIter and nTimeSteps must be >= 1");
   if (rq.checkInterval < 0) throw ArgError("request: checkInterval must be >= 0");
+  if (fixed_ && twoIter(shape_.kind) && !two_iter_fits(rq.nX, rq.nX))
+    throw ArgError("request: march2Rows and pairRows need every field under 2 GiB (nX <= 16382)");
   queue_.push_back({nextId_, rq, clock()});
   st_.submitted++;
   return nextId_++;
@@ -337,19 +339,27 @@ Batch::Seq Batch::launchChunk(long c) {
   std::vector<long long> ids;
   ids.reserve(active_.size() + 3);
   double bytes = 0.0, cells = 0.0;
+  bool twoOk = true;  // every field under the two-iteration kernels' bound
   for (auto& sl : active_) {
     ids.push_back(sl.id);
     const double n = sl.s->nX();
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
+    twoOk = twoOk && two_iter_fits(sl.s->nX(), sl.s->nX());
   }
   if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
+  if (!fixed_ && !twoOk && twoIter(shape_.kind)) {
+    shape_ = Shape{kTiles, 14};
+    tuneDue_ = true;
+  }
+  auto allowed = [&](int k) { return twoOk || !twoIter(kCand[k].kind); };
   // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
   // every candidate, as many as the chunk holds (at least one timed round)
   constexpr long kTuneEach = 4;
   long roundIters = 0;
-  for (int k = 0; k < kNCand; k++) roundIters += twoIter(kCand[k].kind) ? 4 : 2;
+  for (int k = 0; k < kNCand; k++)
+    if (allowed(k)) roundIters += twoIter(kCand[k].kind) ? 4 : 2;
   const long rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
   const bool tuning = !fixed_ && tuneDue_ && rounds >= 2;
   ids.push_back(tuning ? 1 : 0);
@@ -377,6 +387,7 @@ Batch::Seq Batch::launchChunk(long c) {
     // buffer before the next candidate)
     for (long round = 0; round < rounds; round++)
       for (int k = 0; k < kNCand; k++) {
+        if (!allowed(k)) continue;
         const Shape sh = kCand[k];
         const int4* w = dTiles_ + candOff_[k];
         const bool dbl = twoIter(sh.kind);
@@ -412,7 +423,7 @@ Batch::Seq Batch::launchChunk(long c) {
       int near[kNCand], nn = 0;
       const double best = st_.candUs[candIndex(shape_)];
       for (int k = 0; k < kNCand && nn < 3; k++)
-        if (st_.candUs[k] > 0 && st_.candUs[k] <= 1.03 * best) near[nn++] = k;
+        if (allowed(k) && st_.candUs[k] > 0 && st_.candUs[k] <= 1.03 * best) near[nn++] = k;
       constexpr int kRefRounds = 4, kRefLaunches = 4;
       long need = 0;
       for (int q = 0; q < nn; q++)

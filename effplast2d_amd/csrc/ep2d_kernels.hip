@@ -261,6 +261,14 @@ __device__ __forceinline__ void gst(T* base, unsigned idx, T v) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void* p) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
 }
+// A resource bounded to the field's bytes (under 2 GiB): an access whose lane
+// offset is kOob or more lies past the bound, so a load returns 0 and a store
+// is dropped by the hardware.  A bound of 0 (bnull) does that for every lane.
+constexpr unsigned kOob = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc_n(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 template <typename T>
 __device__ __forceinline__ T bld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
@@ -1513,9 +1521,11 @@ struct MarchStage {
 
 // Stage A's inputs in a two-iteration segment: the first rows (start) and,
 // for step t, Ux row t+1, Uy row t+2, mask row t+1 and V row t of the state in
-// HBM.  Interior segments read through buffer resources (the lane's byte
-// offset fixed, the row's in an SGPR) and clamp rows to the segment's last
-// step tEnd; edge segments guard every load.
+// HBM, through buffer resources (the lane's byte offset fixed, the row's in an
+// SGPR).  Interior segments clamp rows to the segment's last step tEnd.  Edge
+// segments bound each resource to its field: lanes outside the field's
+// columns carry the offset kOob, and a row outside the field reads through a
+// resource of bound 0, so every such load returns 0 without a branch.
 template <typename CT, typename VT, bool EDGE>
 struct StageInput {
   using Ld = typename March<CT, VT>::Ld;
@@ -1527,14 +1537,36 @@ struct StageInput {
   int nX, nY, sx, i, tEnd;
   unsigned o8, oV;
   __amdgpu_buffer_rsrc_t bUx, bUy, bVx, bVy, bM;
+  // edge segments: lane offsets with the column test folded in (Ux columns
+  // 0..nX, the others 0..nX-1; V is read for columns 0..nX-1 only)
+  unsigned oUx, oC8, oM, oVc;
 
   __device__ __forceinline__ StageInput(const RveDev& r, int cur, int vpar, int col, int tEnd_)
       : Uxi(r.Ux[cur]), Uyi(r.Uy[cur]),
         Vxi(static_cast<const VT*>(vpar ? r.Vx2 : r.Vx)),
         Vyi(static_cast<const VT*>(vpar ? r.Vy2 : r.Vy)), mask(r.mask), nX(r.nX), nY(r.nY),
         sx(r.nX + 1), i(col), tEnd(tEnd_), o8((unsigned)col * 8u),
-        oV((unsigned)col * (unsigned)sizeof(VT)), bUx(brsrc(Uxi)), bUy(brsrc(Uyi)),
-        bVx(brsrc(Vxi)), bVy(brsrc(Vyi)), bM(brsrc(mask)) {}
+        oV((unsigned)col * (unsigned)sizeof(VT)) {
+    if (!EDGE) {
+      bUx = brsrc(Uxi);
+      bUy = brsrc(Uyi);
+      bVx = brsrc(Vxi);
+      bVy = brsrc(Vyi);
+      bM = brsrc(mask);
+    } else {
+      const unsigned ux = (unsigned)nY * (unsigned)sx, uy = (unsigned)(nY + 1) * (unsigned)nX;
+      bUx = brsrc_n(Uxi, ux * 8u);
+      bUy = brsrc_n(Uyi, uy * 8u);
+      bVx = brsrc_n(Vxi, ux * (unsigned)sizeof(VT));
+      bVy = brsrc_n(Vyi, uy * (unsigned)sizeof(VT));
+      bM = brsrc_n(mask, (unsigned)nY * (unsigned)nX);
+      const bool cx = col >= 0 && col <= nX, cc = col >= 0 && col < nX;
+      oUx = cx ? o8 : kOob;
+      oC8 = cc ? o8 : kOob;
+      oM = cc ? (unsigned)col : kOob;
+      oVc = cc ? oV : kOob;
+    }
+  }
 
   // Ux row jb, Uy row jb+1, mask row jb into the stage
   __device__ __forceinline__ void start(MarchStage<CT, VT, EDGE>& st, int jb) const {
@@ -1563,13 +1595,18 @@ struct StageInput {
       d.vx = bld<VT>(bVx, oV, (unsigned)t * (unsigned)sx * (unsigned)sizeof(VT));
       d.vy = bld<VT>(bVy, oV, (unsigned)t * (unsigned)nX * (unsigned)sizeof(VT));
     } else {
-      const bool ci = i >= 0 && i < nX;
-      d.ux = (i >= 0 && i <= nX && t + 1 >= 0 && t + 1 < nY) ? gld(Uxi, (unsigned)((t + 1) * sx + i)) : 0.0;
-      d.uy = (ci && t + 2 >= 0 && t + 2 <= nY) ? gld(Uyi, (unsigned)((t + 2) * nX + i)) : 0.0;
-      d.m = (ci && t + 1 >= 0 && t + 1 < nY) ? (unsigned)gld(mask, (unsigned)((t + 1) * nX + i)) : 0u;
-      const bool vr = ci && t >= 0 && t < nY;
-      d.vx = vr ? gld(Vxi, (unsigned)(t * sx + i)) : (VT)0;
-      d.vy = vr ? gld(Vyi, (unsigned)(t * nX + i)) : (VT)0;
+      // rows outside a field read through a resource of bound 0 (the row
+      // tests are uniform: scalar selects); the row offset is then 0
+      const __amdgpu_buffer_rsrc_t none = brsrc_n(Uxi, 0u);
+      const bool r1 = t + 1 >= 0 && t + 1 < nY, r2 = t + 2 >= 0 && t + 2 <= nY,
+                 r0 = t >= 0 && t < nY;
+      const unsigned s1 = r1 ? (unsigned)(t + 1) : 0u, s2 = r2 ? (unsigned)(t + 2) : 0u,
+                     s0 = r0 ? (unsigned)t : 0u;
+      d.ux = bld<double>(r1 ? bUx : none, oUx, s1 * (unsigned)sx * 8u);
+      d.uy = bld<double>(r2 ? bUy : none, oC8, s2 * (unsigned)nX * 8u);
+      d.m = bld<unsigned char>(r1 ? bM : none, oM, s1 * (unsigned)nX);
+      d.vx = bld<VT>(r0 ? bVx : none, oVc, s0 * (unsigned)sx * (unsigned)sizeof(VT));
+      d.vy = bld<VT>(r0 ? bVy : none, oVc, s0 * (unsigned)nX * (unsigned)sizeof(VT));
     }
     return d;
   }
@@ -1728,8 +1765,13 @@ __device__ __forceinline__ void march2p_run(const RveDev& r, int cur, int vpar, 
     VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
     const bool outLane = L >= 4 && L < 60 && (!EDGE || i < nX);
     const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
-    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo), bVx = brsrc(Vxo),
-                                 bVy = brsrc(Vyo);
+    // edge segments bound the V resources to their fields: a lane whose V
+    // ComputeDisp leaves alone stores at kOob, which the hardware drops
+    const unsigned vxb = (unsigned)nY * (unsigned)sx * (unsigned)sizeof(VT),
+                   vyb = (unsigned)(nY + 1) * (unsigned)nX * (unsigned)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo),
+                                 bVx = EDGE ? brsrc_n(Vxo, vxb) : brsrc(Vxo),
+                                 bVy = EDGE ? brsrc_n(Vyo, vyb) : brsrc(Vyo);
     // slot phase ph = q & 3 known at compile time (the loop is unrolled by
     // four): ring addresses are immediate offsets
     auto stepB = [&](int q, auto ph) {
@@ -1745,18 +1787,12 @@ __device__ __forceinline__ void march2p_run(const RveDev& r, int cur, int vpar, 
         const double uxsB = st.uxs_c;  // Ux(i+1, j) of B's input, before the step
         const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
         if (j >= j0 && outLane) {
-          if (!EDGE) {
-            // interior: every lane's V is updated
-            bst(bVx, oV, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
-            bst(bVy, oV, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
-            bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
-            bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
-          } else {
-            if (o.updX) gst(Vxo + (size_t)j * sx, (unsigned)i, (VT)o.vx);
-            if (o.updY) gst(Vyo + (size_t)j * nX, (unsigned)i, (VT)o.vy);
-            gst(Uxo + (size_t)j * sx, (unsigned)i, o.ux);
-            gst(Uyo + (size_t)j * nX, (unsigned)i, o.uy);
-          }
+          // interior: every lane's V is updated
+          const unsigned ovx = EDGE && !o.updX ? kOob : oV, ovy = EDGE && !o.updY ? kOob : oV;
+          bst(bVx, ovx, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
+          bst(bVy, ovy, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
+          bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
+          bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
           if (EDGE) {
             // nodes ComputeDisp never updates: Ux column nX, Uy row nY
             if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsB);

@@ -143,7 +143,11 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   mode_ = opt_.mode;
   if (nX < 8 || nY < 8) throw ArgError("nX and nY must be >= 8");
   if (nX != nY) throw ArgError("the RVE grid must be square (nX == nY), as in the reference");
-  if ((long)(nX + 1) * (long)(nY + 1) > (1L << 30)) throw ArgError("grid too large");
+  // the fused kernels address a field with 32-bit byte offsets
+  if ((long)(nX + 1) * (long)(nY + 1) > (1L << 29))
+    throw ArgError("grid too large: every field must stay under 4 GiB (nX, nY <= 23169)");
+  if (opt_.itersPerLaunch == 2 && !two_iter_fits(nX, nY))
+    throw ArgError("itersPerLaunch 2 needs every field under 2 GiB (nX, nY <= 16382)");
   if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");
   if (opt_.nTasks < 1) throw ArgError("nTasks must be >= 1");
   if (opt_.nIter < 1) throw ArgError("nIter must be >= 1");
@@ -267,7 +271,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     // where that measured faster than the tile kernel (split from 1024^2,
     // FP64 from 2048^2), with segments short enough to give the GPU waves
     const int n = std::max(nX, nY);
-    if ((mode_ == EP2D_MODE_SPLIT && n >= 1024) || (mode_ == EP2D_MODE_FP64 && n >= 2048)) {
+    if (((mode_ == EP2D_MODE_SPLIT && n >= 1024) || (mode_ == EP2D_MODE_FP64 && n >= 2048)) &&
+        two_iter_fits(nX, nY)) {
       opt_.marchRows = n <= 1024 ? 8 : (n <= 2048 ? 16 : 26);
       opt_.itersPerLaunch = 2;
     }

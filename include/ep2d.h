@@ -103,9 +103,10 @@ typedef struct ep2d_options {
                            (EffPlast2D.log, :1607) */
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
   int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
-  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch on wave pairs;
-                           1: one; 0 (default) with tileRows = marchRows = 0: the
-                           shape is chosen by grid size and scheme (same results) */
+  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch on wave pairs
+                           (nX, nY <= 16382: fields under 2 GiB); 1: one; 0
+                           (default) with tileRows = marchRows = 0: the shape is
+                           chosen by grid size and scheme (same results) */
   int adaptiveCheck;    /* extension, 0: check every checkInterval iterations;
                            1: first check after checkInterval, then where the
                            residual's decay rate over the last two checks
@@ -167,7 +168,8 @@ void ep2d_default_options(ep2d_options* opt);
 
 /* ---- solver handle -------------------------------------------------------
  * EffPlast2D::EffPlast2D (EffPlast2D.cu:1541-1612): allocate and set up the
- * fields for an nX x nY grid described by pa[]. */
+ * fields for an nX x nY grid described by pa[] (8 <= nX = nY <= 23169: every
+ * field under 4 GiB). */
 int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY,
                 const ep2d_options* opt, ep2d_handle** out);
 /* Convenience: derive pa[] from phys, then ep2d_create. */
@@ -274,7 +276,9 @@ typedef struct ep2d_batch_options {
   int persistRows;       /* fixed persistent tile kernel of this height        */
   int march2Rows;        /* fixed two-iteration marching kernel, segment rows  */
   int pairRows;          /* fixed two-iteration marching on wave pairs, rows   */
+                         /* (both: requests up to nX = 16382)                   */
                          /* all 0: kernel and shape tuned on the running batch  */
+                         /* (two-iteration shapes while every RVE has nX <= 16382) */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
