@@ -317,6 +317,46 @@ __device__ __forceinline__ bool rm_scale(CT s2, CT coh, CT c2, CT& sc) {
   }
 }
 
+// The split scheme's FP32 arithmetic.  Every kernel computes these quantities
+// through the functions below, so the split kernels agree with each other bit
+// for bit; a product is fused into the sum that takes it (one rounding, one
+// instruction).  The FP64 scheme keeps the reference's separate products and
+// sums (EffPlast2D.cu:16-33, 54-75, 109-176) and does not use them.
+//
+// trial cell stress from the strain pair: P0 - K (exx + eyy) and
+// 2G (e - (exx + eyy) / 3), with g2 = 2G
+__device__ __forceinline__ void split_cell_trial(float exx, float eyy, float K, float g2, float P0,
+                                                 float& pp, float& txx, float& tyy) {
+  const float s = exx + eyy;
+  pp = __builtin_fmaf(-K, s, P0);
+  txx = g2 * __builtin_fmaf(s, -1.0f / 3.0f, exx);
+  tyy = g2 * __builtin_fmaf(s, -1.0f / 3.0f, eyy);
+}
+// trial vertex shear: g (a / dY + b / dX) from the differences a of Ux in y
+// and b of Uy in x
+__device__ __forceinline__ float split_vertex_trial(float g, float a, float rdY, float b,
+                                                   float rdX) {
+  return g * __builtin_fmaf(a, rdY, b * rdX);
+}
+// squared invariants the return mapping tests: vertex, from the four-cell
+// sums of the normal stresses; cell, from the four-vertex mean av
+__device__ __forceinline__ float split_s2_vertex(float sxx, float syy, float txy) {
+  return __builtin_fmaf(txy + txy, txy, 0.0625f * __builtin_fmaf(sxx, sxx, syy * syy));
+}
+__device__ __forceinline__ float split_s2_cell(float txx, float tyy, float av) {
+  return __builtin_fmaf(av + av, av, __builtin_fmaf(txx, txx, tyy * tyy));
+}
+// damped velocity: v damp + dTrho (a ra + b rb), ra and rb the reciprocal
+// grid steps of the two stress differences a and b
+__device__ __forceinline__ float split_velocity(float v, float damp, float dTrho, float a,
+                                               float ra, float b, float rb) {
+  return __builtin_fmaf(v, damp, dTrho * __builtin_fmaf(a, ra, b * rb));
+}
+// displacement, FP64: u + v dT with one rounding
+__device__ __forceinline__ double split_disp(double u, float v, double dT) {
+  return __builtin_fma((double)v, dT, u);
+}
+
 // Workgroup barrier that waits for LDS traffic only: global loads issued
 // before it (the next tile's prefetch) stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -512,9 +552,13 @@ struct Fused {
           const CT G = (m & M_CELL_MAT) ? Gp : G0;
           const CT exx = dx(sUx[(b + 1) * BX + a + 2] - sUx[(b + 1) * BX + a + 1], r);
           const CT eyy = dy(sUy[(b + 2) * BX + a + 1] - sUy[(b + 1) * BX + a + 1], r);
-          pp = (CT)r.P0 - K * (exx + eyy);
-          txx = (CT)2.0 * G * (exx - third(exx + eyy));
-          tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
+          if constexpr (sizeof(CT) == 8) {
+            pp = (CT)r.P0 - K * (exx + eyy);
+            txx = (CT)2.0 * G * (exx - third(exx + eyy));
+            tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
+          } else {
+            split_cell_trial(exx, eyy, K, (CT)2.0 * G, (CT)r.P0, pp, txx, tyy);
+          }
           if (m & M_CELL_ZERO) {
             pp = 0;
             txx = 0;
@@ -531,8 +575,10 @@ struct Fused {
         if (!EDGE || (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1)) {
           const unsigned char m00 = sM[b * BX + a];
           const CT g = gt[(m00 >> M_VCLASS_SHIFT) & 15];
-          txy = g * (dy(sUx[(b + 1) * BX + a + 1] - sUx[b * BX + a + 1], r) +
-                     dx(sUy[(b + 1) * BX + a + 1] - sUy[(b + 1) * BX + a], r));
+          const double du = sUx[(b + 1) * BX + a + 1] - sUx[b * BX + a + 1];
+          const double dv = sUy[(b + 1) * BX + a + 1] - sUy[(b + 1) * BX + a];
+          if constexpr (sizeof(CT) == 8) txy = g * (dy(du, r) + dx(dv, r));
+          else txy = split_vertex_trial(g, (CT)du, (CT)r.rdY, (CT)dv, (CT)r.rdX);
           if (m00 & M_VERT_ZERO) txy = 0;
         }
         sTxy[b * S + a] = txy;
@@ -558,7 +604,11 @@ struct Fused {
           if (!EDGE || (vi >= 0 && vi < nX - 1 && vj >= 0 && vj < nY - 1)) {
             const CT sxx = sTxx[b * S + a] + sTxx[b * S + a + 1] + sTxx[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1];
             const CT syy = sTyy[b * S + a] + sTyy[b * S + a + 1] + sTyy[(b + 1) * S + a] + sTyy[(b + 1) * S + a + 1];
-            const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+            CT s2;
+            if constexpr (sizeof(CT) == 8)
+              s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+            else
+              s2 = split_s2_vertex(sxx, syy, txy);
             CT sc;
             if (rm_scale(s2, coh, c2lo, sc)) txy *= sc;
           }
@@ -580,7 +630,9 @@ struct Fused {
             const CT av = (CT)0.25 * (sTxy[vb * S + va] + sTxy[vb * S + va + 1] +
                                       sTxy[(vb + 1) * S + va] + sTxy[(vb + 1) * S + va + 1]);
             const CT txx = sTxx[b * S + a], tyy = sTyy[b * S + a];
-            const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+            CT s2;
+            if constexpr (sizeof(CT) == 8) s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+            else s2 = split_s2_cell(txx, tyy, av);
             CT sc;
             if (rm_scale(s2, coh, c2lo, sc)) {
               sTxx[b * S + a] = txx * sc;
@@ -611,17 +663,26 @@ struct Fused {
       CT vx = (CT)p.vxr[k];
       CT vy = (CT)p.vyr[k];
       if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
-        vx = vx * vdampX + dTrho * (dx(-sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a], r) +
-                                    dy(Txy(a, b + 1) - Txy(a, b), r));
+        const CT fa = -sP[(b + 1) * S + a + 1] + sP[(b + 1) * S + a] + sTxx[(b + 1) * S + a + 1] - sTxx[(b + 1) * S + a];
+        const CT fb = Txy(a, b + 1) - Txy(a, b);
+        if constexpr (sizeof(CT) == 8) vx = vx * vdampX + dTrho * (dx(fa, r) + dy(fb, r));
+        else vx = split_velocity(vx, vdampX, dTrho, fa, (CT)r.rdX, fb, (CT)r.rdY);
         gst(Vx + (size_t)j * sx, (unsigned)i, (VT)vx);
       }
       if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
-        vy = vy * vdampY + dTrho * (dy(-sP[(b + 1) * S + a + 1] + sP[b * S + a + 1] + sTyy[(b + 1) * S + a + 1] - sTyy[b * S + a + 1], r) +
-                                    dx(Txy(a + 1, b) - Txy(a, b), r));
+        const CT fa = -sP[(b + 1) * S + a + 1] + sP[b * S + a + 1] + sTyy[(b + 1) * S + a + 1] - sTyy[b * S + a + 1];
+        const CT fb = Txy(a + 1, b) - Txy(a, b);
+        if constexpr (sizeof(CT) == 8) vy = vy * vdampY + dTrho * (dy(fa, r) + dx(fb, r));
+        else vy = split_velocity(vy, vdampY, dTrho, fa, (CT)r.rdY, fb, (CT)r.rdX);
         gst(Vy + (size_t)j * nX, (unsigned)i, (VT)vy);
       }
-      gst(Uxo + (size_t)j * sx, (unsigned)i, uxo[k] + (double)vx * r.dT);
-      gst(Uyo + (size_t)j * nX, (unsigned)i, uyo[k] + (double)vy * r.dT);
+      if constexpr (sizeof(CT) == 8) {
+        gst(Uxo + (size_t)j * sx, (unsigned)i, uxo[k] + (double)vx * r.dT);
+        gst(Uyo + (size_t)j * nX, (unsigned)i, uyo[k] + (double)vy * r.dT);
+      } else {
+        gst(Uxo + (size_t)j * sx, (unsigned)i, split_disp(uxo[k], vx, r.dT));
+        gst(Uyo + (size_t)j * nX, (unsigned)i, split_disp(uyo[k], vy, r.dT));
+      }
       if (EDGE) {
         if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxe[k]);
         if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uye[k]);
@@ -1086,13 +1147,19 @@ struct March {
   // Return mapping of a vertex from its trial shear stress and the four trial
   // cells around it, summed in the reference's order.
   static __device__ __forceinline__ CT vertex_rm(CT txy, CT sxx, CT syy, CT coh, CT c2lo) {
-    const CT s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+    CT s2;
+    if constexpr (sizeof(CT) == 8)
+      s2 = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * txy * txy;
+    else
+      s2 = split_s2_vertex(sxx, syy, txy);
     CT sc;
     if (rm_scale(s2, coh, c2lo, sc)) txy *= sc;
     return txy;
   }
   static __device__ __forceinline__ void cell_rm(CT& txx, CT& tyy, CT av, CT coh, CT c2lo) {
-    const CT s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+    CT s2;
+    if constexpr (sizeof(CT) == 8) s2 = txx * txx + tyy * tyy + (CT)2.0 * av * av;
+    else s2 = split_s2_cell(txx, tyy, av);
     CT sc;
     if (rm_scale(s2, coh, c2lo, sc)) {
       txx = txx * sc;
@@ -1163,9 +1230,14 @@ struct March {
         const CT G = (m_n & M_CELL_MAT) ? Gp : G0;
         const CT exx = dx(uxs_n - ux_n, r);
         const CT eyy = dy(uy_nn - uy_n, r);
-        CT pp = P0 - K * (exx + eyy);
-        CT txx = (CT)2.0 * G * (exx - third(exx + eyy));
-        CT tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
+        CT pp, txx, tyy;
+        if constexpr (sizeof(CT) == 8) {
+          pp = P0 - K * (exx + eyy);
+          txx = (CT)2.0 * G * (exx - third(exx + eyy));
+          tyy = (CT)2.0 * G * (eyy - third(exx + eyy));
+        } else {
+          split_cell_trial(exx, eyy, K, (CT)2.0 * G, P0, pp, txx, tyy);
+        }
         const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
         if ((m_n & M_CELL_ZERO) || !ok) {
           pp = 0;
@@ -1183,7 +1255,10 @@ struct March {
       CT tXY_c;
       {
         const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
-        CT txy = g * (dy(uxs_n - uxs_c, r) + dx(uys_n - uy_n, r));
+        CT txy;
+        if constexpr (sizeof(CT) == 8) txy = g * (dy(uxs_n - uxs_c, r) + dx(uys_n - uy_n, r));
+        else txy = split_vertex_trial(g, (CT)(uxs_n - uxs_c), (CT)r.rdY, (CT)(uys_n - uy_n),
+                                      (CT)r.rdX);
         const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
         if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
         tXY_c = txy;
@@ -1237,16 +1312,29 @@ struct March {
       if (j >= j0) {
         CT vx = (CT)ld.vx, vy = (CT)ld.vy;
         if (!EDGE || (i > 0 && j > 0 && j < nY - 1)) {
-          vx = vx * vdampX + dTrho * (dx(-cP_c + rP + cXX_c - rXX, r) + dy(rXY_c - rXY_p, r));
+          if constexpr (sizeof(CT) == 8)
+            vx = vx * vdampX + dTrho * (dx(-cP_c + rP + cXX_c - rXX, r) + dy(rXY_c - rXY_p, r));
+          else
+            vx = split_velocity(vx, vdampX, dTrho, -cP_c + rP + cXX_c - rXX, (CT)r.rdX,
+                                rXY_c - rXY_p, (CT)r.rdY);
           if (outLane) gst(Vx + (size_t)j * sx, (unsigned)i, (VT)vx);
         }
         if (!EDGE || (i > 0 && i < nX - 1 && j > 0)) {
-          vy = vy * vdampY + dTrho * (dy(-cP_c + cP_p + cYY_c - cYY_p, r) + dx(cXY_p - rXY_p, r));
+          if constexpr (sizeof(CT) == 8)
+            vy = vy * vdampY + dTrho * (dy(-cP_c + cP_p + cYY_c - cYY_p, r) + dx(cXY_p - rXY_p, r));
+          else
+            vy = split_velocity(vy, vdampY, dTrho, -cP_c + cP_p + cYY_c - cYY_p, (CT)r.rdY,
+                                cXY_p - rXY_p, (CT)r.rdX);
           if (outLane) gst(Vy + (size_t)j * nX, (unsigned)i, (VT)vy);
         }
         if (outLane && (!EDGE || j < nY)) {
-          gst(Uxo + (size_t)j * sx, (unsigned)i, ux_c + (double)vx * dT);
-          gst(Uyo + (size_t)j * nX, (unsigned)i, uy_c + (double)vy * dT);
+          if constexpr (sizeof(CT) == 8) {
+            gst(Uxo + (size_t)j * sx, (unsigned)i, ux_c + (double)vx * dT);
+            gst(Uyo + (size_t)j * nX, (unsigned)i, uy_c + (double)vy * dT);
+          } else {
+            gst(Uxo + (size_t)j * sx, (unsigned)i, split_disp(ux_c, vx, dT));
+            gst(Uyo + (size_t)j * nX, (unsigned)i, split_disp(uy_c, vy, dT));
+          }
           if (EDGE) {
             if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxs_c);
             if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), uy_n);
@@ -1418,9 +1506,14 @@ struct MarchStage {
       const CT G2 = (m_n & M_CELL_MAT) ? G2_p : G2_0;
       const CT exx = dxs(uxs_n - ux_n);
       const CT eyy = dys(uy_nn - uy_n);
-      CT pp = P0 - K * (exx + eyy);
-      CT txx = G2 * (exx - M::third(exx + eyy));
-      CT tyy = G2 * (eyy - M::third(exx + eyy));
+      CT pp, txx, tyy;
+      if constexpr (sizeof(CT) == 8) {
+        pp = P0 - K * (exx + eyy);
+        txx = G2 * (exx - M::third(exx + eyy));
+        tyy = G2 * (eyy - M::third(exx + eyy));
+      } else {
+        split_cell_trial(exx, eyy, K, G2, P0, pp, txx, tyy);
+      }
       const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
       if ((m_n & M_CELL_ZERO) || !ok) {
         pp = 0;
@@ -1437,7 +1530,9 @@ struct MarchStage {
     CT tXY_c;
     {
       const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
-      CT txy = g * (dys(uxs_n - uxs_c) + dxs(uys_n - uy_n));
+      CT txy;
+      if constexpr (sizeof(CT) == 8) txy = g * (dys(uxs_n - uxs_c) + dxs(uys_n - uy_n));
+      else txy = split_vertex_trial(g, (CT)(uxs_n - uxs_c), rdYc, (CT)(uys_n - uy_n), rdXc);
       const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
       if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
       tXY_c = txy;
@@ -1487,15 +1582,29 @@ struct MarchStage {
     CT vx = vx_in, vy = vy_in;
     o.updX = !EDGE || (i > 0 && i < nX && j > 0 && j < nY - 1);
     o.updY = !EDGE || (i > 0 && i < nX - 1 && j > 0 && j < nY);
-    if (o.updX)
-      vx = (CT)(VT)(vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p)));
-    if (o.updY)
-      vy = (CT)(VT)(vy * vdampY + dTrho * (dys(-cP_c + cP_p + cYY_c - cYY_p) + dxs(cXY_p - rXY_p)));
+    if constexpr (sizeof(CT) == 8) {
+      if (o.updX)
+        vx = (CT)(VT)(vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p)));
+      if (o.updY)
+        vy = (CT)(VT)(vy * vdampY + dTrho * (dys(-cP_c + cP_p + cYY_c - cYY_p) + dxs(cXY_p - rXY_p)));
+    } else {
+      if (o.updX)
+        vx = (CT)(VT)split_velocity(vx, vdampX, dTrho, -cP_c + rP + cXX_c - rXX, rdXc,
+                                    rXY_c - rXY_p, rdYc);
+      if (o.updY)
+        vy = (CT)(VT)split_velocity(vy, vdampY, dTrho, -cP_c + cP_p + cYY_c - cYY_p, rdYc,
+                                    cXY_p - rXY_p, rdXc);
+    }
     const bool in = !EDGE || (inX && j >= 0 && j < nY);
     o.vx = vx;
     o.vy = vy;
-    o.ux = in ? ux_c + (double)vx * dT : ux_c;
-    o.uy = in ? uy_c + (double)vy * dT : uy_c;
+    if constexpr (sizeof(CT) == 8) {
+      o.ux = in ? ux_c + (double)vx * dT : ux_c;
+      o.uy = in ? uy_c + (double)vy * dT : uy_c;
+    } else {
+      o.ux = in ? split_disp(ux_c, vx, dT) : ux_c;
+      o.uy = in ? split_disp(uy_c, vy, dT) : uy_c;
+    }
     // rotate
     ux_c = ux_n;
     uxs_c = uxs_n;
