@@ -141,7 +141,8 @@ def percentile(xs, q):
 
 
 def shape_name(kind, rows):
-    return f"{ {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair'}[kind]}{rows}"
+    names = {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair', 6: 'triple', 7: 'quad'}
+    return f"{names[kind]}{rows}"
 
 
 def cand_us(st):
@@ -157,7 +158,7 @@ def batch_stats_summary(st, mode):
     single bracketed launch also carries the dispatch latency of the launch
     after a replay, so it reads longer than the kernel itself."""
     nan = float("nan")
-    ipl = 2 if st["kernel"] in (4, 5) else 1   # iterations per launch
+    ipl = {4: 2, 5: 2, 6: 3, 7: 4}.get(st["kernel"], 1)   # iterations per launch
     it_disp = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else nan
     it_graph = st["graphMs"] / st["graphLaunches"] if st["graphLaunches"] else None
     bpl = ipl * st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"] else nan
@@ -165,7 +166,8 @@ def batch_stats_summary(st, mode):
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
             3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>",
-            5: f"k_march2p<{ct}, {ct}>"}[st["kernel"]]
+            5: f"k_march2p<{ct}, {ct}>", 6: f"k_chain<{ct}, {ct}, 3>",
+            7: f"k_chain<{ct}, {ct}, 4>"}[st["kernel"]]
     return {"kernel": kern, "iterations_per_launch": ipl,
             "kernel_avg_us": kms * 1e3,
             "kernel_timing": "graph replays" if it_graph is not None else "dispatch events",
@@ -455,7 +457,8 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0):
+def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0, triple=0,
+          quad=0):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
     preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
     over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
@@ -468,7 +471,9 @@ def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0)
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
-        shape = ({"pairRows": pair} if pair else {"march2Rows": march2} if march2 else
+        shape = ({"quadRows": quad} if quad else {"tripleRows": triple} if triple else
+                 {"pairRows": pair} if pair else
+                 {"march2Rows": march2} if march2 else
                  {"marchRows": march} if march else {"tileRows": tile or 8})
         run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False, **shape)
     else:
@@ -667,6 +672,10 @@ def main():
                     help="probe: two-iteration marching segment rows (batched only)")
     ap.add_argument("--probe-pair", type=int, default=0,
                     help="probe: two-iteration marching on wave pairs, segment rows (batched only)")
+    ap.add_argument("--probe-triple", type=int, default=0,
+                    help="probe: three-iteration marching on wave chains, segment rows (batched only)")
+    ap.add_argument("--probe-quad", type=int, default=0,
+                    help="probe: four-iteration marching on wave chains, segment rows (batched only)")
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
@@ -693,7 +702,7 @@ def main():
         return 2
     if a.probe:
         probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
-              a.probe_march, a.probe_march2, a.probe_pair)
+              a.probe_march, a.probe_march2, a.probe_pair, a.probe_triple, a.probe_quad)
         return 0
     if a.sweep:
         shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]

@@ -91,7 +91,7 @@ class Result(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [("mode", C.c_int), ("checkInterval", C.c_long), ("tileRows", C.c_int),
                 ("marchRows", C.c_int), ("persistRows", C.c_int), ("march2Rows", C.c_int),
-                ("pairRows", C.c_int),
+                ("pairRows", C.c_int), ("tripleRows", C.c_int), ("quadRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
                 ("useGraphs", C.c_int), ("timingEvery", C.c_long), ("adaptiveCheck", C.c_int)]
 

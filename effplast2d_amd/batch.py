@@ -19,7 +19,8 @@ from ._lib import check, lib
 
 class Batch:
     def __init__(self, mode="split", checkInterval=500, tileRows=0, marchRows=0, persistRows=0,
-                 march2Rows=0, pairRows=0, device=0, memoryBudget=0, maxActive=0, useGraphs=True,
+                 march2Rows=0, pairRows=0, tripleRows=0, quadRows=0, device=0, memoryBudget=0,
+                 maxActive=0, useGraphs=True,
                  timingEvery=0, adaptiveCheck=False):
         o = L.BatchOptions()
         lib().ep2d_batch_default_options(C.byref(o))
@@ -30,6 +31,8 @@ class Batch:
         o.persistRows = int(persistRows)
         o.march2Rows = int(march2Rows)
         o.pairRows = int(pairRows)
+        o.tripleRows = int(tripleRows)
+        o.quadRows = int(quadRows)
         o.device = int(device)
         o.memoryBudget = int(memoryBudget)
         o.maxActive = int(maxActive)

@@ -29,7 +29,9 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
     {kTiles, 8},  {kTiles, 14}, {kTiles, 30}, {kMarch, 32}, {kPersist, 14},
     {kMarch2, 40}, {kPair, 8},   {kPair, 14},  {kPair, 20},  {kPair, 24},
-    {kPair, 26},  {kPair, 28},  {kPair, 32},  {kPair, 40},  {kPair, 52}};
+    {kPair, 26},  {kPair, 28},  {kPair, 32},  {kPair, 40},  {kPair, 52},
+    {kTriple, 28}, {kTriple, 42}, {kTriple, 56}, {kTriple, 86}, {kQuad, 32},
+    {kQuad, 64},  {kQuad, 128}};
 
 namespace {
 
@@ -46,16 +48,22 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
     throw ArgError("persistRows must be 0, 6, 8, 14 or 30");
   if (opt_.march2Rows < 0) throw ArgError("march2Rows must be >= 0");
   if (opt_.pairRows < 0) throw ArgError("pairRows must be >= 0");
+  if (opt_.tripleRows < 0) throw ArgError("tripleRows must be >= 0");
+  if (opt_.quadRows < 0) throw ArgError("quadRows must be >= 0");
   const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +
-                     (opt_.march2Rows != 0) + (opt_.pairRows != 0);
+                     (opt_.march2Rows != 0) + (opt_.pairRows != 0) + (opt_.tripleRows != 0) +
+                     (opt_.quadRows != 0);
   if (nfixed > 1)
-    throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows, pairRows");
+    throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows, pairRows, "
+                   "tripleRows, quadRows");
   fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
   else if (opt_.persistRows) shape_ = {kPersist, opt_.persistRows};
   else if (opt_.march2Rows) shape_ = {kMarch2, opt_.march2Rows};
   else if (opt_.pairRows) shape_ = {kPair, opt_.pairRows};
+  else if (opt_.tripleRows) shape_ = {kTriple, opt_.tripleRows};
+  else if (opt_.quadRows) shape_ = {kQuad, opt_.quadRows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -98,7 +106,9 @@ long long Batch::submit(const ep2d_request& rq) {
     throw ArgError("request: nTasks, nIter and nTimeSteps must be >= 1");
   if (rq.checkInterval < 0) throw ArgError("request: checkInterval must be >= 0");
   if (fixed_ && twoIter(shape_.kind) && !two_iter_fits(rq.nX, rq.nX))
-    throw ArgError("request: march2Rows and pairRows need every field under 2 GiB (nX <= 16382)");
+    throw ArgError(
+        "request: march2Rows, pairRows, tripleRows and quadRows need every field under 2 GiB "
+        "(nX <= 16382)");
   queue_.push_back({nextId_, rq, clock()});
   st_.submitted++;
   return nextId_++;
@@ -240,6 +250,10 @@ int Batch::buildWork(Shape sh, int4* out) const {
       t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
+    if (chain(sh.kind)) {
+      t += chain_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
+      continue;
+    }
     if (twoIter(sh.kind)) {
       t += march2_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
@@ -255,14 +269,18 @@ int Batch::buildWork(Shape sh, int4* out) const {
   return t;
 }
 
-// One launch of shape sh as the l-th launch of the chunk after d two-iteration
-// launches: U parity l, V parity d.  Two-iteration shapes launch k_march2
-// when dbl, else one marching iteration over the auxiliary list (V in place
-// in the current buffer).  The other kernels update V in place in the table's
-// buffer, so they are only enqueued when d is even.
+// One launch of shape sh as the l-th launch of the chunk after d multi-
+// iteration launches: U parity l, V parity d.  Two- and three-iteration
+// shapes launch their kernel when dbl, else one marching iteration over the
+// auxiliary list (V in place in the current buffer).  The other kernels
+// update V in place in the table's buffer, so they are only enqueued when d
+// is even.
 void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d, hipEvent_t e0,
                       hipEvent_t e1) {
-  if (twoIter(sh.kind) && dbl)
+  if (chain(sh.kind) && dbl)
+    launch_chain(ipl(sh.kind), dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_,
+                 e0, e1);
+  else if (twoIter(sh.kind) && dbl)
     launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1,
                   sh.kind == kPair);
   else if (twoIter(sh.kind))
@@ -282,12 +300,13 @@ int Batch::candIndex(Shape sh) {
   return 0;
 }
 
-// Split of `iters` iterations for the two-iteration shape: pairs first, then
-// one or two single iterations, so the chunk's last launch reads U^{n-1}
-// (the stress fields are recomputed from that buffer after a load step).
-static void split_two(long iters, long& pairs, long& singles) {
-  singles = iters <= 0 ? 0 : ((iters & 1) ? 1 : (iters >= 2 ? 2 : 0));
-  pairs = (iters - singles) / 2;
+// Split of `iters` iterations for a shape of n iterations per launch: the
+// multi-iteration launches first, then 1..n single iterations, so the chunk's
+// last launch reads U^{n-1} (the stress fields are recomputed from that
+// buffer after a load step).
+static void split_multi(long iters, int n, long& packs, long& singles) {
+  singles = iters <= 0 ? 0 : (iters - 1) % n + 1;
+  packs = (iters - singles) / n;
 }
 
 // `n` launches of the current shape from launch index l and double index d
@@ -359,7 +378,7 @@ Batch::Seq Batch::launchChunk(long c) {
   constexpr long kTuneEach = 4;
   long roundIters = 0;
   for (int k = 0; k < kNCand; k++)
-    if (allowed(k)) roundIters += twoIter(kCand[k].kind) ? 4 : 2;
+    if (allowed(k)) roundIters += 2 * ipl(kCand[k].kind);
   const long rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
   const bool tuning = !fixed_ && tuneDue_ && rounds >= 2;
   ids.push_back(tuning ? 1 : 0);
@@ -395,14 +414,14 @@ Batch::Seq Batch::launchChunk(long c) {
         if (round > 0) {
           ev = takeEvents();
           ev.cand = k;
-          ev.launches = dbl ? 4 : 2;   // iterations
+          ev.launches = 2 * ipl(sh.kind);   // iterations
           EP2D_HIP(hipEventRecord(ev.a, stream_));
         }
         for (int rep = 0; rep < 2; rep++) {
           launchOne(sh, w, candCnt_[k], dbl, l, d);
           l++;
           if (dbl) d++;
-          it += dbl ? 2 : 1;
+          it += ipl(sh.kind);
         }
         if (round > 0) {
           EP2D_HIP(hipEventRecord(ev.b, stream_));
@@ -427,7 +446,7 @@ Batch::Seq Batch::launchChunk(long c) {
       constexpr int kRefRounds = 4, kRefLaunches = 4;
       long need = 0;
       for (int q = 0; q < nn; q++)
-        need += kRefRounds * kRefLaunches * (twoIter(kCand[near[q]].kind) ? 2 : 1);
+        need += kRefRounds * kRefLaunches * ipl(kCand[near[q]].kind);
       if (nn > 1 && c - it >= need + 2) {
         for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
         for (int round = 0; round < kRefRounds; round++)
@@ -437,13 +456,13 @@ Batch::Seq Batch::launchChunk(long c) {
             const bool dbl = twoIter(sh.kind);
             Pending ev = takeEvents();
             ev.cand = k;
-            ev.launches = kRefLaunches * (dbl ? 2 : 1);
+            ev.launches = kRefLaunches * ipl(sh.kind);
             EP2D_HIP(hipEventRecord(ev.a, stream_));
             for (int rep = 0; rep < kRefLaunches; rep++) {
               launchOne(sh, dTiles_ + candOff_[k], candCnt_[k], dbl, l, d);
               l++;
               if (dbl) d++;
-              it += dbl ? 2 : 1;
+              it += ipl(sh.kind);
             }
             EP2D_HIP(hipEventRecord(ev.b, stream_));
             evPending_.push_back(ev);
@@ -502,8 +521,9 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
   const long rest = c - it;
   const bool timed = opt_.timingEvery > 0;
   const bool two = twoIter(shape_.kind);
+  const int n = ipl(shape_.kind);
   long pairs = 0, singles = rest;
-  if (two) split_two(rest, pairs, singles);
+  if (two) split_multi(rest, n, pairs, singles);
   const long main = two ? pairs : singles;   // launches of the main kernel
   if (opt_.useGraphs && !tuning) {
     // graph of the main launches; with timing on, the last of them runs
@@ -520,7 +540,7 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
       if (timed) {
         gv = takeEvents();
         gv.graph = true;
-        gv.launches = two ? 2 * glen : glen;
+        gv.launches = n * glen;
         EP2D_HIP(hipEventRecord(gv.a, stream_));
       }
       EP2D_HIP(hipGraphLaunch(ge, stream_));
@@ -533,8 +553,8 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
     }
     if (timed && main > 0) {
       Pending ev = takeEvents();
-      ev.bytes = (two ? 2 : 1) * bytes;
-      ev.launches = two ? 2 : 1;
+      ev.bytes = n * bytes;
+      ev.launches = n;
       launchOne(shape_, dTiles_, nTiles_, two, l, d, ev.a, ev.b);
       evPending_.push_back(ev);
       l++;
@@ -547,8 +567,8 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
       Pending ev{};
       if (t) {
         ev = takeEvents();
-        ev.bytes = (dbl ? 2 : 1) * bytes;
-        ev.launches = dbl ? 2 : 1;
+        ev.bytes = (dbl ? n : 1) * bytes;
+        ev.launches = dbl ? n : 1;
       }
       launchOne(shape_, dTiles_, nTiles_, dbl, l, d, ev.a, ev.b);
       l++;

@@ -45,14 +45,23 @@ class Batch {
     int cand = -1;     // shape candidate under tuning, -1 for statistics
     bool graph = false;  // a whole graph replay (launches counted in graphLaunches)
   };
-  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4, kPair = 5;
-  static bool twoIter(int kind) { return kind == kMarch2 || kind == kPair; }
+  static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4, kPair = 5,
+                       kTriple = 6, kQuad = 7;
+  // iterations per launch of a kernel kind; a kind with more than one runs
+  // the chunk's last iterations through the one-iteration marching kernel
+  static int ipl(int kind) {
+    return kind == kQuad ? 4 : kind == kTriple ? 3 : (kind == kMarch2 || kind == kPair ? 2 : 1);
+  }
+  static bool chain(int kind) { return kind == kTriple || kind == kQuad; }
+  static bool twoIter(int kind) { return ipl(kind) > 1; }
   struct Shape {
     int kind = kTiles;  // fused tile, row-marching, persistent tile, two-iteration
-                        // marching on one wave or on a pair of waves
+                        // marching on one wave or on a pair of waves,
+                        // three- or four-iteration marching on a chain of
+                        // three or four waves
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 15;
+  static constexpr int kNCand = 22;
   static const Shape kCand[kNCand];
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {

@@ -138,6 +138,13 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out);
 void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                    int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                    bool pairs = false);
+// ns = 3 or 4 iterations per launch on chains of ns waves (one per
+// iteration, rows handed over in LDS): strips of 64 - 4 ns output columns; U
+// and V buffers as for launch_march2.
+constexpr int chain_cols(int ns) { return 64 - 4 * ns; }
+int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out);
+void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
+                  int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // batched fused iteration over many RVEs: tiles[k] = (rve, i0, j0, parity
 // offset); RVE k reads U[parity ^ offset].  All RVEs share mode and tile height.
 void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,

@@ -1961,6 +1961,165 @@ __global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves
     march2p_run<CT, VT, true>(r, cur, vpar, s, j0, j1, gt[wave], stageB, ring[wave >> 1], nSteps);
 }
 
+// NS iterations per launch on a chain of NS waves (one workgroup).  Stage 0
+// reads the state from HBM and runs the first iteration, stage k the (k+1)-th
+// from the rows of stage k-1, and the last stage writes the result.  Each
+// stage hands its rows on through a four-row LDS ring, and each lags the one
+// before by three rows, as stage B lags stage A in march2p_run: lockstep step
+// q is row t = j0-2NS+q of stage 0 and row t-3k of stage k.  Every stage loses
+// two columns and two rows of halo, so strips carry 64-4NS output columns
+// (lanes 2NS..63-2NS) and stage 0 starts 2NS rows above the segment.  A launch
+// reads and writes every field once per NS iterations.
+//
+// Stage k >= 1 marches rows jb = j0-2(NS-k) .. j1-1+2(NS-1-k): it takes its
+// first rows at t = jb+3k-1 and steps row j = t-3k for t >= jb+3k, reading
+// Ux' of row j+1 (slot q-2), Uy' of row j+2 (slot q-1), V' of row j and mask
+// row j+1 (slot q-3) of its ring; a middle stage writes its row and that mask
+// row to slot q of the next ring.  Reads and the write of one step never share
+// a slot, and the barrier after each step orders a write before its reads.
+template <typename CT, typename VT, bool EDGE, int NS>
+__device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, int s, int j0,
+                                          int j1, const CT* gt, int role, PairRing<CT>* ring,
+                                          int nSteps) {
+  using M = March<CT, VT>;
+  using Ld = typename M::Ld;
+  const int nX = r.nX, nY = r.nY, sx = nX + 1;
+  const int L = threadIdx.x & 63;
+  const int i = s + L;
+  const int jbA = j0 - 2 * NS, tEnd = j1 + 2 * NS - 3;
+  MarchStage<CT, VT, EDGE> st;
+  st.setup(r, gt, i);
+  if (role == 0) {
+    // stage 0: rows jbA..tEnd from HBM into ring 0 (slot q & 3)
+    PairRing<CT>& out = ring[0];
+    const StageInput<CT, VT, EDGE> in(r, cur, vpar, i, tEnd);
+    in.start(st, jbA);
+    auto fetch = [&](int t) { return in.fetch(t); };
+    auto stepA = [&](int q, const Ld& ld, int sl) {
+      const int t = jbA + q;
+      if (t <= tEnd) {
+        const auto o = st.step(t, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
+        out.ux[sl][L] = o.ux;
+        out.uy[sl][L] = o.uy;
+        out.vx[sl][L] = o.vx;
+        out.vy[sl][L] = o.vy;
+        out.m[sl][L] = ld.m;
+      }
+      lds_barrier();
+    };
+    Ld P = fetch(jbA);
+    for (int q = 0; q < nSteps; q += 2) {
+      const Ld Q = fetch(jbA + q + 1);
+      stepA(q, P, q & 3);
+      if (q + 1 >= nSteps) break;
+      P = fetch(jbA + q + 2);
+      stepA(q + 1, Q, (q + 1) & 3);
+    }
+    return;
+  }
+  // every later stage is its own loop with its stage number K known at
+  // compile time: ring addresses are constants and no stage keeps another's
+  // values live
+  auto march = [&](auto K) {
+    constexpr int k = decltype(K)::value, lag = 3 * k;
+    constexpr bool last = k == NS - 1;
+    const int jb = j0 - 2 * (NS - k), tLast = tEnd + k;
+    PairRing<CT>& in = ring[k - 1];
+    PairRing<CT>& out = ring[last ? k - 1 : k];  // not written by the last stage
+    double* __restrict__ Uxo = r.Ux[cur ^ 1];
+    double* __restrict__ Uyo = r.Uy[cur ^ 1];
+    VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
+    VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
+    const bool outLane = L >= 2 * NS && L < 64 - 2 * NS && (!EDGE || i < nX);
+    const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
+    // edge segments bound the V resources to their fields: a lane whose V
+    // ComputeDisp leaves alone stores at kOob, which the hardware drops
+    const unsigned vxb = (unsigned)nY * (unsigned)sx * (unsigned)sizeof(VT),
+                   vyb = (unsigned)(nY + 1) * (unsigned)nX * (unsigned)sizeof(VT);
+    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo),
+                                 bVx = EDGE ? brsrc_n(Vxo, vxb) : brsrc(Vxo),
+                                 bVy = EDGE ? brsrc_n(Vyo, vyb) : brsrc(Vyo);
+    // slot phase ph = q & 3 known at compile time (the loop is unrolled by
+    // four): ring addresses are immediate offsets
+    auto stepK = [&](int q, auto ph) {
+      constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;  // q-1, q-2, q-3
+      const int t = jbA + q;
+      if (t == jb + lag - 1) {
+        st.start(in.ux[s2][L], in.uy[s1][L], in.m[s3][L]);
+      } else if (t >= jb + lag && t <= tLast) {
+        const int j = t - lag;
+        const double uxn = in.ux[s2][L], uynn = in.uy[s1][L];
+        const unsigned mn = in.m[s3][L];
+        const CT vxi = in.vx[s3][L], vyi = in.vy[s3][L];
+        const double uxsK = st.uxs_c;  // Ux(i+1, j) of this stage's input, before the step
+        const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
+        if constexpr (!last) {
+          out.ux[ph][L] = o.ux;
+          out.uy[ph][L] = o.uy;
+          out.vx[ph][L] = o.vx;
+          out.vy[ph][L] = o.vy;
+          out.m[ph][L] = mn;
+        } else if (j >= j0 && outLane) {
+          const unsigned ovx = EDGE && !o.updX ? kOob : oV, ovy = EDGE && !o.updY ? kOob : oV;
+          bst(bVx, ovx, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
+          bst(bVy, ovy, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
+          bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
+          bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
+          if (EDGE) {
+            // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+            if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK);
+            if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
+          }
+        }
+      }
+      lds_barrier();
+    };
+    for (int q = 0; q < nSteps; q += 4) {
+      stepK(q, std::integral_constant<int, 0>{});
+      if (q + 1 >= nSteps) break;
+      stepK(q + 1, std::integral_constant<int, 1>{});
+      if (q + 2 >= nSteps) break;
+      stepK(q + 2, std::integral_constant<int, 2>{});
+      if (q + 3 >= nSteps) break;
+      stepK(q + 3, std::integral_constant<int, 3>{});
+    }
+  };
+  static_assert(NS == 3 || NS == 4, "chains of three or four stages");
+  if (role == 1) {
+    march(std::integral_constant<int, 1>{});
+  } else if (role == 2) {
+    march(std::integral_constant<int, 2>{});
+  } else {
+    if constexpr (NS == 4) march(std::integral_constant<int, 3>{});
+  }
+}
+
+// One segment per workgroup of NS waves (stage k on wave k).
+template <typename CT, typename VT, int NS>
+__global__ __launch_bounds__(64 * NS) void k_chain(const RveDev* __restrict__ rves,
+                                                   const int4* __restrict__ segs, int nsegs,
+                                                   int parity, int vpar) {
+  __shared__ CT gt[NS][16];
+  __shared__ PairRing<CT> ring[NS - 1];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int k = tile_of_block(blockIdx.x, gridDim.x);
+  if (k >= nsegs) return;  // the whole workgroup
+  const int4 sg = segs[k];
+  const RveDev& r = rves[sg.x];
+  if (lane < 16) gt[wave][lane] = (CT)r.gtab[lane];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const int cur = parity ^ r.cur;
+  const int s = sg.y, j0 = sg.z, j1 = sg.w;
+  const int nSteps = j1 - j0 + 5 * NS - 3;
+  const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1);
+  if (interior)
+    chain_run<CT, VT, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+  else
+    chain_run<CT, VT, true, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+}
+
 }  // namespace
 
 // ---- launchers --------------------------------------------------------------
@@ -2110,6 +2269,41 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
       n++;
     }
   return n;
+}
+
+int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out) {
+  if (rows < 1) rows = 1;
+  const int cols = chain_cols(ns_);
+  const int ns = (nX + cols - 1) / cols;
+  int n = 0;
+  for (int j0 = 0; j0 < nY; j0 += rows)
+    for (int k = 0; k < ns; k++) {
+      if (out) out[n] = make_int4(rve, cols * k - 2 * ns_, j0, std::min(j0 + rows, nY));
+      n++;
+    }
+  return n;
+}
+
+void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
+                  int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (nsegs <= 0) return;
+  const dim3 grid(nsegs), block(64 * ns);
+  auto go = [&](auto kern) {
+    if (e0)
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
+    else
+      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
+  };
+  if (ns == 3) {
+    if (mode == 2) go(k_chain<float, float, 3>);  // split: FP32 stress and velocity
+    else go(k_chain<double, double, 3>);
+  } else if (ns == 4) {
+    if (mode == 2) go(k_chain<float, float, 4>);
+    else go(k_chain<double, double, 4>);
+  } else {
+    throw std::invalid_argument("chain kernels run 3 or 4 iterations per launch");
+  }
+  EP2D_HIP(hipGetLastError());
 }
 
 void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,

@@ -276,9 +276,13 @@ typedef struct ep2d_batch_options {
   int persistRows;       /* fixed persistent tile kernel of this height        */
   int march2Rows;        /* fixed two-iteration marching kernel, segment rows  */
   int pairRows;          /* fixed two-iteration marching on wave pairs, rows   */
-                         /* (both: requests up to nX = 16382)                   */
+  int tripleRows;        /* fixed three-iteration marching on chains of three
+                            waves, segment rows                                */
+  int quadRows;          /* fixed four-iteration marching on chains of four
+                            waves, segment rows                                */
+                         /* (the last four: requests up to nX = 16382)          */
                          /* all 0: kernel and shape tuned on the running batch  */
-                         /* (two-iteration shapes while every RVE has nX <= 16382) */
+                         /* (multi-iteration shapes while every RVE has nX <= 16382) */
   int device;
   size_t memoryBudget;   /* bytes of device memory for RVE state (0: 90% of free) */
   int maxActive;         /* cap on admitted RVEs (0: none)                     */
@@ -327,7 +331,9 @@ typedef struct ep2d_batch_stats {
   int kernel;                     /* kernel in use: 1 tiles, 2 row marching,
                                      3 persistent tiles, 4 row marching with
                                      two iterations per launch, 5 the same on
-                                     pairs of waves (one iteration each)       */
+                                     pairs of waves (one iteration each), 6
+                                     three iterations per launch on chains of
+                                     three waves, 7 four on chains of four     */
   int kernelRows;                 /* its tile height or segment rows          */
   int tunings;                    /* kernel tunings run                       */
   int nCand;                      /* kernel shapes the tuner compares        */

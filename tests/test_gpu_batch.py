@@ -117,11 +117,12 @@ def test_tuned_kernel_equals_fixed():
     results."""
     _note("tuned vs fixed kernel shapes")
     tuned, st = _batch(REQS, "split")
-    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4, 5)
+    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4, 5, 6, 7)
     assert all(u > 0 for u in st["candUs"])
     for kw in ({"tileRows": 6}, {"tileRows": 30}, {"marchRows": 16}, {"marchRows": 128},
                {"persistRows": 6}, {"persistRows": 14}, {"march2Rows": 8}, {"march2Rows": 48},
-               {"pairRows": 8}, {"pairRows": 40}):
+               {"pairRows": 8}, {"pairRows": 40}, {"tripleRows": 8}, {"tripleRows": 42},
+               {"quadRows": 8}, {"quadRows": 64}):
         fixed, st2 = _batch(REQS, "split", **kw)
         assert st2["tunings"] == 0
         assert st2["kernelRows"] == list(kw.values())[0]
@@ -177,14 +178,15 @@ def test_persistent_kernel_tiny_batch():
 
 @pytest.mark.parametrize("mode", ["fp64", "split"])
 @pytest.mark.parametrize("graphs", [True, False])
-@pytest.mark.parametrize("shape", ["march2Rows", "pairRows"])
+@pytest.mark.parametrize("shape", ["march2Rows", "pairRows", "tripleRows", "quadRows"])
 def test_march2_batch_equals_single(mode, graphs, shape):
-    """Two-iteration launches in the engine (pairs, then one or two single
-    iterations per chunk; odd check intervals included), on one wave or on a
-    pair of waves per segment, give the single-RVE results."""
+    """Multi-iteration launches in the engine (two iterations per launch on one
+    wave or on a pair of waves per segment, three or four on a chain of waves;
+    then one to four single iterations per chunk; odd check intervals
+    included) give the single-RVE results."""
     _note(f"{shape} batch {mode} graphs={graphs}")
     got, st = _batch(REQS, mode, useGraphs=graphs, timingEvery=1, **{shape: 32})
-    assert st["kernel"] == (4 if shape == "march2Rows" else 5)
+    assert st["kernel"] == {"march2Rows": 4, "pairRows": 5, "tripleRows": 6, "quadRows": 7}[shape]
     for t, rq in enumerate(REQS):
         _same(got[t]["result"], _single(rq, mode))
     nX, nP, phi, Y, nl, load = REQS[0]
@@ -223,3 +225,27 @@ def test_adaptive_check_schedule(mode):
         assert ra["num"][k] == pytest.approx(rf["num"][k], rel=1e-4), k
     got, _ = _batch(REQS, mode, adaptiveCheck=True)
     _same(got[0]["result"], ra)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split"])
+def test_chains_full_size_equal_tiles(mode):
+    """1024^2 (interior and edge segments, the headline size): 3 x 300 plastic
+    iterations through the three- and four-stage chains give the tile
+    kernel's records and moduli bit for bit."""
+    _note(f"chains 1024 {mode}")
+    nX = 1024
+    pa = make_pa(nX, nPores=1, porosity=0.01, Y=2.5e-3)
+
+    def run(**shape):
+        with Batch(mode=mode, checkInterval=100, **shape) as b:
+            b.submit(pa, nX, nTasks=3, nIter=300, eIter=1e-300, initLoad=-3e-3, loadType=LT,
+                     tag=0)
+            b.run()
+            return b.poll()[0], b.stats()
+
+    ref, _ = run(tileRows=14)
+    for shape in ({"tripleRows": 86}, {"tripleRows": 42}, {"quadRows": 64}):
+        got, st = run(**shape)
+        assert st["kernel"] == (6 if "tripleRows" in shape else 7)
+        _same(got["result"], ref["result"])
+        assert got["result"]["totalIters"] == 900
