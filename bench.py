@@ -774,7 +774,14 @@ def main():
                      "kernel_avg_us_dispatch": head["kernel_avg_us_dispatch"],
                      "bytes_per_launch": head["bytes_per_launch"],
                      "kernel_samples": head["kernel_samples"],
-                     "kernel_shape": head["kernel_shape"]},
+                     "kernel_shape": head["kernel_shape"],
+                     "iterations_per_launch": head["iterations_per_launch"],
+                     # HBM bytes the launch moves (traffic) over its duration:
+                     # a multi-iteration launch reads and writes each field
+                     # once for all its iterations, so `achieved` (48 B per
+                     # point and iteration) can exceed the peak
+                     "traffic_GBs": (traffic / (head["kernel_avg_us"] * 1e-6) / 1e9
+                                     if traffic else None)},
         "cell_iter_per_s": head["cell_iter_per_s"],
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],
