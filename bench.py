@@ -487,6 +487,27 @@ def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0,
         f"{ms.value:.3f} ms ({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
 
 
+SHAPE_OPTION = {"tiles": "tileRows", "march": "marchRows", "persist": "persistRows",
+                "march2x": "march2Rows", "pair": "pairRows", "triple": "tripleRows",
+                "quad": "quadRows"}
+
+
+def probe_shapes(mode, nx, iters, batch, shapes):
+    """Kernel-trace workload: for each shape name (as shape_name gives it),
+    `iters` iterations of the batched engine over `batch` headline RVEs with
+    that shape fixed, launched one by one (no graphs), so a trace averages
+    each shape's launch duration by its grid size."""
+    pa, _ = make_pa(nx)
+    for name in shapes:
+        kind = name.rstrip("0123456789")
+        rows = int(name[len(kind):])
+        reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
+                for k in range(batch)]
+        run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False,
+                  **{SHAPE_OPTION[kind]: rows})
+        log(f"[probe] {mode} {nx} batch {batch}: {name}, {iters} iterations")
+
+
 def sweep(sizes, modes, shapes, iters):
     """Average fused-kernel duration per (scheme, kernel shape, grid size) on
     the preloaded single RVE, every launch timed with dispatch events."""
@@ -662,6 +683,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--probe", action="store_true")
+    ap.add_argument("--probe-shapes", default="",
+                    help="kernel trace workload: the batched probe once per listed shape "
+                         "(e.g. pair26,triple42,quad64)")
     ap.add_argument("--probe-mode", default="fp64", choices=["fp64", "split", "plain"])
     ap.add_argument("--probe-iters", type=int, default=200)
     ap.add_argument("--probe-batch", type=int, default=0,
@@ -700,6 +724,10 @@ def main():
               "a multi-GPU run would shard independent RVEs across devices and is not "
               "part of this build", file=sys.stderr)
         return 2
+    if a.probe_shapes:
+        probe_shapes(a.probe_mode, a.nx, a.probe_iters, a.probe_batch or 8,
+                     a.probe_shapes.split(","))
+        return 0
     if a.probe:
         probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
               a.probe_march, a.probe_march2, a.probe_pair, a.probe_triple, a.probe_quad)
