@@ -1819,8 +1819,11 @@ __global__ __launch_bounds__(256) void k_march2(const RveDev* __restrict__ rves,
 // workgroup advances one row per barrier.  A wave holds one stage's state
 // instead of two, so about twice as many waves fit on a SIMD to hide the
 // memory latency.  Segments, arithmetic and outputs are those of k_march2.
+//
+// RowRing: the four-row LDS ring through which a stage hands its output rows
+// to the next (also the ring of the wave chains below).
 template <typename CT>
-struct PairRing {
+struct RowRing {
   double ux[4][64];
   double uy[4][64];
   CT vx[4][64];
@@ -1836,7 +1839,7 @@ struct PairRing {
 template <typename CT, typename VT, bool EDGE>
 __device__ __forceinline__ void march2p_run(const RveDev& r, int cur, int vpar, int s, int j0,
                                             int j1, const CT* gt, bool stageB,
-                                            PairRing<CT>& ring, int nSteps) {
+                                            RowRing<CT>& ring, int nSteps) {
   using M = March<CT, VT>;
   using Ld = typename M::Ld;
   const int nX = r.nX, nY = r.nY, sx = nX + 1;
@@ -1932,7 +1935,7 @@ __global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves
                                                  const int4* __restrict__ segs, int nsegs,
                                                  int parity, int vpar) {
   __shared__ CT gt[4][16];
-  __shared__ PairRing<CT> ring[2];
+  __shared__ RowRing<CT> ring[2];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int b = tile_of_block(blockIdx.x, gridDim.x);
@@ -1983,7 +1986,7 @@ __global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves
 // a slot, and the barrier after each step orders a write before its reads.
 template <typename CT, typename VT, bool EDGE, int NS>
 __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, int s, int j0,
-                                          int j1, const CT* gt, int role, PairRing<CT>* ring,
+                                          int j1, const CT* gt, int role, RowRing<CT>* ring,
                                           int nSteps) {
   using M = March<CT, VT>;
   using Ld = typename M::Ld;
@@ -1995,7 +1998,7 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   st.setup(r, gt, i);
   if (role == 0) {
     // stage 0: rows jbA..tEnd from HBM into ring 0 (slot q & 3)
-    PairRing<CT>& out = ring[0];
+    RowRing<CT>& out = ring[0];
     const StageInput<CT, VT, EDGE> in(r, cur, vpar, i, tEnd);
     in.start(st, jbA);
     auto fetch = [&](int t) { return in.fetch(t); };
@@ -2028,8 +2031,8 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
     constexpr int k = decltype(K)::value, lag = 3 * k;
     constexpr bool last = k == NS - 1;
     const int jb = j0 - 2 * (NS - k), tLast = tEnd + k;
-    PairRing<CT>& in = ring[k - 1];
-    PairRing<CT>& out = ring[last ? k - 1 : k];  // not written by the last stage
+    RowRing<CT>& in = ring[k - 1];
+    RowRing<CT>& out = ring[last ? k - 1 : k];  // not written by the last stage
     double* __restrict__ Uxo = r.Ux[cur ^ 1];
     double* __restrict__ Uyo = r.Uy[cur ^ 1];
     VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
@@ -2104,7 +2107,7 @@ __global__ __launch_bounds__(64 * NS) void k_chain(const RveDev* __restrict__ rv
                                                    const int4* __restrict__ segs, int nsegs,
                                                    int parity, int vpar) {
   __shared__ CT gt[NS][16];
-  __shared__ PairRing<CT> ring[NS - 1];
+  __shared__ RowRing<CT> ring[NS - 1];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int k = tile_of_block(blockIdx.x, gridDim.x);
