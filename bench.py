@@ -629,9 +629,9 @@ def run_config3(check, nIter=100_000):
     out = {}
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
+        # automatic single-RVE shape: three-wave chains (split), wave pairs (FP64)
         with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
-                        mode=mode, shearLoadType=LOAD["shearLoadType"], marchRows=24,
-                        itersPerLaunch=2) as g:
+                        mode=mode, shearLoadType=LOAD["shearLoadType"]) as g:
             g.set_timing(64)
             res = g.ComputeEffModuli(LOAD["initLoad"], LOAD["loadType"])
             ms, n, name = g.timing()

@@ -146,8 +146,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   // the fused kernels address a field with 32-bit byte offsets
   if ((long)(nX + 1) * (long)(nY + 1) > (1L << 29))
     throw ArgError("grid too large: every field must stay under 4 GiB (nX, nY <= 23169)");
-  if (opt_.itersPerLaunch == 2 && !two_iter_fits(nX, nY))
-    throw ArgError("itersPerLaunch 2 needs every field under 2 GiB (nX, nY <= 16382)");
+  if (opt_.itersPerLaunch >= 2 && !two_iter_fits(nX, nY))
+    throw ArgError("itersPerLaunch 2 to 4 needs every field under 2 GiB (nX, nY <= 16382)");
   if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");
   if (opt_.nTasks < 1) throw ArgError("nTasks must be >= 1");
   if (opt_.nIter < 1) throw ArgError("nIter must be >= 1");
@@ -267,13 +267,17 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
   buildGatherTable();
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
-    // automatic shape for one RVE: two iterations per launch on wave pairs
-    // where that measured faster than the tile kernel (split from 1024^2,
-    // FP64 from 2048^2), with segments short enough to give the GPU waves
+    // automatic shape for one RVE, from one-RVE measurements: split from
+    // 1024^2 on chains of waves (four iterations per launch and 16-row
+    // segments at 1024^2, 32 at 2048^2; three and 56 rows beyond), FP64 from
+    // 2048^2 on wave pairs (26 rows); the tile kernel below those sizes.
+    // Segments stay short enough to give the GPU waves.
     const int n = std::max(nX, nY);
-    if (((mode_ == EP2D_MODE_SPLIT && n >= 1024) || (mode_ == EP2D_MODE_FP64 && n >= 2048)) &&
-        two_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 8 : (n <= 2048 ? 16 : 26);
+    if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && two_iter_fits(nX, nY)) {
+      opt_.marchRows = n <= 1024 ? 16 : (n <= 2048 ? 32 : 56);
+      opt_.itersPerLaunch = n <= 2048 ? 4 : 3;
+    } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && two_iter_fits(nX, nY)) {
+      opt_.marchRows = 26;
       opt_.itersPerLaunch = 2;
     }
   }
@@ -283,11 +287,12 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (ty == 0) ty = default_tile_rows(mode_, nX);
     fplan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
     if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
-    if (opt_.itersPerLaunch < 0 || opt_.itersPerLaunch > 2)
-      throw ArgError("itersPerLaunch must be 1 or 2");
-    if (opt_.itersPerLaunch == 2 && opt_.marchRows <= 0)
-      throw ArgError("itersPerLaunch 2 needs the row-marching kernel (marchRows > 0)");
-    twoIter_ = opt_.itersPerLaunch == 2;
+    if (opt_.itersPerLaunch < 0 || opt_.itersPerLaunch > 4)
+      throw ArgError("itersPerLaunch must be 1, 2, 3 or 4");
+    if (opt_.itersPerLaunch >= 2 && opt_.marchRows <= 0)
+      throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
+    ipl_ = opt_.itersPerLaunch >= 2 ? opt_.itersPerLaunch : 1;
+    twoIter_ = ipl_ > 1;
     if (opt_.marchRows > 0) {
       // row-marching kernel: the parameter table and the segment list live
       // on the device (the kernel reads the current buffer as parity ^ cur)
@@ -303,9 +308,11 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       EP2D_HIP(hipMemcpyAsync(dSegs_, segs.data(), segs.size() * sizeof(int4),
                               hipMemcpyHostToDevice, stream_));
       if (twoIter_) {
-        nSegs2_ = march2_segments(nX, nY, marchRows_, 0, nullptr);
+        nSegs2_ = ipl_ == 2 ? march2_segments(nX, nY, marchRows_, 0, nullptr)
+                            : chain_segments(ipl_, nX, nY, marchRows_, 0, nullptr);
         std::vector<int4> s2((size_t)nSegs2_);
-        march2_segments(nX, nY, marchRows_, 0, s2.data());
+        if (ipl_ == 2) march2_segments(nX, nY, marchRows_, 0, s2.data());
+        else chain_segments(ipl_, nX, nY, marchRows_, 0, s2.data());
         dSegs2_ = (int4*)alloc(s2.size() * sizeof(int4));
         EP2D_HIP(hipMemcpyAsync(dSegs2_, s2.data(), s2.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, stream_));
@@ -477,20 +484,25 @@ std::pair<hipEvent_t, hipEvent_t> Solver::takeTimingEvents() {
   return ev;
 }
 
-// Two iterations in one launch (each segment on a pair of waves, one iteration
-// each); V is read from buffer vIdx_ ^ (q & 1) and written to the other, so
-// after an odd number of double launches the other buffer is current (flipV).
+// ipl_ iterations in one launch (each segment on a pair of waves or a chain
+// of ipl_ waves, one iteration each); V is read from buffer vIdx_ ^ (q & 1)
+// and written to the other, so after an odd number of such launches the
+// other buffer is current (flipV).
 void Solver::launchDouble(int q) {
   const bool timed = sampleEvery_ > 0 && (launchCount_ % sampleEvery_) == 0;
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) ev = takeTimingEvents();
-  launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
-                timed ? ev.first : nullptr, timed ? ev.second : nullptr, true);
+  if (ipl_ == 2)
+    launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
+                  timed ? ev.first : nullptr, timed ? ev.second : nullptr, true);
+  else
+    launch_chain(ipl_, dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
+                 timed ? ev.first : nullptr, timed ? ev.second : nullptr);
   cur_ ^= 1;
   stressStale_ = true;
   if (timed) {
     pending_.push_back(ev);
-    pendingIters_.push_back(2);
+    pendingIters_.push_back(ipl_);
     if (pending_.size() >= 256) resolveTiming(false);
   }
   launchCount_++;
@@ -508,11 +520,12 @@ void Solver::iterate(long n) {
     for (long k = 0; k < n; k++) launchIteration();
     return;
   }
-  // pairs first, then one or two single iterations (in place), so the last
-  // launch reads U^{n-1}, the state the stress fields are recomputed from
+  // multi-iteration launches first, then 1..ipl_ single iterations (in
+  // place), so the last launch reads U^{n-1}, the state the stress fields are
+  // recomputed from
   if (n <= 0) return;
-  const long singles = (n & 1) ? 1 : 2;
-  const long d = (n - singles) / 2;
+  const long singles = (n - 1) % ipl_ + 1;
+  const long d = (n - singles) / ipl_;
   for (long q = 0; q < d; q++) launchDouble((int)(q & 1));
   if (d & 1) flipV();
   for (long q = 0; q < singles; q++) launchIteration();
@@ -550,7 +563,11 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
     if (marchRows_ > 0 && twoIter_)
-      nm = std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>" : "k_march2p<double, double>") +
+      nm = (ipl_ == 2 ? std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>"
+                                                              : "k_march2p<double, double>")
+                      : std::string(mode_ == EP2D_MODE_SPLIT ? "k_chain<float, float, "
+                                                              : "k_chain<double, double, ") +
+                            std::to_string(ipl_) + ">") +
            " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs2_) +
            " (per iteration)";
     else if (marchRows_ > 0)

@@ -103,10 +103,11 @@ typedef struct ep2d_options {
                            (EffPlast2D.log, :1607) */
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
   int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
-  int itersPerLaunch;   /* 2: with marchRows, two iterations per launch on wave pairs
-                           (nX, nY <= 16382: fields under 2 GiB); 1: one; 0
-                           (default) with tileRows = marchRows = 0: the shape is
-                           chosen by grid size and scheme (same results) */
+  int itersPerLaunch;   /* with marchRows: 2 iterations per launch on wave pairs,
+                           3 or 4 on chains of 3 or 4 waves (nX, nY <= 16382:
+                           fields under 2 GiB); 1: one; 0 (default) with
+                           tileRows = marchRows = 0: the shape is chosen by grid
+                           size and scheme (same results) */
   int adaptiveCheck;    /* extension, 0: check every checkInterval iterations;
                            1: first check after checkInterval, then where the
                            residual's decay rate over the last two checks

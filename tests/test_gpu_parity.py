@@ -145,6 +145,30 @@ def test_march2_matches_oracle(rows, iters, nX, nPores):
             assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
 
 
+@pytest.mark.parametrize("ipl", [3, 4])
+@pytest.mark.parametrize("rows,iters", [(1, 7), (5, 13), (16, 3), (64, 10)])
+@pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (8, 1)])
+def test_chain_matches_oracle(ipl, rows, iters, nX, nPores):
+    """Three or four iterations per launch on a chain of waves, FP64: every
+    remainder of the iteration count (the chain launches, then one to four
+    single iterations), short and long segments, bit for bit against the
+    oracle."""
+    nl = 3
+    _note(f"chain{ipl} fp64 rows={rows} iters={iters} nX={nX}")
+    pa = make_pa(nX, nPores=nPores, porosity=0.03)
+    init = random_state(nX, 900 + rows + nX + ipl, pa=pa, boundary_v=True)
+    s = oracle_state(nX, pa, nl, init)
+    s.iterate(iters)
+    with EffPlast2D(pa, nX, nTasks=nl, mode="fp64", marchRows=rows, itersPerLaunch=ipl) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(iters)
+        for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS:
+            got = g.get(k)
+            e = rel_err(got, s[k])
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
+
+
 @pytest.mark.parametrize("nl", [1, 3])
 def test_march2_split_equals_tiles(nl):
     nX, iters = 190, 21
@@ -315,8 +339,10 @@ def test_split_moduli_close_to_fp64_oracle(case):
         assert l2_rel(fields[k], s[k]) <= 1e-3, k
 
 
-@pytest.mark.parametrize("kw", [{"marchRows": 26, "itersPerLaunch": 2}, {"tileRows": 8}],
-                         ids=["pairs26", "tiles8"])
+@pytest.mark.parametrize("kw", [{"marchRows": 26, "itersPerLaunch": 2}, {"tileRows": 8},
+                                {"marchRows": 56, "itersPerLaunch": 3},
+                                {"marchRows": 16, "itersPerLaunch": 4}],
+                         ids=["pairs26", "tiles8", "chain3x56", "chain4x16"])
 def test_full_size_fp64_matches_oracle(kw):
     """The headline's RVE at full size (1024^2, one pore, porosity 0.01,
     Y = 2.5e-3, G0 = 0.5, three tasks): from a noisy affine compression, 101
@@ -343,19 +369,22 @@ def test_full_size_fp64_matches_oracle(kw):
 
 
 def test_full_size_split_pairs_equal_tiles():
-    """Split scheme at 1024^2: the two-iteration kernel on wave pairs gives the
-    tile kernel's fields bit for bit (201 iterations)."""
+    """Split scheme at 1024^2: the two-iteration kernel on wave pairs and the
+    three- and four-iteration chains give the tile kernel's fields bit for bit
+    (201 iterations)."""
     nX, iters = 1024, 201
-    _note(f"full size split pairs vs tiles iters={iters}")
+    _note(f"full size split pairs and chains vs tiles iters={iters}")
     pa = make_pa(nX, nPores=1, porosity=0.01)
     init = random_state(nX, 4243, pa=pa, boundary_v=True)
     out = []
-    for kw in ({"marchRows": 26, "itersPerLaunch": 2}, {"tileRows": 8}):
+    for kw in ({"tileRows": 8}, {"marchRows": 26, "itersPerLaunch": 2},
+               {"marchRows": 56, "itersPerLaunch": 3}, {"marchRows": 16, "itersPerLaunch": 4}):
         with EffPlast2D(pa, nX, nTasks=3, mode="split", **kw) as g:
             for k in STATE_FIELDS:
                 g.set(k, init[k])
             g.iterate(iters)
             out.append({k: g.get(k) for k in STATE_FIELDS + STRESS_FIELDS})
-    for k in STATE_FIELDS + STRESS_FIELDS:
-        assert np.array_equal(out[0][k], out[1][k]), \
-            f"{k}: {np.count_nonzero(out[0][k] != out[1][k])} differ"
+    for o in out[1:]:
+        for k in STATE_FIELDS + STRESS_FIELDS:
+            assert np.array_equal(o[k], out[0][k]), \
+                f"{k}: {np.count_nonzero(o[k] != out[0][k])} differ"

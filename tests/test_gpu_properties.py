@@ -1,6 +1,5 @@
 """GPU properties that fix the result without an oracle run, at sizes up to
-the headline's 1024^2, for every scheme and for the tile, marching and
-wave-pair kernels:
+the headline's 1024^2, for every scheme:
 
   * zero state stays zero (no load, no motion);
   * exact linearity of the elastic scheme: doubling U and V doubles every
@@ -8,7 +7,9 @@ wave-pair kernels:
   * mirror symmetry of a centred pore under an isotropic load (x -> -x, and
     y -> -y), and the x <-> y transpose (Ux(i, j) <-> Uy(j, i));
   * the yield bound after the return mapping: J2 <= coh (1 + 4 eps);
-  * a long run (1000 iterations) bit for bit against the oracle.
+  * a long run (1000 iterations) bit for bit against the oracle,
+
+for the tile, marching, wave-pair and wave-chain kernels.
 """
 import os
 import sys
@@ -24,7 +25,8 @@ from tests.pairs import (PLASTIC_FIELDS, STATE_FIELDS, STRESS_FIELDS, make_pa, o
 pytestmark = pytest.mark.gpu
 
 SHAPES = {"tiles8": {"tileRows": 8}, "march16": {"marchRows": 16},
-          "pairs26": {"marchRows": 26, "itersPerLaunch": 2}}
+          "pairs26": {"marchRows": 26, "itersPerLaunch": 2},
+          "chain3": {"marchRows": 42, "itersPerLaunch": 3}}
 # the plain scheme has one kernel set; the fused schemes run every shape
 MODE_SHAPES = [("plain", "tiles8")] + [(m, sh) for m in ("fp64", "split") for sh in SHAPES]
 
@@ -133,7 +135,7 @@ def test_yield_bound_after_return_mapping(mode, shape):
     assert np.count_nonzero(out["J2"] >= coh * (1 - 1e-9)) > 0, "nothing yielded"
 
 
-@pytest.mark.parametrize("shape", ["tiles8", "march16", "pairs26"])
+@pytest.mark.parametrize("shape", ["tiles8", "march16", "pairs26", "chain3"])
 def test_long_run_matches_oracle(shape):
     """1000 plastic iterations at 128^2 (three pores), FP64, against the
     oracle at <= 1e-12 relative (bit for bit in practice)."""
