@@ -61,7 +61,7 @@ def test_zero_state_stays_zero(mode, shape):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "split"])
-@pytest.mark.parametrize("shape", ["tiles8", "pairs26"])
+@pytest.mark.parametrize("shape", ["tiles8", "pairs26", "chain3"])
 def test_elastic_linearity_exact(mode, shape):
     """Elastic (one task), 1024^2: doubling the state doubles every field."""
     nX, iters = 1024, 41
@@ -91,7 +91,7 @@ def _centred_affine_state(pa, nX, e):
 
 
 @pytest.mark.parametrize("mode", ["fp64", "split"])
-@pytest.mark.parametrize("shape", ["tiles8", "pairs26"])
+@pytest.mark.parametrize("shape", ["tiles8", "pairs26", "chain3"])
 @pytest.mark.parametrize("nX", [128, 1024])
 def test_centred_pore_symmetries(mode, shape, nX):
     """One centred pore, isotropic compression (plastic, three tasks): P is
