@@ -105,7 +105,7 @@ long long Batch::submit(const ep2d_request& rq) {
   if (rq.nTasks < 1 || rq.nIter < 1 || rq.nTimeSteps < 1)
     throw ArgError("request: nTasks, nIter and nTimeSteps must be >= 1");
   if (rq.checkInterval < 0) throw ArgError("request: checkInterval must be >= 0");
-  if (fixed_ && twoIter(shape_.kind) && !two_iter_fits(rq.nX, rq.nX))
+  if (fixed_ && multiIter(shape_.kind) && !multi_iter_fits(rq.nX, rq.nX))
     throw ArgError(
         "request: march2Rows, pairRows, tripleRows and quadRows need every field under 2 GiB "
         "(nX <= 16382)");
@@ -254,7 +254,7 @@ int Batch::buildWork(Shape sh, int4* out) const {
       t += chain_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
-    if (twoIter(sh.kind)) {
+    if (multiIter(sh.kind)) {
       t += march2_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
@@ -280,10 +280,10 @@ void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long 
   if (chain(sh.kind) && dbl)
     launch_chain(ipl(sh.kind), dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_,
                  e0, e1);
-  else if (twoIter(sh.kind) && dbl)
+  else if (multiIter(sh.kind) && dbl)
     launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1,
                   sh.kind == kPair);
-  else if (twoIter(sh.kind))
+  else if (multiIter(sh.kind))
     launch_march(dRves_, dTiles_ + auxOff_, auxCnt_, opt_.mode, (int)(l & 1), (int)(d & 1),
                  stream_, e0, e1);
   else if (sh.kind == kMarch)
@@ -341,7 +341,7 @@ hipGraphExec_t Batch::graphFor(long len) {
   g.lastUse = st_.chunks;
   long l = 0, d = 0;
   EP2D_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  enqueue(len, twoIter(shape_.kind), l, d);
+  enqueue(len, multiIter(shape_.kind), l, d);
   EP2D_HIP(hipStreamEndCapture(stream_, &g.graph));
   EP2D_HIP(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
   graphs_.push_back(g);
@@ -364,15 +364,15 @@ Batch::Seq Batch::launchChunk(long c) {
     const double n = sl.s->nX();
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
-    twoOk = twoOk && two_iter_fits(sl.s->nX(), sl.s->nX());
+    twoOk = twoOk && multi_iter_fits(sl.s->nX(), sl.s->nX());
   }
   if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
-  if (!fixed_ && !twoOk && twoIter(shape_.kind)) {
+  if (!fixed_ && !twoOk && multiIter(shape_.kind)) {
     shape_ = Shape{kTiles, 14};
     tuneDue_ = true;
   }
-  auto allowed = [&](int k) { return twoOk || !twoIter(kCand[k].kind); };
+  auto allowed = [&](int k) { return twoOk || !multiIter(kCand[k].kind); };
   // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
   // every candidate, as many as the chunk holds (at least one timed round)
   constexpr long kTuneEach = 4;
@@ -409,7 +409,7 @@ Batch::Seq Batch::launchChunk(long c) {
         if (!allowed(k)) continue;
         const Shape sh = kCand[k];
         const int4* w = dTiles_ + candOff_[k];
-        const bool dbl = twoIter(sh.kind);
+        const bool dbl = multiIter(sh.kind);
         Pending ev{};
         if (round > 0) {
           ev = takeEvents();
@@ -453,7 +453,7 @@ Batch::Seq Batch::launchChunk(long c) {
           for (int q = 0; q < nn; q++) {
             const int k = near[q];
             const Shape sh = kCand[k];
-            const bool dbl = twoIter(sh.kind);
+            const bool dbl = multiIter(sh.kind);
             Pending ev = takeEvents();
             ev.cand = k;
             ev.launches = kRefLaunches * ipl(sh.kind);
@@ -488,7 +488,7 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
     dropGraphs();
     const Shape aux{kMarch, shape_.rows};
     size_t need = (size_t)buildWork(shape_, nullptr);
-    if (twoIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
+    if (multiIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
     if (tuning)
       for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
     ensureCapacity(active_.size(), need);
@@ -496,7 +496,7 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
     int off = nTiles_;
     auxOff_ = off;
     auxCnt_ = 0;
-    if (twoIter(shape_.kind)) {
+    if (multiIter(shape_.kind)) {
       auxCnt_ = buildWork(aux, hTiles_ + off);
       off += auxCnt_;
     }
@@ -520,7 +520,7 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
                           bool tuning) {
   const long rest = c - it;
   const bool timed = opt_.timingEvery > 0;
-  const bool two = twoIter(shape_.kind);
+  const bool two = multiIter(shape_.kind);
   const int n = ipl(shape_.kind);
   long pairs = 0, singles = rest;
   if (two) split_multi(rest, n, pairs, singles);

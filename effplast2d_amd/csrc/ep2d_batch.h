@@ -53,7 +53,7 @@ class Batch {
     return kind == kQuad ? 4 : kind == kTriple ? 3 : (kind == kMarch2 || kind == kPair ? 2 : 1);
   }
   static bool chain(int kind) { return kind == kTriple || kind == kQuad; }
-  static bool twoIter(int kind) { return ipl(kind) > 1; }
+  static bool multiIter(int kind) { return ipl(kind) > 1; }
   struct Shape {
     int kind = kTiles;  // fused tile, row-marching, persistent tile, two-iteration
                         // marching on one wave or on a pair of waves,

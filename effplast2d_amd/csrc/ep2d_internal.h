@@ -131,7 +131,7 @@ constexpr int kMarch2Cols = 56;
 // 32-bit byte offsets, and edge lanes outside the grid read past a bound set
 // to the field's size at offset 2^31: every field must stay under 2 GiB,
 // nX, nY <= 16382 (the largest field is (nX+1)(nY+1) doubles at most).
-inline bool two_iter_fits(int nX, int nY) {
+inline bool multi_iter_fits(int nX, int nY) {
   return (long long)(nX + 1) * (long long)(nY + 1) * 8 < (1LL << 31);
 }
 int march2_segments(int nX, int nY, int rows, int rve, int4* out);

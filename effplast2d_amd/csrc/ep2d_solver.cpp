@@ -146,7 +146,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   // the fused kernels address a field with 32-bit byte offsets
   if ((long)(nX + 1) * (long)(nY + 1) > (1L << 29))
     throw ArgError("grid too large: every field must stay under 4 GiB (nX, nY <= 23169)");
-  if (opt_.itersPerLaunch >= 2 && !two_iter_fits(nX, nY))
+  if (opt_.itersPerLaunch >= 2 && !multi_iter_fits(nX, nY))
     throw ArgError("itersPerLaunch 2 to 4 needs every field under 2 GiB (nX, nY <= 16382)");
   if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");
   if (opt_.nTasks < 1) throw ArgError("nTasks must be >= 1");
@@ -273,10 +273,10 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     // 2048^2 on wave pairs (26 rows); the tile kernel below those sizes.
     // Segments stay short enough to give the GPU waves.
     const int n = std::max(nX, nY);
-    if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && two_iter_fits(nX, nY)) {
+    if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
       opt_.marchRows = n <= 1024 ? 16 : (n <= 2048 ? 32 : 56);
       opt_.itersPerLaunch = n <= 2048 ? 4 : 3;
-    } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && two_iter_fits(nX, nY)) {
+    } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {
       opt_.marchRows = 26;
       opt_.itersPerLaunch = 2;
     }
@@ -292,7 +292,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (opt_.itersPerLaunch >= 2 && opt_.marchRows <= 0)
       throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
     ipl_ = opt_.itersPerLaunch >= 2 ? opt_.itersPerLaunch : 1;
-    twoIter_ = ipl_ > 1;
+    multiIter_ = ipl_ > 1;
     if (opt_.marchRows > 0) {
       // row-marching kernel: the parameter table and the segment list live
       // on the device (the kernel reads the current buffer as parity ^ cur)
@@ -307,7 +307,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       EP2D_HIP(hipMemcpyAsync(dRve_, &rd, sizeof(RveDev), hipMemcpyHostToDevice, stream_));
       EP2D_HIP(hipMemcpyAsync(dSegs_, segs.data(), segs.size() * sizeof(int4),
                               hipMemcpyHostToDevice, stream_));
-      if (twoIter_) {
+      if (multiIter_) {
         nSegs2_ = ipl_ == 2 ? march2_segments(nX, nY, marchRows_, 0, nullptr)
                             : chain_segments(ipl_, nX, nY, marchRows_, 0, nullptr);
         std::vector<int4> s2((size_t)nSegs2_);
@@ -488,7 +488,7 @@ std::pair<hipEvent_t, hipEvent_t> Solver::takeTimingEvents() {
 // of ipl_ waves, one iteration each); V is read from buffer vIdx_ ^ (q & 1)
 // and written to the other, so after an odd number of such launches the
 // other buffer is current (flipV).
-void Solver::launchDouble(int q) {
+void Solver::launchMulti(int q) {
   const bool timed = sampleEvery_ > 0 && (launchCount_ % sampleEvery_) == 0;
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) ev = takeTimingEvents();
@@ -516,7 +516,7 @@ void Solver::flipV() {
 
 void Solver::iterate(long n) {
   EP2D_HIP(hipSetDevice(device_));
-  if (!twoIter_) {
+  if (!multiIter_) {
     for (long k = 0; k < n; k++) launchIteration();
     return;
   }
@@ -526,7 +526,7 @@ void Solver::iterate(long n) {
   if (n <= 0) return;
   const long singles = (n - 1) % ipl_ + 1;
   const long d = (n - singles) / ipl_;
-  for (long q = 0; q < d; q++) launchDouble((int)(q & 1));
+  for (long q = 0; q < d; q++) launchMulti((int)(q & 1));
   if (d & 1) flipV();
   for (long q = 0; q < singles; q++) launchIteration();
 }
@@ -562,7 +562,7 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   if (name) {
     static thread_local std::string nm;
     nm = fused_kernel_name(mode_);
-    if (marchRows_ > 0 && twoIter_)
+    if (marchRows_ > 0 && multiIter_)
       nm = (ipl_ == 2 ? std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>"
                                                               : "k_march2p<double, double>")
                       : std::string(mode_ == EP2D_MODE_SPLIT ? "k_chain<float, float, "
