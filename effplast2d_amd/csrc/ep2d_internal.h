@@ -127,7 +127,7 @@ void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int
 // vpar = 0 (else Vx2/Vy2) and written to the other pair.  pairs: each segment
 // on two waves (one per iteration, rows handed over in LDS), else on one.
 constexpr int kMarch2Cols = 56;
-// The two-iteration kernels address fields through buffer resources with
+// The multi-iteration kernels (pairs, chains) address fields through buffer resources with
 // 32-bit byte offsets, and edge lanes outside the grid read past a bound set
 // to the field's size at offset 2^31: every field must stay under 2 GiB,
 // nX, nY <= 16382 (the largest field is (nX+1)(nY+1) doubles at most).

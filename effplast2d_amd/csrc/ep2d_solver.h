@@ -154,7 +154,7 @@ class Solver {
   int marchRows_ = 0, nSegs_ = 0;   // row-marching kernel when marchRows_ > 0
   RveDev* dRve_ = nullptr;          // device copy of r_ with Vx/Vy = vbuf 0, Vx2/Vy2 = vbuf 1
   int4* dSegs_ = nullptr;
-  bool multiIter_ = false;            // more than one iteration per launch
+  bool multiIter_ = false;          // more than one iteration per launch
   int ipl_ = 1;                     // iterations per launch: 1, 2 (pairs), 3 or 4 (chains)
   int nSegs2_ = 0;
   int4* dSegs2_ = nullptr;
