@@ -23,9 +23,9 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
 // Kernel shapes the tuner times: the fused tile kernel at three tile
-// heights, the row-marching kernel at two segment lengths, the persistent
-// tile kernel at three tile heights and the two-iteration marching kernel at
-// two segment lengths.
+// heights, the row-marching and persistent tile kernels at one each, the
+// two-iteration marching kernel on one wave at one segment length and on wave
+// pairs at nine, and the chains of three and four waves at four and three.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
     {kTiles, 8},  {kTiles, 14}, {kTiles, 30}, {kMarch, 32}, {kPersist, 14},
     {kMarch2, 40}, {kPair, 8},   {kPair, 14},  {kPair, 20},  {kPair, 24},
@@ -239,8 +239,8 @@ void Batch::dropGraphs() {
 // Work list of every admitted RVE for a kernel shape.  Tiles (fused tile
 // kernels): ceil(nX/62) columns of 62 outputs by ceil(nY/rows) rows, entries
 // (RVE index, i0, j0, unused).  Segments (marching kernels): strips of 60
-// (one iteration) or 56 (two iterations) output columns by runs of `rows`
-// rows, entries (RVE index, s, j0, j1).  The RVE's current buffers are in its
+// (one iteration), 56 (two), 52 (three) or 48 (four) output columns by runs
+// of `rows` rows, entries (RVE index, s, j0, j1).  The RVE's current buffers are in its
 // table entry, so the list stays valid while the RVEs iterate.
 int Batch::buildWork(Shape sh, int4* out) const {
   int t = 0;
@@ -319,9 +319,9 @@ void Batch::enqueue(long n, bool dbl, long& l, long& d) {
   }
 }
 
-// Graph of `len` launches of the current shape (pairs for the two-iteration
-// shape) from launch and double index 0 over the current tables, captured
-// once per length while the tables stay the same.
+// Graph of `len` launches of the current shape (multi-iteration launches for
+// a multi-iteration shape) from launch and double index 0 over the current
+// tables, captured once per length while the tables stay the same.
 hipGraphExec_t Batch::graphFor(long len) {
   for (auto& g : graphs_)
     if (g.len == len) {
@@ -349,30 +349,31 @@ hipGraphExec_t Batch::graphFor(long len) {
 }
 
 // c iterations of every admitted RVE.  Launch l reads U buffer cur_k ^ (l&1)
-// of RVE k (cur_k in its table entry); two-iteration launch number d reads V
-// buffer d&1 (the table's Vx/Vy are the current ones) and writes the other.
+// of RVE k (cur_k in its table entry); multi-iteration launch number d reads
+// V buffer d&1 (the table's Vx/Vy are the current ones) and writes the other.
 // The shape does not change any result, so a tuning chunk spends its first
-// launches timing each candidate (two-iteration candidates in pairs, so V is
-// back in its buffer before the next candidate).  Returns the launch counts.
+// launches timing each candidate (multi-iteration candidates in pairs of
+// launches, so V is back in its buffer before the next candidate).  Returns
+// the launch counts.
 Batch::Seq Batch::launchChunk(long c) {
   std::vector<long long> ids;
   ids.reserve(active_.size() + 3);
   double bytes = 0.0, cells = 0.0;
-  bool twoOk = true;  // every field under the two-iteration kernels' bound
+  bool multiOk = true;  // every field under the multi-iteration kernels' bound
   for (auto& sl : active_) {
     ids.push_back(sl.id);
     const double n = sl.s->nX();
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
-    twoOk = twoOk && multi_iter_fits(sl.s->nX(), sl.s->nX());
+    multiOk = multiOk && multi_iter_fits(sl.s->nX(), sl.s->nX());
   }
   if (!fixed_ && !tuneDue_ && (cells > 1.25 * tunedCells_ || cells < 0.75 * tunedCells_))
     tuneDue_ = true;
-  if (!fixed_ && !twoOk && multiIter(shape_.kind)) {
+  if (!fixed_ && !multiOk && multiIter(shape_.kind)) {
     shape_ = Shape{kTiles, 14};
     tuneDue_ = true;
   }
-  auto allowed = [&](int k) { return twoOk || !multiIter(kCand[k].kind); };
+  auto allowed = [&](int k) { return multiOk || !multiIter(kCand[k].kind); };
   // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
   // every candidate, as many as the chunk holds (at least one timed round)
   constexpr long kTuneEach = 4;
@@ -402,8 +403,8 @@ Batch::Seq Batch::launchChunk(long c) {
   if (tuning) {
     // a warm-up round, then the timed rounds, candidates interleaved; a
     // candidate's two launches of a round run back to back between one pair
-    // of events (two-iteration candidates in pairs, so V is back in its
-    // buffer before the next candidate)
+    // of events (so a multi-iteration candidate leaves V back in its buffer
+    // before the next candidate)
     for (long round = 0; round < rounds; round++)
       for (int k = 0; k < kNCand; k++) {
         if (!allowed(k)) continue;
@@ -481,7 +482,7 @@ Batch::Seq Batch::launchChunk(long c) {
 }
 
 // Work lists of the admitted RVEs for the current shape (and its single-
-// iteration list for a two-iteration shape, and every candidate's list when
+// iteration list for a multi-iteration shape, and every candidate's list when
 // tuning), uploaded when the RVE set, the tuning state or the shape changed.
 void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
   if (ids != key_) {
@@ -515,7 +516,7 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
 
 // The chunk's iterations after the `it` tuning ones, from launch index l and
 // double index d: the main launches (a graph replay when allowed), then for a
-// two-iteration shape its one or two single iterations.
+// multi-iteration shape its one to four single iterations.
 Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double cells,
                           bool tuning) {
   const long rest = c - it;

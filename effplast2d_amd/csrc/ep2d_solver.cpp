@@ -196,7 +196,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   r_.Uy[1] = fused ? (double*)alloc(nUy * 8) : r_.Uy[0];
   r_.Vx = alloc(nUx * vbytes);
   r_.Vy = alloc(nUy * vbytes);
-  // second velocity buffers: two-iteration launches read one and write the
+  // second velocity buffers: multi-iteration launches read one and write the
   // other; boundary velocities (never updated) are kept equal in both
   r_.Vx2 = fused ? alloc(nUx * vbytes) : r_.Vx;
   r_.Vy2 = fused ? alloc(nUy * vbytes) : r_.Vy;
@@ -1233,7 +1233,7 @@ void Solver::setField(int field, const double* host, size_t count) {
     case EP2D_FIELD_UX: h2d(r_.Ux[cur_], host, N); return;
     case EP2D_FIELD_UY: h2d(r_.Uy[cur_], host, N); return;
     case EP2D_FIELD_VX: case EP2D_FIELD_VY: {
-      // both buffers, so the velocities two-iteration launches never write
+      // both buffers, so the velocities multi-iteration launches never write
       // (the boundary) agree
       void* dst[2] = {field == EP2D_FIELD_VX ? r_.Vx : r_.Vy,
                       field == EP2D_FIELD_VX ? r_.Vx2 : r_.Vy2};
