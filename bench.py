@@ -31,6 +31,8 @@ replays on the engine's stream (the sampled single-dispatch figure beside it).
 Other entry points:
   --probe      a fixed 200-iteration run of one scheme plus a copy kernel of
                known size (for rocprofv3 --pmc counter runs)
+  --probe-shapes  the batched probe once per listed kernel shape (for a
+               rocprofv3 kernel trace by grid size)
   --sweep      fused-kernel duration per grid size, scheme and tile height
   --config3    4096^2, 16 pores, porosity 0.05: split against FP64 moduli
   --summarize  turn rocprofv3 csv output into the short text summaries kept
