@@ -146,6 +146,10 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   // the fused kernels address a field with 32-bit byte offsets
   if ((long)(nX + 1) * (long)(nY + 1) > (1L << 29))
     throw ArgError("grid too large: every field must stay under 4 GiB (nX, nY <= 23169)");
+  if (opt_.itersPerLaunch < 0 || opt_.itersPerLaunch > 4)
+    throw ArgError("itersPerLaunch must be 1, 2, 3 or 4");
+  if (opt_.itersPerLaunch >= 2 && opt_.marchRows <= 0)
+    throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
   if (opt_.itersPerLaunch >= 2 && !multi_iter_fits(nX, nY))
     throw ArgError("itersPerLaunch 2 to 4 needs every field under 2 GiB (nX, nY <= 16382)");
   if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");

@@ -182,3 +182,20 @@ def test_struct_layouts_match_the_header():
     for name, cls in pairs.items():
         assert lib.ep2d_struct_size(name.encode()) == C.sizeof(cls), name
     assert lib.ep2d_struct_size(b"no_such_struct") == 0
+
+
+@pytest.mark.parametrize("nX,kw,msg", [
+    (23170, {}, "grid too large"),
+    (16383, {"marchRows": 16, "itersPerLaunch": 3}, "under 2 GiB"),
+    (64, {"marchRows": 16, "itersPerLaunch": 5}, "itersPerLaunch must be"),
+    (64, {"itersPerLaunch": 3}, "marchRows > 0"),
+])
+def test_shape_and_size_limits_fail_before_the_device(nX, kw, msg):
+    """Shape options and grid-size limits are checked before any device call:
+    fields under 4 GiB (32-bit offsets of the fused kernels), multi-iteration
+    launches only with fields under 2 GiB, 1 to 4 iterations per launch, and
+    multi-iteration launches only with a segment height."""
+    pa, _ = ep.derive_params(Lx=20.0, Ly=20.0, nX=nX, nPores=1, porosity=0.01)
+    with pytest.raises(ep.Ep2dError, match=msg) as ei:
+        ep.EffPlast2D(pa, nX, mode="split", **kw)
+    assert "EP2D_ERR_ARG" in str(ei.value)
