@@ -376,11 +376,16 @@ Batch::Seq Batch::launchChunk(long c) {
   auto allowed = [&](int k) { return multiOk || !multiIter(kCand[k].kind); };
   // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
   // every candidate, as many as the chunk holds (at least one timed round)
+  // (leaving room, when the chunk is long enough for two rounds besides, for
+  // the refinement of the closest candidates below: longer back-to-back runs
+  // rank them more reliably than the interleaved rounds)
   constexpr long kTuneEach = 4;
   long roundIters = 0;
   for (int k = 0; k < kNCand; k++)
     if (allowed(k)) roundIters += 2 * ipl(kCand[k].kind);
-  const long rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
+  const long refineRoom = (long)kRefCands * kRefRounds * kRefLaunches * 4;
+  long rounds = std::min<long>(kTuneEach + 1, (c - 2 - refineRoom) / roundIters);
+  if (rounds < 2) rounds = std::min<long>(kTuneEach + 1, (c - 2) / roundIters);
   const bool tuning = !fixed_ && tuneDue_ && rounds >= 2;
   ids.push_back(tuning ? 1 : 0);
   ids.push_back(shape_.kind);
@@ -442,9 +447,8 @@ Batch::Seq Batch::launchChunk(long c) {
     {
       int near[kNCand], nn = 0;
       const double best = st_.candUs[candIndex(shape_)];
-      for (int k = 0; k < kNCand && nn < 3; k++)
+      for (int k = 0; k < kNCand && nn < kRefCands; k++)
         if (allowed(k) && st_.candUs[k] > 0 && st_.candUs[k] <= 1.03 * best) near[nn++] = k;
-      constexpr int kRefRounds = 4, kRefLaunches = 4;
       long need = 0;
       for (int q = 0; q < nn; q++)
         need += kRefRounds * kRefLaunches * ipl(kCand[near[q]].kind);

@@ -62,6 +62,9 @@ class Batch {
     int rows = 8;       // tile height or segment rows
   };
   static constexpr int kNCand = 22;
+  // refinement of the tuning: up to kRefCands candidates within 3% of the
+  // best, kRefRounds rounds of kRefLaunches back-to-back launches each
+  static constexpr int kRefCands = 3, kRefRounds = 4, kRefLaunches = 4;
   static const Shape kCand[kNCand];
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
