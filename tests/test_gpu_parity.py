@@ -8,7 +8,7 @@ Tolerances (written per test):
     bit for bit -- and the masks, which are integer, exactly.
   * split scheme (FP32 stress/velocity, FP64 displacement): fields after a
     fixed number of iterations within 1e-4 relative (max-norm); the converged
-    moduli within the north-star bound 1e-4 relative where the moduli are not
+    moduli within 1e-4 relative of the FP64 solve where the moduli are not
     a small difference quotient (Kd, G), field relative L2 <= 1e-3, iteration
     counts within 10%.
 """
