@@ -34,7 +34,7 @@ Other entry points:
   --probe-shapes  the batched probe once per listed kernel shape (for a
                rocprofv3 kernel trace by grid size)
   --sweep      fused-kernel duration per grid size, scheme and tile height
-  --config3    4096^2, 16 pores, porosity 0.05: split against FP64 moduli
+  --rve4096    4096^2, 16 pores, porosity 0.05: split against FP64 moduli
   --summarize  turn rocprofv3 csv output into the short text summaries kept
                under profiles/
 """
@@ -239,7 +239,7 @@ def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
 ITER_CAP_PER_N = 40
 
 
-def c4_requests(n, seed):
+def mixed_requests(n, seed):
     """Batched sweep: sizes uniform in {128,256,512,1024}, porosity U[0.005,0.05],
     nPores in {1,2,3}, Y/K0 U[1e-3,5e-3], preload magnitude U[1e-3,5e-3] along a
     random unit direction in (Exx, Eyy, Exy); nTasks=3."""
@@ -263,7 +263,7 @@ def c4_requests(n, seed):
     return reqs
 
 
-def run_config1(steps=5):
+def run_elastic_128(steps=5):
     """128^2, one central pore (porosity 0.005), elastic (nTasks=1), plain
     FP64 (the reference's four kernels), preload -1e-5 along [1,1,0],
     eIter=1e-8: solves/s and the numerical moduli against the analytic
@@ -284,13 +284,13 @@ def run_config1(steps=5):
            "KdExactElast": res["KdExactElast"], "KphiExact": res["an"]["Kphi"],
            "Kd_rel_vs_exact": abs(res["num"]["Kd"] - res["KdExactElast"]) / res["KdExactElast"],
            "Kphi_rel_vs_exact": abs(res["num"]["Kphi"] - res["an"]["Kphi"]) / res["an"]["Kphi"]}
-    log(f"[bench] config1: {out['solves_per_s']:.1f} solves/s, Kd {out['Kd']:.5f} (exact "
+    log(f"[bench] elastic 128: {out['solves_per_s']:.1f} solves/s, Kd {out['Kd']:.5f} (exact "
         f"{out['KdExactElast']:.5f}), Kphi {out['Kphi']:.4f} (exact {out['KphiExact']:.4f})")
     return out
 
 
-def run_config4(n, seed, check):
-    reqs = c4_requests(n, seed)
+def run_mixed_batch(n, seed, check):
+    reqs = mixed_requests(n, seed)
     res, st, dt = run_batch(reqs, "split", check)
     lat = [r["tDone"] - r["tSubmit"] for r in res.values()]
     cells = st["cellIterations"]
@@ -313,12 +313,12 @@ def run_config4(n, seed, check):
            "converged": sum(bool(r["result"]["converged"]) for r in res.values()),
            "by_size": by_size, "iteration_cap": f"{ITER_CAP_PER_N} N per load step"}
     out.update(batch_stats_summary(st, "split"))
-    log(f"[bench] config4 {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
+    log(f"[bench] mixed batch {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
         f"{cells / dt / 1e9:.1f} G cell-iter/s, kernel {out['achieved_GBs']:.0f} GB/s")
     return out, res
 
 
-def c5_stream(rate, duration, seed):
+def serving_stream(rate, duration, seed):
     import numpy as np
     from effplast2d_amd import derive_params
     rng = np.random.default_rng(seed)
@@ -357,13 +357,13 @@ def plain_iteration_ms(nx, iters=200):
         return 1e3 * (time.perf_counter() - t0) / iters
 
 
-def run_config5(rate, duration, seed, check, budget_gib):
+def run_serving(rate, duration, seed, check, budget_gib):
     """Continuous batching under Poisson arrivals: requests are submitted at
     their arrival times between chunks; each chunk admits what the HBM budget
     allows, runs every admitted RVE to its next residual check, retires the
     finished ones and backfills."""
     from effplast2d_amd import Batch
-    stream = c5_stream(rate, duration, seed)
+    stream = serving_stream(rate, duration, seed)
     done = {}
     with Batch(mode="split", checkInterval=check, timingEvery=1,
                memoryBudget=int(budget_gib * (1 << 30)), adaptiveCheck=ADAPTIVE_CHECK) as b:
@@ -619,7 +619,7 @@ def pmc_traffic(fetch_dir, write_dir, kernel_key, copy_mib, dest, command):
     print(text)
 
 
-def run_config3(check, nIter=100_000):
+def run_rve4096(check, nIter=100_000):
     """4096^2, 16 pores (nPores=4), porosity 0.05, Y=2.5e-3, nTasks=3: the
     split scheme's moduli (including the periodic internal-cell ones) against
     an FP64 run of the same case.  Each load step runs to eIter=1e-8 or to
@@ -645,7 +645,7 @@ def run_config3(check, nIter=100_000):
                      "converged": bool(res["converged"]),
                      "num": res["num"], "num_per": res["num_per"], "Gxy": res["Gxy"],
                      "GxyPer": res["GxyPer"]}
-        log(f"[bench] config3 {mode}: {dt:.1f} s, iters {out[mode]['iters']}")
+        log(f"[bench] rve4096 {mode}: {dt:.1f} s, iters {out[mode]['iters']}")
     err = {}
     for grp in ("num", "num_per"):
         for k in ("Kphi", "Kd", "G"):
@@ -655,7 +655,7 @@ def run_config3(check, nIter=100_000):
         a, b = out["split"][k], out["fp64"][k]
         err[k] = abs(a - b) / abs(b) if b else None
     out["split_rel_err_vs_fp64"] = err
-    print(json.dumps({"config3": out}), flush=True)
+    print(json.dumps({"rve4096_16pores": out}), flush=True)
 
 
 def main():
@@ -676,8 +676,8 @@ def main():
                     help="launch the batched chunks on the stream instead of replaying graphs "
                          "(for kernel tracers that cannot follow graph replays)")
     ap.add_argument("--single-steps", type=int, default=2)
-    ap.add_argument("--c4-rves", type=int, default=256)
-    ap.add_argument("--c4-seed", type=int, default=2024)
+    ap.add_argument("--mixed-rves", type=int, default=256)
+    ap.add_argument("--mixed-seed", type=int, default=2024)
     ap.add_argument("--serve-rate", type=float, default=1.5)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-seed", type=int, default=7)
@@ -708,7 +708,7 @@ def main():
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--sweep-sizes", default="256,1024,4096")
     ap.add_argument("--sweep-iters", type=int, default=300)
-    ap.add_argument("--config3", action="store_true")
+    ap.add_argument("--rve4096", action="store_true")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None)
     ap.add_argument("--pmc-kernel", default="k_fused<double")
     ap.add_argument("--pmc-command", default="")
@@ -741,8 +741,8 @@ def main():
         return 0
     global ADAPTIVE_CHECK
     ADAPTIVE_CHECK = a.adaptive_check
-    if a.config3:
-        run_config3(a.check_interval)
+    if a.rve4096:
+        run_rve4096(a.check_interval)
         return 0
 
     from effplast2d_amd import device_count
@@ -767,9 +767,9 @@ def main():
             if isinstance(r, dict) and "moduli" in r}
         legs["speedup_vs_single_plain_fp64"] = head["solves_per_s"] / \
             legs["single_plain"]["solves_per_s"]
-        legs["config1_elastic_plain"] = run_config1()
-        legs["config4_batched_sweep"], _ = run_config4(a.c4_rves, a.c4_seed, a.check_interval)
-        legs["config5_serving"] = run_config5(a.serve_rate, a.serve_seconds, a.serve_seed,
+        legs["elastic_128_plain"] = run_elastic_128()
+        legs["mixed_batch"], _ = run_mixed_batch(a.mixed_rves, a.mixed_seed, a.check_interval)
+        legs["serving"] = run_serving(a.serve_rate, a.serve_seconds, a.serve_seed,
                                               a.check_interval, a.serve_budget_gib)
     frac = head["achieved_GBs"] / HBM_PEAK_GBS
     traffic = read_traffic(head["kernel_shape"], a.mode, nx)

@@ -1,4 +1,4 @@
-"""Golden pa.dat vectors for the configurations of the benchmark suite.
+"""Golden pa.dat vectors for the RVEs the benchmark and tests use.
 
 Independent restatement of get_sigma_2D.m:3-32 in Python floats (IEEE
 binary64), evaluated in MATLAB's left-to-right order:
@@ -22,9 +22,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 CASES = {
     # name: Lx, Ly, nGrid, nPores, porosity, Y, K0, G0, rho0, CFL
-    "config1_128_elastic": (20.0, 20.0, 4, 1, 0.005, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
-    "config2_1024_plastic": (20.0, 20.0, 32, 1, 0.01, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
-    "config3_4096_16pores": (20.0, 20.0, 128, 4, 0.05, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
+    "rve128_elastic": (20.0, 20.0, 4, 1, 0.005, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
+    "rve1024_plastic": (20.0, 20.0, 32, 1, 0.01, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
+    "rve4096_16pores": (20.0, 20.0, 128, 4, 0.05, 2.5e-3, 1.0, 0.5, 1.0, 0.5),
     "driver_defaults_2048": (20.0, 20.0, 64, 1, 0.005, 0.00001, 1.0, 0.01, 1.0, 0.5),
 }
 

@@ -57,7 +57,7 @@ def test_derive_params_matches_golden_and_oracle():
 
 
 def test_pa_file_round_trip(tmp_path):
-    src = os.path.join(GOLDEN, "pa_config2_1024_plastic.dat")
+    src = os.path.join(GOLDEN, "pa_rve1024_plastic.dat")
     pa = ep.read_pa(src)
     out = tmp_path / "pa.dat"
     ep.write_pa(out, pa)
