@@ -5,4 +5,4 @@ from ._lib import (FIELDS, MODE_FP64, MODE_PLAIN, MODE_SPLIT, MODES, Ep2dError, 
                    LIB_PATH)
 from .params import derive_params, read_pa, write_pa  # noqa: F401
 from .solver import EffPlast2D, device_count  # noqa: F401
-from .batch import Batch  # noqa: F401,E402
+from .batch import Batch, request_bytes  # noqa: F401,E402

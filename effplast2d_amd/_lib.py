@@ -6,8 +6,11 @@ there is no fallback implementation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libep2d.so")
@@ -196,6 +199,8 @@ SIGNATURES = {
     "ep2d_sync": ([C.c_void_p], C.c_int),
     "ep2d_device_bytes": ([C.c_void_p], C.c_size_t),
     "ep2d_batch_default_options": ([C.POINTER(BatchOptions)], None),
+    "ep2d_batch_request_bytes": ([C.POINTER(BatchOptions), C.POINTER(Request),
+                                  C.POINTER(C.c_size_t)], C.c_int),
     "ep2d_batch_create": ([C.POINTER(BatchOptions), C.POINTER(C.c_void_p)], C.c_int),
     "ep2d_batch_destroy": ([C.c_void_p], None),
     "ep2d_batch_submit": ([C.c_void_p, C.POINTER(Request), C.POINTER(C.c_longlong)], C.c_int),
@@ -230,3 +235,28 @@ def check(rc):
     if rc != 0:
         raise Ep2dError(rc, lib().ep2d_last_error().decode())
     return rc
+
+
+# Handles (EffPlast2D, Batch) still open when the interpreter exits are
+# closed by an exit handler, while the HIP runtime is still up; a close that
+# would only happen during the interpreter's own finalisation (garbage
+# collection of module globals) is skipped instead, so no HIP call runs while
+# the runtime or a profiler attached to it is shutting down.
+_live = weakref.WeakSet()
+
+
+def track(obj):
+    _live.add(obj)
+
+
+def finalizing():
+    return sys.is_finalizing()
+
+
+@atexit.register
+def _close_all():
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass

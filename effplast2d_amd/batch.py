@@ -43,6 +43,7 @@ class Batch:
         h = C.c_void_p()
         check(lib().ep2d_batch_create(C.byref(o), C.byref(h)))
         self._h = h
+        L.track(self)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -50,6 +51,8 @@ class Batch:
             self._h = None
 
     def __del__(self):
+        if L.finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -64,23 +67,15 @@ class Batch:
     def submit(self, pa, nX, nTasks=3, nIter=100_000, eIter=1e-8, nTimeSteps=1, initLoad=-3e-3,
                loadType=(1.0, 1.0, 0.0), loadValue=0.0, shearLoadType=None, checkInterval=0,
                tag=0):
-        rq = L.Request()
-        pa = np.ascontiguousarray(pa, dtype=np.float64)
-        for k in range(L.NPARS):
-            rq.pa[k] = float(pa[k])
-        rq.nX, rq.nTasks, rq.nIter, rq.eIter = int(nX), int(nTasks), int(nIter), float(eIter)
-        rq.nTimeSteps, rq.initLoadValue, rq.loadValue = int(nTimeSteps), float(initLoad), float(loadValue)
-        for k in range(3):
-            rq.loadType[k] = float(loadType[k])
-        if shearLoadType is not None:
-            rq.useShearLoadType = 1
-            for k in range(3):
-                rq.shearLoadType[k] = float(shearLoadType[k])
-        rq.checkInterval = int(checkInterval)
-        rq.tag = int(tag)
+        rq = make_request(pa, nX, nTasks, nIter, eIter, nTimeSteps, initLoad, loadType,
+                          loadValue, shearLoadType, checkInterval, tag)
         rid = C.c_longlong()
         check(lib().ep2d_batch_submit(self._h, C.byref(rq), C.byref(rid)))
         return rid.value
+
+    def request_bytes(self, pa, nX, **kw):
+        """Device bytes a request counts against memoryBudget when admitted."""
+        return request_bytes(self._opts.mode, pa, nX, **kw)
 
     def step(self):
         a, q = C.c_int(), C.c_int()
@@ -103,3 +98,35 @@ class Batch:
         st = L.BatchStats()
         check(lib().ep2d_batch_stats_get(self._h, C.byref(st)))
         return st.as_dict()
+
+
+def make_request(pa, nX, nTasks=3, nIter=100_000, eIter=1e-8, nTimeSteps=1, initLoad=-3e-3,
+                 loadType=(1.0, 1.0, 0.0), loadValue=0.0, shearLoadType=None, checkInterval=0,
+                 tag=0):
+    rq = L.Request()
+    pa = np.ascontiguousarray(pa, dtype=np.float64)
+    for k in range(L.NPARS):
+        rq.pa[k] = float(pa[k])
+    rq.nX, rq.nTasks, rq.nIter, rq.eIter = int(nX), int(nTasks), int(nIter), float(eIter)
+    rq.nTimeSteps, rq.initLoadValue, rq.loadValue = int(nTimeSteps), float(initLoad), float(loadValue)
+    for k in range(3):
+        rq.loadType[k] = float(loadType[k])
+    if shearLoadType is not None:
+        rq.useShearLoadType = 1
+        for k in range(3):
+            rq.shearLoadType[k] = float(shearLoadType[k])
+    rq.checkInterval = int(checkInterval)
+    rq.tag = int(tag)
+    return rq
+
+
+def request_bytes(mode, pa, nX, **kw):
+    """Device bytes the engine counts for one request of this grid and pore
+    count in the given scheme (host only, ep2d_batch_request_bytes)."""
+    o = L.BatchOptions()
+    lib().ep2d_batch_default_options(C.byref(o))
+    o.mode = L.MODES[mode] if isinstance(mode, str) else int(mode)
+    rq = make_request(pa, nX, **kw)
+    n = C.c_size_t()
+    check(lib().ep2d_batch_request_bytes(C.byref(o), C.byref(rq), C.byref(n)))
+    return int(n.value)

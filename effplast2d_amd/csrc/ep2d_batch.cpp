@@ -114,13 +114,10 @@ long long Batch::submit(const ep2d_request& rq) {
   return nextId_++;
 }
 
-// Device bytes one RVE holds (Solver allocations), to admit within budget.
+// Device bytes one RVE holds (the Solver's allocations), to admit within
+// budget.
 size_t Batch::estimateBytes(const ep2d_request& rq) const {
-  const size_t n = (size_t)rq.nX;
-  const size_t pts = (n + 1) * n;
-  const size_t vb = opt_.mode == EP2D_MODE_SPLIT ? 4 : 8;
-  return pts * 8 * 4 + pts * vb * 2 + n * n * (1 + 5 * 8) + (n - 1) * (n - 1) * 16 +
-         ((n + 63) / 64) * ((n + 3) / 4) * 88 + 64 * n * 8 + (1 << 16);
+  return Solver::plannedDeviceBytes(rq.nX, rq.nX, opt_.mode, rq.pa[EP2D_PA_NPORES]);
 }
 
 void Batch::admit() {
@@ -674,6 +671,13 @@ void Batch::step(int* nActive, int* nQueued) {
         gone[k] = 1;
       } catch (const ArgError& e) {
         fail(sl, EP2D_ERR_ARG, e.what());
+        gone[k] = 1;
+      } catch (const HipError& e) {
+        // a failed copy or launch of this RVE's post-processing or next
+        // load step fails this request only; slots already finished in this
+        // pass are still removed below, so a later step cannot finish them
+        // twice
+        fail(sl, EP2D_ERR_HIP, e.what());
         gone[k] = 1;
       }
     }

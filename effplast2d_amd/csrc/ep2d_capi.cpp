@@ -16,7 +16,21 @@ struct ep2d_batch {
 };
 
 namespace {
-thread_local std::string g_last_error;
+// Message of the calling thread's last failure.  A plain character array:
+// nothing is registered to run at thread or process exit.
+thread_local char g_last_error[512];
+
+void set_error(const char* msg) {
+  std::snprintf(g_last_error, sizeof(g_last_error), "%s", msg ? msg : "");
+}
+
+// Set by the library's unload hook.  Destroy calls made after it (for
+// instance from a language runtime's own teardown, after the HIP runtime or
+// a profiler attached to it has shut down) release host memory only and
+// make no HIP call; the process is ending and the driver reclaims the device
+// memory.
+bool g_unloading = false;
+__attribute__((destructor)) void on_unload() { g_unloading = true; }
 
 template <typename F>
 int guarded(F&& f) {
@@ -24,22 +38,22 @@ int guarded(F&& f) {
     f();
     return EP2D_OK;
   } catch (const ep2d::StatusError& e) {
-    g_last_error = e.what();
+    set_error(e.what());
     return e.status;
   } catch (const ep2d::ArgError& e) {
-    g_last_error = e.what();
+    set_error(e.what());
     return EP2D_ERR_ARG;
   } catch (const ep2d::HipError& e) {
-    g_last_error = e.what();
+    set_error(e.what());
     return EP2D_ERR_HIP;
   } catch (const std::invalid_argument& e) {
-    g_last_error = e.what();
+    set_error(e.what());
     return EP2D_ERR_ARG;
   } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
+    set_error("host allocation failed");
     return EP2D_ERR_MEMORY;
   } catch (const std::exception& e) {
-    g_last_error = e.what();
+    set_error(e.what());
     return EP2D_ERR_STATE;
   }
 }
@@ -57,7 +71,7 @@ ep2d::Batch* B(ep2d_batch* b) {
 
 extern "C" {
 
-const char* ep2d_last_error(void) { return g_last_error.c_str(); }
+const char* ep2d_last_error(void) { return g_last_error; }
 
 const char* ep2d_version(void) { return "ep2d 0.1 gfx950"; }
 
@@ -159,6 +173,7 @@ int ep2d_create_from_physics(const ep2d_physics* phys, const ep2d_options* opt, 
 
 void ep2d_destroy(ep2d_handle* h) {
   if (!h) return;
+  if (g_unloading) return;   // process teardown: no HIP call (see on_unload)
   try {
     delete h->solver;
   } catch (...) {
@@ -367,6 +382,18 @@ void ep2d_batch_default_options(ep2d_batch_options* o) {
   o->timingEvery = 0;
 }
 
+int ep2d_batch_request_bytes(const ep2d_batch_options* opt, const ep2d_request* rq,
+                             size_t* bytes) {
+  return guarded([&] {
+    if (!opt || !rq || !bytes) throw ep2d::ArgError("null argument");
+    if (opt->mode != EP2D_MODE_FP64 && opt->mode != EP2D_MODE_SPLIT)
+      throw ep2d::ArgError("batch mode must be EP2D_MODE_FP64 or EP2D_MODE_SPLIT");
+    if (rq->nX < 8) throw ep2d::ArgError("request: nX must be >= 8");
+    *bytes = ep2d::Solver::plannedDeviceBytes(rq->nX, rq->nX, opt->mode,
+                                              rq->pa[EP2D_PA_NPORES]);
+  });
+}
+
 int ep2d_batch_create(const ep2d_batch_options* opt, ep2d_batch** out) {
   return guarded([&] {
     if (!out) throw ep2d::ArgError("null argument");
@@ -386,6 +413,7 @@ int ep2d_batch_create(const ep2d_batch_options* opt, ep2d_batch** out) {
 
 void ep2d_batch_destroy(ep2d_batch* b) {
   if (!b) return;
+  if (g_unloading) return;   // process teardown: no HIP call (see on_unload)
   try {
     delete b->engine;
   } catch (...) {

@@ -365,6 +365,45 @@ void Solver::release() {
   stream_ = nullptr;
 }
 
+// Entries of the gather table buildGatherTable() makes (same loops, counted).
+static size_t gather_entries(long nX, long nY, double nPores) {
+  // init() rejects pore counts outside [1, 64]; count within that range
+  if (!(nPores >= 1.0)) nPores = 1.0;
+  if (nPores > 64.0) nPores = 64.0;
+  size_t n = 3 * (size_t)(2 * (nX - 2) + 2 * (nY - 2));
+  if (nPores > 2) {
+    const long nP = (long)nPores;
+    long c = 0;
+    for (long i = nX / nP; i < nX * (nP - 1) / nP; i++) c++;
+    n += 3 * 2 * (size_t)c;
+    c = 0;
+    for (long j = nY / nP; j < nY * (nP - 1) / nPores; j++) c++;
+    n += 3 * 2 * (size_t)c;
+  }
+  return n + 20 * (size_t)nPores * (size_t)nPores;
+}
+
+// Device bytes init() allocates for an RVE on the one-iteration shapes the
+// batched engine uses (no segment tables): the sum of alloc() sizes, each at
+// least 256 bytes, in the same order.
+size_t Solver::plannedDeviceBytes(int nX, int nY, int mode, double nPores) {
+  auto a = [](size_t b) { return std::max<size_t>(b, 256); };
+  const size_t nC = (size_t)nX * nY, nV = (size_t)(nX - 1) * (nY - 1);
+  const size_t nUx = (size_t)(nX + 1) * nY, nUy = (size_t)nX * (nY + 1);
+  const bool fused = mode != EP2D_MODE_PLAIN;
+  const size_t vb = mode == EP2D_MODE_SPLIT ? sizeof(float) : sizeof(double);
+  size_t t = a(nUx * 8) + a(nUy * 8);
+  if (fused) t += a(nUx * 8) + a(nUy * 8);
+  t += a(nUx * vb) + a(nUy * vb);
+  if (fused) t += a(nUx * vb) + a(nUy * vb);
+  t += a(nC) + 5 * a(nC * 8) + 2 * a(nV * 8);
+  t += a(4 * sizeof(unsigned long long)) + a((size_t)avg_partial_blocks(nX, nY) * NAVG * 8) +
+       a(NAVG * 8);
+  const size_t g = gather_entries(nX, nY, nPores);
+  t += a(g * sizeof(int2)) + a(g * sizeof(double)) + a(8 * sizeof(double*));
+  return t;
+}
+
 // Gather table of the sampled values the host post-processing needs:
 // boundary rows/columns of P, tauXX, tauYY (EffPlast2D.cu:825-882, 925-942)
 // and the pore-edge displacement windows (:549-594).
@@ -564,7 +603,9 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
   if (avgMs) *avgMs = timedSamples_ ? timedMs_ / timedSamples_ : 0.0;
   if (samples) *samples = timedSamples_;
   if (name) {
-    static thread_local std::string nm;
+    // held by the handle: the pointer stays valid until the next call or
+    // the handle's destruction
+    std::string& nm = timingName_;
     nm = fused_kernel_name(mode_);
     if (marchRows_ > 0 && multiIter_)
       nm = (ipl_ == 2 ? std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>"

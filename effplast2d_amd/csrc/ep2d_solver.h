@@ -54,6 +54,10 @@ class Solver {
   Solver(const Solver&) = delete;
   Solver& operator=(const Solver&) = delete;
 
+  // Device bytes a Solver of this grid, scheme and pore count allocates when
+  // it runs on one-iteration shapes (the batched engine's solvers).
+  static size_t plannedDeviceBytes(int nX, int nY, int mode, double nPores);
+
   // EffPlast2D::ComputeEffModuli / ComputeEffDamping (EffPlast2D.cu:179-310)
   void computeEffModuli(double initLoadValue, const double loadType[3], ep2d_result* out);
   void computeEffDamping(double initLoadValue, const double loadType[3], double loadValue,
@@ -193,6 +197,7 @@ class Solver {
   std::vector<int> pendingIters_;   // iterations in each pending timed launch
   double timedMs_ = 0.0;
   long timedSamples_ = 0;
+  std::string timingName_;         // kernel name getTiming() hands out
 
   // job state
   struct TaskPlan {

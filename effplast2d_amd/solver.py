@@ -68,6 +68,7 @@ class EffPlast2D:
         check(lib().ep2d_create(_dptr(self.pa), self.nX, self.nY, C.byref(self._opts),
                                 C.byref(h)))
         self._h = h
+        L.track(self)
 
     @classmethod
     def from_physics(cls, Lx=20.0, Ly=None, nGrid=4, nX=0, nPores=1, porosity=0.005, Y=2.5e-3,
@@ -85,6 +86,8 @@ class EffPlast2D:
             self._h = None
 
     def __del__(self):
+        if L.finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -347,6 +347,11 @@ typedef struct ep2d_batch_stats {
 typedef struct ep2d_batch ep2d_batch;
 
 void ep2d_batch_default_options(ep2d_batch_options* opt);
+/* Device bytes the engine counts against memoryBudget for one request in the
+ * given options' scheme (host only, no device needed): what the request's
+ * RVE state allocates when admitted.  No reference counterpart. */
+int ep2d_batch_request_bytes(const ep2d_batch_options* opt, const ep2d_request* rq,
+                             size_t* bytes);
 int ep2d_batch_create(const ep2d_batch_options* opt, ep2d_batch** out);
 void ep2d_batch_destroy(ep2d_batch* b);
 /* Queue a request; *id receives its identifier.  Validation errors return

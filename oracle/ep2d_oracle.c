@@ -305,7 +305,11 @@ double ora_find_max_abs(const double* arr, long size) {
 double ora_find_max_abs_signed(const double* v, long size) {
   double max_el = 0.0;
   for (long i = 0; i < size; i++) {
+#if ORA_MUTANT == 15   /* planted mistake: the magnitude instead of the signed element */
+    if (fabs(v[i]) > fabs(max_el)) max_el = fabs(v[i]);
+#else
     if (fabs(v[i]) > fabs(max_el)) max_el = v[i];
+#endif
   }
   return max_el;
 }
@@ -360,7 +364,9 @@ void ora_averages(const ora_config* c, const ora_fields* f, ora_step_record* r) 
       if (i < nX - 1 && j < nY - 1) {
         sigma[2] += tauXY[j * (nX - 1) + i];
       }
+#if ORA_MUTANT != 10   /* planted mistake 10: the nu0 (zz) fold dropped */
       sigma[2] += nu0 * (tauXX[j * nX + i] + tauYY[j * nX + i] - 2.0 * P[j * nX + i]);
+#endif
 
       if (i > perOffsetX && i < nX - perOffsetX && j > perOffsetY && j < nY - perOffsetY) {
         sigmaPer[0] += tauXX[j * nX + i] - P[j * nX + i];
@@ -382,7 +388,11 @@ void ora_averages(const ora_config* c, const ora_fields* f, ora_step_record* r) 
       }
     }
   }
+#if ORA_MUTANT == 9   /* planted mistake: divided by the summed cell count */
+  sigma[0] /= (nX - 2) * (nY - 2);
+#else
   sigma[0] /= nX * nY;
+#endif
   sigma[1] /= nX * nY;
   sigma[2] /= (nX - 1) * (nY - 1);
   sigma[3] /= nX * nY;
@@ -402,9 +412,14 @@ double ora_delta_p_honest(const ora_config* c, const ora_fields* f) {
   const long nX = c->nX, nY = c->nY;
   const double* P = f->P; const double* tauXX = f->tauXX; const double* tauYY = f->tauYY;
   double deltaPx = 0.0, deltaPy = 0.0;
+#if ORA_MUTANT == 5   /* planted mistake: bottom row 1 instead of row 0 */
+  const long jb = 1;
+#else
+  const long jb = 0;
+#endif
   for (int i = 1; i < nX - 1; i++) {
-    deltaPx += tauXX[0 * nX + i] - P[0 * nX + i];
-    deltaPx += tauYY[0 * nX + i] - P[0 * nX + i];
+    deltaPx += tauXX[jb * nX + i] - P[jb * nX + i];
+    deltaPx += tauYY[jb * nX + i] - P[jb * nX + i];
     deltaPx += tauXX[(nY - 1) * nX + i] - P[(nY - 1) * nX + i];
     deltaPx += tauYY[(nY - 1) * nX + i] - P[(nY - 1) * nX + i];
   }
@@ -435,11 +450,20 @@ double ora_delta_p_periodic(const ora_config* c, const ora_fields* f) {
     deltaPx += tauYY[nY * (nP - 1) / nP * nX + i] - P[nY * (nP - 1) / nP * nX + i];
   }
   deltaPx /= nX * (nP - 2) / nP;
+#if ORA_MUTANT == 6   /* planted mistake: the mirrored column N(nP-1)/nP */
+  const long cr = nX * (nP - 1) / nP;
+#else
+  const long cr = nX * (nP - 2) / nP;                /* :875-876 */
+#endif
+#if ORA_MUTANT == 7   /* planted mistake: integer bound of the y loop */
+  for (int j = nY / nP; j < nY * (nP - 1) / nP; j++) {
+#else
   for (int j = nY / nP; j < nY * (nP - 1) / nPores; j++) {
+#endif
     deltaPy += tauXX[j * nX + nX / nP] - P[j * nX + nX / nP];
     deltaPy += tauYY[j * nX + nX / nP] - P[j * nX + nX / nP];
-    deltaPy += tauXX[j * nX + nX * (nP - 2) / nP] - P[j * nX + nX * (nP - 2) / nP];
-    deltaPy += tauYY[j * nX + nX * (nP - 2) / nP] - P[j * nX + nX * (nP - 2) / nP];
+    deltaPy += tauXX[j * nX + cr] - P[j * nX + cr];
+    deltaPy += tauYY[j * nX + cr] - P[j * nX + cr];
   }
   deltaPy /= nY * (nP - 2) / nP;
   return -0.125 * (deltaPx + deltaPy);
@@ -451,7 +475,11 @@ double ora_tau_infty_honest(const ora_config* c, const ora_fields* f) {
   const double* tauXX = f->tauXX; const double* tauYY = f->tauYY;
   double tx = 0.0, ty = 0.0;
   for (int i = 1; i < nX - 1; i++) {
+#if ORA_MUTANT == 8   /* planted mistake: yy - xx on the bottom row */
+    tx += tauYY[0 * nX + i] - tauXX[0 * nX + i];
+#else
     tx += tauXX[0 * nX + i] - tauYY[0 * nX + i];
+#endif
     tx += tauXX[(nY - 1) * nX + i] - tauYY[(nY - 1) * nX + i];
   }
   tx /= (nX - 2);
@@ -520,8 +548,13 @@ int ora_hole_area(const ora_config* c, const ora_fields* f, ora_step_record* r) 
       for (int i = 0; i < 5; i++)
         if (sample_u(c, f, 0, cyIdx * (nX + 1) + rxIdx - 1 + i, &left[i])) return -1;
       rxIdx = (size_t)(cxdX + rad / dX + 0.5 * nX);
+#if ORA_MUTANT == 14   /* planted mistake: right window [-1, +3] like the left one */
+      for (int i = 0; i < 5; i++)
+        if (sample_u(c, f, 0, cyIdx * (nX + 1) + rxIdx - 1 + i, &right[i])) return -1;
+#else
       for (int i = 0; i < 5; i++)
         if (sample_u(c, f, 0, cyIdx * (nX + 1) + rxIdx - 2 + i, &right[i])) return -1;
+#endif
       const size_t cxIdx = (size_t)(cxdX + 0.5 * (nX - 1));
       size_t ryIdx = (size_t)(cydY - rad / dY + 0.5 * nY);
       for (int j = 0; j < 5; j++)
@@ -549,17 +582,59 @@ int ora_hole_area(const ora_config* c, const ora_fields* f, ora_step_record* r) 
   return 0;
 }
 
-/* ComputeEffParams, EffPlast2D.cu:311-698. */
-int ora_compute_eff_params(const ora_config* c, ora_fields* f, int step,
-                           double loadStepValue, const double loadType[3],
-                           long nTimeSteps, double curEffStrain[3],
-                           ora_step_record* recs, ora_result* res) {
+/* Per-step analytic moduli, EffPlast2D.cu:643-696: elastic (NL <= 1) from
+ * Phi0; elastoplastic from the load step 0 record (Yarushina's formulas with
+ * the boundary means deltaP, deltaPper and tauInfty). */
+void ora_analytic_moduli(const ora_config* c, int step, const ora_step_record* r,
+                         ora_result* res) {
   const long nX = c->nX, nY = c->nY;
   const double dX = c->pa[ORA_PA_DX], dY = c->pa[ORA_PA_DY];
   const double K0 = c->pa[ORA_PA_K0], G0 = c->pa[ORA_PA_G0];
   const double rad = c->pa[ORA_PA_RAD];
   const double Y = c->pa[ORA_PA_COH] / sqrt(2.0);   /* :1572 */
   const int NL = c->nl;
+  const double Phi0 = r->Phi0, Phi = r->Phi;
+  const double phi = 3.1415926 * rad * rad / (dX * (nX - 1) * dY * (nY - 1));  /* :643 */
+  if (NL <= 1) {                                    /* :645-655 */
+    const double KphiExactElast = G0 / Phi0;
+    res->KphiExactElast = KphiExactElast;
+    res->an.Kphi = G0 / Phi0;
+    res->KdExactElast = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactElast);
+    res->an.Kd = G0 / Phi0;
+  } else if (step == 0) {                           /* :656-680 */
+#if ORA_MUTANT == 13   /* planted mistake: the -1 dropped from the exponent */
+    const double KphiExactPlast = G0 / Phi / exp(fabs(r->deltaP) / Y) /
+      (1.0 + r->tauInfty * r->tauInfty / Y / Y);
+#else
+    const double KphiExactPlast = G0 / Phi / exp(fabs(r->deltaP) / Y - 1.0) /
+      (1.0 + r->tauInfty * r->tauInfty / Y / Y);
+#endif
+    res->KphiExactPlast = KphiExactPlast;
+    res->an.Kphi = KphiExactPlast;
+    res->KdExactPlast = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactPlast);
+    res->an.Kd = res->KdExactPlast;
+    const double KphiExactPlastPer = G0 / Phi0 / exp(fabs(r->deltaPper) / Y - 1.0) /
+      (1.0 + r->tauInfty * r->tauInfty / Y / Y);
+    res->KphiExactPlastPer = KphiExactPlastPer;
+    res->an_per.Kphi = KphiExactPlastPer;
+    res->KdExactPlastPer = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactPlastPer);
+    res->an_per.Kd = res->KdExactPlastPer;
+  }
+  if (NL == 3 && step == 0) {                       /* :682-696 */
+    res->GexactElast = G0 / (1.0 + Phi0);
+    res->GexactPlast = G0 / (1.0 + Phi * exp(fabs(r->deltaP) / Y - 1.0));
+    res->an.G = res->GexactPlast;
+    res->GexactPlastPer = G0 / (1.0 + Phi0 * exp(fabs(r->deltaPper) / Y - 1.0));
+    res->an_per.G = res->GexactPlastPer;
+  }
+}
+
+/* ComputeEffParams, EffPlast2D.cu:311-698. */
+int ora_compute_eff_params(const ora_config* c, ora_fields* f, int step,
+                           double loadStepValue, const double loadType[3],
+                           long nTimeSteps, double curEffStrain[3],
+                           ora_step_record* recs, ora_result* res) {
+  const long nX = c->nX, nY = c->nY;
 
   if (step == 0) {                                    /* :329-333 */
     curEffStrain[0] = curEffStrain[1] = curEffStrain[2] = 0.0;
@@ -598,37 +673,60 @@ int ora_compute_eff_params(const ora_config* c, ora_fields* f, int step,
     r->tauInfty = ora_tau_infty_honest(c, f);
     if (ora_hole_area(c, f, r)) return -2;            /* :514-620 */
 
-    const double Phi0 = r->Phi0, Phi = r->Phi;
-    const double phi = 3.1415926 * rad * rad / (dX * (nX - 1) * dY * (nY - 1));  /* :643 */
-    if (NL <= 1) {                                    /* :645-655 */
-      const double KphiExactElast = G0 / Phi0;
-      res->KphiExactElast = KphiExactElast;
-      res->an.Kphi = G0 / Phi0;
-      res->KdExactElast = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactElast);
-      res->an.Kd = G0 / Phi0;
-    } else if (step == 0) {                           /* :656-680 */
-      const double KphiExactPlast = G0 / Phi / exp(fabs(r->deltaP) / Y - 1.0) /
-        (1.0 + r->tauInfty * r->tauInfty / Y / Y);
-      res->KphiExactPlast = KphiExactPlast;
-      res->an.Kphi = KphiExactPlast;
-      res->KdExactPlast = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactPlast);
-      res->an.Kd = res->KdExactPlast;
-      const double KphiExactPlastPer = G0 / Phi0 / exp(fabs(r->deltaPper) / Y - 1.0) /
-        (1.0 + r->tauInfty * r->tauInfty / Y / Y);
-      res->KphiExactPlastPer = KphiExactPlastPer;
-      res->an_per.Kphi = KphiExactPlastPer;
-      res->KdExactPlastPer = (1.0 - phi) / (1.0 / K0 + 1.0 / KphiExactPlastPer);
-      res->an_per.Kd = res->KdExactPlastPer;
-    }
-    if (NL == 3 && step == 0) {                       /* :682-696 */
-      res->GexactElast = G0 / (1.0 + Phi0);
-      res->GexactPlast = G0 / (1.0 + Phi * exp(fabs(r->deltaP) / Y - 1.0));
-      res->an.G = res->GexactPlast;
-      res->GexactPlastPer = G0 / (1.0 + Phi0 * exp(fabs(r->deltaPper) / Y - 1.0));
-      res->an_per.G = res->GexactPlastPer;
-    }
+    ora_analytic_moduli(c, step, r, res);            /* :643-696 */
   }
   return 0;
+}
+
+/* The moduli getters, EffPlast2D.cu:1412-1539: r0 is the last record of
+ * load step 0, r1 the record of the hydrostatic increment (NL > 1), r2 the
+ * record of the deviatoric increment (NL == 3). */
+void ora_moduli_getters(int NL, const ora_step_record* r0, const ora_step_record* r1,
+                        const ora_step_record* r2, ora_result* res) {
+  if (NL == 1) {                                         /* :1412-1428 */
+    res->num.Kphi = r0->deltaP / r0->dPhi;
+    res->num_per.Kphi = r0->deltaPper / r0->dPhiPer;
+    {
+      const double pInc = -0.33333333 * (r0->sigma[0] + r0->sigma[1] + r0->sigma[2]);
+      const double epsInc = r0->epsilon[0] + r0->epsilon[1];
+      res->num.Kd = -pInc / epsInc;
+    }
+    {
+      const double pIncPer = -0.33333333 * (r0->sigmaPer[0] + r0->sigmaPer[1] + r0->sigmaPer[2]);
+      const double epsIncPer = r0->epsilonPer[0] + r0->epsilonPer[1];
+      res->num_per.Kd = -pIncPer / epsIncPer;
+    }
+    return;
+  }
+  res->num.Kphi = (r1->deltaP - r0->deltaP) / (r1->dPhi - r0->dPhi);   /* :1483-1514 */
+  res->num_per.Kphi = (r1->deltaPper - r0->deltaPper) / (r1->dPhiPer - r0->dPhiPer);
+  {
+    const double pInc = -0.33333333 * (r1->sigma[0] + r1->sigma[1] + r1->sigma[2] - r0->sigma[0] - r0->sigma[1] - r0->sigma[2]);
+#if ORA_MUTANT == 11   /* planted mistake: measured instead of applied strain */
+    const double epsInc = r1->epsilonPer[0] + r1->epsilonPer[1] - r0->epsilonPer[0] - r0->epsilonPer[1];
+#else
+    const double epsInc = r1->epsilon[0] + r1->epsilon[1] - r0->epsilon[0] - r0->epsilon[1];
+#endif
+    res->num.Kd = -pInc / epsInc;
+  }
+  {
+    const double pIncPer = -0.33333333 * (r1->sigmaPer[0] + r1->sigmaPer[1] + r1->sigmaPer[2] - r0->sigmaPer[0] - r0->sigmaPer[1] - r0->sigmaPer[2]);
+    const double epsIncPer = r1->epsilonPer[0] + r1->epsilonPer[1] - r0->epsilonPer[0] - r0->epsilonPer[1];
+    res->num_per.Kd = -pIncPer / epsIncPer;
+  }
+  if (NL != 3) return;
+#if ORA_MUTANT == 12   /* planted mistake: wrong sign of the yy stress difference */
+  res->num.G = 0.5 * (r2->sigma[0] - r1->sigma[0] + r2->sigma[1] - r1->sigma[1]) /
+               (r2->epsilon[0] - r1->epsilon[0] - r2->epsilon[1] + r1->epsilon[1]);
+#else
+  res->num.G = 0.5 * (r2->sigma[0] - r1->sigma[0] - r2->sigma[1] + r1->sigma[1]) /   /* :1516-1539 */
+               (r2->epsilon[0] - r1->epsilon[0] - r2->epsilon[1] + r1->epsilon[1]);
+#endif
+  res->num_per.G = 0.5 * (r2->sigmaPer[0] - r1->sigmaPer[0] - r2->sigmaPer[1] + r1->sigmaPer[1]) /
+                   (r2->epsilonPer[0] - r1->epsilonPer[0] - r2->epsilonPer[1] + r1->epsilonPer[1]);
+  /* extension: xy shear modulus, G = d(sigma_xy) / (2 d(eps_xy)) */
+  res->Gxy = 0.5 * (r2->sigmaXYclean - r1->sigmaXYclean) / (r2->epsilon[2] - r1->epsilon[2]);
+  res->GxyPer = 0.5 * (r2->sigmaPer[2] - r1->sigmaPer[2]) / (r2->epsilonPer[2] - r1->epsilonPer[2]);
 }
 
 /* ComputeEffModuli (EffPlast2D.cu:179-229) and the getters (:1412-1539). */
@@ -650,49 +748,15 @@ int ora_compute_eff_moduli(const ora_config* c, ora_fields* f,
   int rc = ora_compute_eff_params(c, f, 0, initLoadValue, loadType, T, cur, &recs[0], res);
   if (rc) return rc;
   const ora_step_record* r0 = &recs[T - 1];
-  if (NL == 1) {                                         /* :1412-1428 */
-    res->num.Kphi = r0->deltaP / r0->dPhi;
-    res->num_per.Kphi = r0->deltaPper / r0->dPhiPer;
-    {
-      const double pInc = -0.33333333 * (r0->sigma[0] + r0->sigma[1] + r0->sigma[2]);
-      const double epsInc = r0->epsilon[0] + r0->epsilon[1];
-      res->num.Kd = -pInc / epsInc;
-    }
-    {
-      const double pIncPer = -0.33333333 * (r0->sigmaPer[0] + r0->sigmaPer[1] + r0->sigmaPer[2]);
-      const double epsIncPer = r0->epsilonPer[0] + r0->epsilonPer[1];
-      res->num_per.Kd = -pIncPer / epsIncPer;
-    }
-  } else {
+  if (NL > 1) {
     rc = ora_compute_eff_params(c, f, 1, initLoadValue * incPercent, sph, 1, cur, &recs[T], res);
     if (rc) return rc;
-    const ora_step_record* r1 = &recs[T];               /* :1483-1514 */
-    res->num.Kphi = (r1->deltaP - r0->deltaP) / (r1->dPhi - r0->dPhi);
-    res->num_per.Kphi = (r1->deltaPper - r0->deltaPper) / (r1->dPhiPer - r0->dPhiPer);
-    {
-      const double pInc = -0.33333333 * (r1->sigma[0] + r1->sigma[1] + r1->sigma[2] - r0->sigma[0] - r0->sigma[1] - r0->sigma[2]);
-      const double epsInc = r1->epsilon[0] + r1->epsilon[1] - r0->epsilon[0] - r0->epsilon[1];
-      res->num.Kd = -pInc / epsInc;
-    }
-    {
-      const double pIncPer = -0.33333333 * (r1->sigmaPer[0] + r1->sigmaPer[1] + r1->sigmaPer[2] - r0->sigmaPer[0] - r0->sigmaPer[1] - r0->sigmaPer[2]);
-      const double epsIncPer = r1->epsilonPer[0] + r1->epsilonPer[1] - r0->epsilonPer[0] - r0->epsilonPer[1];
-      res->num_per.Kd = -pIncPer / epsIncPer;
-    }
   }
   if (NL == 3) {
     rc = ora_compute_eff_params(c, f, 2, initLoadValue * incPercent, dev, 1, cur, &recs[2 * T], res);
     if (rc) return rc;
-    const ora_step_record* r1 = &recs[T];
-    const ora_step_record* r2 = &recs[2 * T];           /* :1516-1539 */
-    res->num.G = 0.5 * (r2->sigma[0] - r1->sigma[0] - r2->sigma[1] + r1->sigma[1]) /
-                 (r2->epsilon[0] - r1->epsilon[0] - r2->epsilon[1] + r1->epsilon[1]);
-    res->num_per.G = 0.5 * (r2->sigmaPer[0] - r1->sigmaPer[0] - r2->sigmaPer[1] + r1->sigmaPer[1]) /
-                     (r2->epsilonPer[0] - r1->epsilonPer[0] - r2->epsilonPer[1] + r1->epsilonPer[1]);
-    /* extension: xy shear modulus, G = d(sigma_xy) / (2 d(eps_xy)) */
-    res->Gxy = 0.5 * (r2->sigmaXYclean - r1->sigmaXYclean) / (r2->epsilon[2] - r1->epsilon[2]);
-    res->GxyPer = 0.5 * (r2->sigmaPer[2] - r1->sigmaPer[2]) / (r2->epsilonPer[2] - r1->epsilonPer[2]);
   }
+  ora_moduli_getters(NL, r0, NL > 1 ? &recs[T] : NULL, NL == 3 ? &recs[2 * T] : NULL, res);
   return 0;
 }
 

@@ -139,6 +139,16 @@ typedef struct {
 int ora_an_static(const ora_config* c, const ora_fields* f, double deltaP, double tauInfty,
                   const double loadType[3], ora_an_out* out);
 
+/* ---- moduli ------------------------------------------------------------- */
+/* Per-step analytic moduli (EffPlast2D.cu:643-696) from one record of load
+ * step `step` into res (elastic when c->nl <= 1, plastic at step 0). */
+void ora_analytic_moduli(const ora_config* c, int step, const ora_step_record* r,
+                         ora_result* res);
+/* The numerical moduli getters (EffPlast2D.cu:1412-1539) from the records of
+ * load steps 0, 1 and 2 (r1, r2 unused when NL is 1, r2 unused when NL is 2). */
+void ora_moduli_getters(int NL, const ora_step_record* r0, const ora_step_record* r1,
+                        const ora_step_record* r2, ora_result* res);
+
 /* ---- drivers ------------------------------------------------------------ */
 /* ComputeEffParams (EffPlast2D.cu:311-698).  recs has nTimeSteps entries. */
 int ora_compute_eff_params(const ora_config* c, ora_fields* f, int step,

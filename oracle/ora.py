@@ -135,6 +135,12 @@ def lib():
                                               _dp, C.c_double, C.c_long,
                                               C.POINTER(StepRecord), C.POINTER(Result)]
         L.ora_compute_eff_damping.restype = C.c_int
+        L.ora_analytic_moduli.argtypes = [C.POINTER(Config), C.c_int, C.POINTER(StepRecord),
+                                          C.POINTER(Result)]
+        L.ora_analytic_moduli.restype = None
+        L.ora_moduli_getters.argtypes = [C.c_int, C.POINTER(StepRecord), C.POINTER(StepRecord),
+                                         C.POINTER(StepRecord), C.POINTER(Result)]
+        L.ora_moduli_getters.restype = None
         L.ora_an_ur_hydro.argtypes = [C.c_double] * 5
         L.ora_an_ur_hydro.restype = C.c_double
         L.ora_an_u_elast.argtypes = [C.c_double] * 10 + [_dp, _dp]
@@ -152,6 +158,31 @@ def lib():
 
 def _ptr(a):
     return a.ctypes.data_as(_dp)
+
+
+def make_record(**kw):
+    """A StepRecord with the given fields (the rest zero)."""
+    r = StepRecord()
+    for n, k in REC_FIELDS:
+        if n in kw:
+            v = kw[n]
+            if k > 1:
+                arr = getattr(r, n)
+                for i in range(k):
+                    arr[i] = float(v[i]) if i < len(v) else 0.0
+            else:
+                setattr(r, n, float(v))
+    return r
+
+
+def moduli_getters(nl, r0, r1=None, r2=None):
+    """The numerical moduli of EffPlast2D.cu:1412-1539 from step records
+    (dicts of record fields)."""
+    recs = [make_record(**(r or {})) for r in (r0, r1, r2)]
+    res = Result()
+    lib().ora_moduli_getters(int(nl), C.byref(recs[0]), C.byref(recs[1]), C.byref(recs[2]),
+                             C.byref(res))
+    return res.as_dict()
 
 
 def derive_params(Lx, Ly, nX, nPores, porosity, Y, K0=1.0, G0=0.5, rho0=1.0, CFL=0.5, nY=None):
@@ -267,6 +298,14 @@ class State:
         out.update({n: getattr(o, n) for n in AN_STATS})
         out["analytic"] = int(an)
         return out
+
+    def analytic_moduli(self, step, rec):
+        """Per-step analytic moduli (EffPlast2D.cu:643-696) of one record."""
+        self.sync_config()
+        res = Result()
+        r = make_record(**rec)
+        lib().ora_analytic_moduli(C.byref(self.config), int(step), C.byref(r), C.byref(res))
+        return res.as_dict()
 
     # -- drivers -------------------------------------------------------------
     def compute_eff_moduli(self, initLoad, loadType, nTimeSteps=1):

@@ -161,6 +161,32 @@ def test_batch_options_and_errors():
     L.ep2d_batch_destroy(None)
 
 
+def test_batch_request_bytes():
+    """The admission estimate covers the four displacement and four velocity
+    buffers of the fused schemes (ping-pong U, double-buffered V), the mask
+    byte, five cell and two vertex stress fields per RVE; FP32 velocities
+    make split smaller by 4 (nX+1) nX x 4 bytes; bad arguments are errors."""
+    pa = ora.derive_params(20.0, 20.0, 1024, 1, 0.01, 2.5e-3)
+    f = ep.request_bytes("fp64", pa, 1024)
+    s = ep.request_bytes("split", pa, 1024)
+    pts, nC, nV = 1025 * 1024, 1024 * 1024, 1023 * 1023
+    assert f - s == 4 * pts * 4
+    floor = 4 * pts * 8 + 4 * pts * 8 + nC * (1 + 5 * 8) + 2 * nV * 8
+    assert floor <= f <= floor + (1 << 20)
+    assert ep.request_bytes("split", pa, 2048) > 3.9 * s
+    L = _lib.lib()
+    o = _lib.BatchOptions()
+    L.ep2d_batch_default_options(C.byref(o))
+    n = C.c_size_t()
+    from effplast2d_amd.batch import make_request
+    rq = make_request(pa, 4)
+    assert L.ep2d_batch_request_bytes(C.byref(o), C.byref(rq), C.byref(n)) == -1
+    o.mode = _lib.MODES["plain"]
+    rq = make_request(pa, 64)
+    assert L.ep2d_batch_request_bytes(C.byref(o), C.byref(rq), C.byref(n)) == -1
+    assert L.ep2d_batch_request_bytes(None, None, None) == -1
+
+
 def test_cli_usage_error():
     """Missing arguments: the usage message and exit status 1 (main.cu:6-10,
     39-41), before any device work."""

@@ -91,3 +91,31 @@ def test_pins_reject_planted_mistakes(tmp_path, use_lib, mutant):
         assert gaps["matlab"] > 1e-3, gaps
         # and each is also caught by at least one closed form or invariant
         assert gaps["symmetry"] > 1e-6 or gaps["hooke"] > 1e-6, gaps
+
+
+# Planted mistakes in the post-processing (-DORA_MUTANT=5..15) and the
+# closed-form pin of tests/test_oracle_postproc_pins.py that must reject each.
+POSTPROC_MUTANTS = {
+    5: ("row 1 for row 0 in getDeltaP_honest", ["delta_p_honest"]),
+    6: ("column N(nP-1)/nP for N(nP-2)/nP in getDeltaP_periodic", ["delta_p_periodic"]),
+    7: ("integer y bound in getDeltaP_periodic", ["delta_p_periodic"]),
+    8: ("yy - xx on one line of getTauInfty_honest", ["delta_p_honest"]),
+    9: ("sigma[0] over the summed cells, not N^2", ["getters_uniform_state"]),
+    10: ("nu0 fold dropped from sigma[2]", ["getters_uniform_state"]),
+    11: ("measured strain in Kd", ["getters_records", "getters_uniform_state"]),
+    12: ("sign of the yy difference in G", ["getters_records", "getters_uniform_state"]),
+    13: ("-1 dropped from the plastic exponent", ["plastic_analytic_reduction"]),
+    14: ("right pore-edge window [-1, +3]", ["hole_area_windows"]),
+    15: ("magnitude for the signed element in FindMaxAbs", ["hole_area_windows"]),
+}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+@pytest.mark.parametrize("mutant", sorted(POSTPROC_MUTANTS))
+def test_postprocessing_pins_reject_planted_mistakes(tmp_path, use_lib, mutant):
+    from tests.test_oracle_postproc_pins import PINS
+    what, pins = POSTPROC_MUTANTS[mutant]
+    use_lib(build(str(tmp_path), mutant))
+    for name in pins:
+        with pytest.raises(AssertionError):
+            PINS[name]()
