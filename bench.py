@@ -53,6 +53,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+from effplast2d_amd.workloads import ITER_CAP_PER_N, mixed_requests, serving_stream  # noqa: E402
+
 METRIC = ("converged RVE solves/s + cell-iter/s (% HBM peak), FP32-split vs FP64; p50/p99")
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per point per iteration: U and V read once and written once
@@ -158,12 +160,20 @@ def batch_stats_summary(st, mode):
     each replay of back-to-back launches of the chosen kernel) when graphs
     ran, else over the sampled single launches between dispatch events; a
     single bracketed launch also carries the dispatch latency of the launch
-    after a replay, so it reads longer than the kernel itself."""
+    after a replay, so it reads longer than the kernel itself.
+
+    Bytes per launch: a launch of an n-iteration shape reads and writes U and
+    V once for its n iterations, so the bytes it must move are 48 B (split) or
+    64 B (FP64) per point once -- `bytes_per_launch`, the roofline's
+    algorithmic bytes.  `effective_GBs` counts those bytes once per iteration
+    instead: the HBM rate a one-iteration kernel would need to match the
+    launch's iteration rate (above the peak for n > 1)."""
     nan = float("nan")
     ipl = {4: 2, 5: 2, 6: 3, 7: 4}.get(st["kernel"], 1)   # iterations per launch
     it_disp = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else nan
     it_graph = st["graphMs"] / st["graphLaunches"] if st["graphLaunches"] else None
-    bpl = ipl * st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"] else nan
+    # timedBytes counts the algorithmic bytes of each iteration
+    bytes_iter = st["timedBytes"] / st["timedLaunches"] if st["timedLaunches"] else nan
     kms = ipl * (it_graph if it_graph is not None else it_disp)   # ms per launch
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
@@ -176,8 +186,9 @@ def batch_stats_summary(st, mode):
             "kernel_samples": (st["graphLaunches"] if it_graph is not None
                                else st["timedLaunches"]) // ipl,
             "kernel_avg_us_dispatch": ipl * it_disp * 1e3,
-            "bytes_per_launch": bpl,
-            "achieved_GBs": bpl / (kms * 1e-3) / 1e9,
+            "bytes_per_launch": bytes_iter,
+            "achieved_GBs": bytes_iter / (kms * 1e-3) / 1e9,
+            "effective_GBs": ipl * bytes_iter / (kms * 1e-3) / 1e9,
             "launches": st["launches"], "chunks": st["chunks"],
             "graph_avg_us_per_launch": ipl * 1e3 * it_graph if it_graph is not None else None,
             "kernel_shape": shape_name(st["kernel"], st["kernelRows"]),
@@ -230,37 +241,6 @@ def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
            "converged": all(bool(r["result"]["converged"]) for r in res.values())}
     out.update(batch_stats_summary(st, mode))
     return out
-
-
-# Iteration cap per load step of the sampled RVEs: 40 N, about five times what
-# a converging RVE of size N needs (6.3 N for the preload at 1024^2).  Some
-# sampled loads exceed the limit load (the plastic zone never stops growing)
-# and would otherwise run to the 100000-iteration default.
-ITER_CAP_PER_N = 40
-
-
-def mixed_requests(n, seed):
-    """Batched sweep: sizes uniform in {128,256,512,1024}, porosity U[0.005,0.05],
-    nPores in {1,2,3}, Y/K0 U[1e-3,5e-3], preload magnitude U[1e-3,5e-3] along a
-    random unit direction in (Exx, Eyy, Exy); nTasks=3."""
-    import numpy as np
-    from effplast2d_amd import derive_params
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for k in range(n):
-        N = int(rng.choice([128, 256, 512, 1024]))
-        phi = float(rng.uniform(0.005, 0.05))
-        npores = int(rng.integers(1, 4))
-        Y = float(rng.uniform(1e-3, 5e-3))
-        mag = float(rng.uniform(1e-3, 5e-3))
-        u = rng.standard_normal(3)
-        u /= np.linalg.norm(u)
-        pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
-                               G0=0.5, rho0=1.0, CFL=0.5)
-        assert pa[9] < 20.0 / (2 * npores), "pore radius guard"
-        reqs.append(dict(pa=pa, nX=nx, nTasks=3, nIter=ITER_CAP_PER_N * nx, eIter=1e-8,
-                         initLoad=-mag, loadType=[float(x) for x in u], tag=k))
-    return reqs
 
 
 def run_elastic_128(steps=5):
@@ -316,30 +296,6 @@ def run_mixed_batch(n, seed, check):
     log(f"[bench] mixed batch {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
         f"{cells / dt / 1e9:.1f} G cell-iter/s, kernel {out['achieved_GBs']:.0f} GB/s")
     return out, res
-
-
-def serving_stream(rate, duration, seed):
-    import numpy as np
-    from effplast2d_amd import derive_params
-    rng = np.random.default_rng(seed)
-    t, reqs = 0.0, []
-    while True:
-        t += float(rng.exponential(1.0 / rate))
-        if t > duration:
-            break
-        N = int(rng.choice([256, 512, 1024, 2048], p=[0.4, 0.3, 0.2, 0.1]))
-        phi = float(rng.uniform(0.005, 0.05))
-        npores = int(rng.integers(1, 4))
-        Y = float(rng.uniform(1e-3, 5e-3))
-        mag = float(rng.uniform(1e-3, 5e-3))
-        u = rng.standard_normal(3)
-        u /= np.linalg.norm(u)
-        nl = int(rng.integers(1, 4))
-        pa, nx = derive_params(Lx=20.0, Ly=20.0, nX=N, nPores=npores, porosity=phi, Y=Y, K0=1.0,
-                               G0=0.5, rho0=1.0, CFL=0.5)
-        reqs.append((t, dict(pa=pa, nX=nx, nTasks=nl, nIter=ITER_CAP_PER_N * nx, eIter=1e-8,
-                             initLoad=-mag, loadType=[float(x) for x in u], tag=len(reqs))))
-    return reqs
 
 
 def plain_iteration_ms(nx, iters=200):
@@ -419,17 +375,19 @@ def run_serving(rate, duration, seed, check, budget_gib):
     return out
 
 
-def read_traffic(shape, mode, nx):
-    """Measured HBM bytes per launch of the dominant kernel in the given shape
-    (e.g. "march2x32"), from the committed rocprofv3 PMC summary of that
-    kernel (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated); None when no
-    summary of that shape is committed."""
-    path = os.path.join(ROOT, "profiles", f"pmc_batch_{shape}_{mode}_{nx}.txt")
+def profile_path(kind, shape, mode, nx, nrve):
+    """Committed counter summary of the batched kernel in `shape` over `nrve`
+    RVEs of nx^2 (kind "pmc": HBM traffic, "sq": instruction counters)."""
+    return os.path.join(ROOT, "profiles", f"{kind}_batch_{shape}_{mode}_{nx}_x{nrve}.txt")
+
+
+def read_profile_value(path, key):
+    """`key = value` line of a committed summary; None when absent."""
     if not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch"):
+            if line.startswith(key + " ="):
                 return float(line.split("=")[1].split()[0])
     return None
 
@@ -772,7 +730,12 @@ def main():
         legs["serving"] = run_serving(a.serve_rate, a.serve_seconds, a.serve_seed,
                                               a.check_interval, a.serve_budget_gib)
     frac = head["achieved_GBs"] / HBM_PEAK_GBS
-    traffic = read_traffic(head["kernel_shape"], a.mode, nx)
+    # HBM traffic and instruction counters of this kernel shape at this batch
+    # size, from the committed rocprofv3 summaries (None when none matches)
+    pmc = profile_path("pmc", head["kernel_shape"], a.mode, nx, a.steps)
+    sq = profile_path("sq", head["kernel_shape"], a.mode, nx, a.steps)
+    traffic = read_profile_value(pmc, "traffic_bytes_per_launch")
+    valu_busy = read_profile_value(sq, "valu_busy_frac")
     out = {
         "metric": METRIC,
         "value": head["solves_per_s"],
@@ -806,12 +769,19 @@ def main():
                      "kernel_samples": head["kernel_samples"],
                      "kernel_shape": head["kernel_shape"],
                      "iterations_per_launch": head["iterations_per_launch"],
-                     # HBM bytes the launch moves (traffic) over its duration:
-                     # a multi-iteration launch reads and writes each field
-                     # once for all its iterations, so `achieved` (48 B per
-                     # point and iteration) can exceed the peak
+                     # achieved: the bytes the launch must move (U and V read
+                     # and written once for all its iterations) over its
+                     # duration; effective: the same bytes counted once per
+                     # iteration (the rate a one-iteration kernel would need)
+                     "effective_GBs": head["effective_GBs"],
+                     "effective_frac": head["effective_GBs"] / HBM_PEAK_GBS,
+                     # measured HBM bytes per launch over its duration
                      "traffic_GBs": (traffic / (head["kernel_avg_us"] * 1e-6) / 1e9
-                                     if traffic else None)},
+                                     if traffic else None),
+                     "traffic_profile": os.path.relpath(pmc, ROOT) if traffic else None,
+                     # fraction of SIMD cycles issuing VALU instructions
+                     "valu_busy_frac": valu_busy,
+                     "valu_profile": os.path.relpath(sq, ROOT) if valu_busy else None},
         "cell_iter_per_s": head["cell_iter_per_s"],
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],

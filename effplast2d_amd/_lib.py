@@ -170,6 +170,9 @@ SIGNATURES = {
     "ep2d_write_pa": ([C.c_char_p, _dp], C.c_int),
     "ep2d_default_options": ([C.POINTER(Options)], None),
     "ep2d_create": ([_dp, C.c_int, C.c_int, C.POINTER(Options), C.POINTER(C.c_void_p)], C.c_int),
+    "ep2d_create_on_stream": ([_dp, C.c_int, C.c_int, C.POINTER(Options), C.c_void_p,
+                               C.POINTER(C.c_void_p)], C.c_int),
+    "ep2d_field_device": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
     "ep2d_create_from_physics": ([C.POINTER(Physics), C.POINTER(Options),
                                   C.POINTER(C.c_void_p)], C.c_int),
     "ep2d_destroy": ([C.c_void_p], None),

@@ -147,6 +147,11 @@ void ep2d_default_options(ep2d_options* o) {
 
 int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options* opt,
                 ep2d_handle** out) {
+  return ep2d_create_on_stream(pa, nX, nY, opt, nullptr, out);
+}
+
+int ep2d_create_on_stream(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options* opt,
+                          void* stream, ep2d_handle** out) {
   return guarded([&] {
     if (!pa || !out) throw ep2d::ArgError("null argument");
     *out = nullptr;
@@ -154,7 +159,7 @@ int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY, const ep2d_options*
     if (opt) o = *opt; else ep2d_default_options(&o);
     auto* h = new ep2d_handle{nullptr};
     try {
-      h->solver = new ep2d::Solver(pa, nX, nY, o);
+      h->solver = new ep2d::Solver(pa, nX, nY, o, static_cast<hipStream_t>(stream));
     } catch (...) {
       delete h;
       throw;
@@ -285,6 +290,15 @@ int ep2d_residual(ep2d_handle* h, const double curEffStrain[3], double* error, l
   return guarded([&] {
     if (!curEffStrain || !error) throw ep2d::ArgError("null argument");
     *error = S(h)->residual(curEffStrain, nonfinite);
+  });
+}
+
+int ep2d_field_device(ep2d_handle* h, int field, const void** ptr) {
+  return guarded([&] {
+    if (!ptr) throw ep2d::ArgError("null argument");
+    *ptr = nullptr;
+    if (field < 0 || field >= EP2D_FIELD_COUNT) throw ep2d::ArgError("unknown field");
+    *ptr = S(h)->fieldDevice(field);
   });
 }
 

@@ -2336,7 +2336,7 @@ void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, in
 
 const char* fused_kernel_name(int mode) {
   switch (mode) {
-    case 0: return "k_plain_disp";
+    case 0: return "k_plain_stress + k_plain_j2 + k_plain_plasticity + k_plain_disp";
     case 1: return "k_fused<double, double, true, TY>";
     default: return "k_fused<float, float, true, TY>";
   }

@@ -489,14 +489,16 @@ void Solver::launchIteration() {
   // which record the dispatch's execution window
   if (mode_ == EP2D_MODE_PLAIN) {
     // EffPlast2D.cu:390-399 (the reference synchronises the device after each
-    // launch; stream order gives the same dependencies)
+    // launch; stream order gives the same dependencies).  A timed iteration
+    // is bracketed by events on the stream around all four kernels.
+    if (timed) EP2D_HIP(hipEventRecord(ev.first, stream_));
     launch_plain_stress(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
     if (plastic) {
       launch_plain_j2(r_, f_, stream_);
       launch_plain_plasticity(r_, f_, stream_);
     }
-    launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_, timed ? ev.first : nullptr,
-                      timed ? ev.second : nullptr);
+    launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
+    if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
   } else {
     if (marchRows_ > 0)
       launch_march(dRve_, dSegs_, nSegs_, mode_, cur_, vIdx_, stream_,
@@ -666,20 +668,25 @@ void Solver::averages(ep2d_record& r) {
   const int poy = nY_ / nPores_;
   launch_averages(r_, f_, r_.Ux[cur_], r_.Uy[cur_], nu0_, pox, poy, dAvgPartial_, dAvg_, stream_);
   d2h(hAvg_, dAvg_, NAVG);
-  const long nX = nX_, nY = nY_;
-  r.sigma[0] = hAvg_[A_S0] / (nX * nY);
-  r.sigma[1] = hAvg_[A_S1] / (nX * nY);
-  r.sigma[2] = hAvg_[A_S2] / ((nX - 1) * (nY - 1));
+  finish_averages(hAvg_, nX_, nY_, pox, poy, r);
+}
+
+// The reference's normalisations of the sums (EffPlast2D.cu:491-503): sigma
+// over nX*nY (0, 1) and (nX-1)*(nY-1) (2), the periodic window over its area.
+void finish_averages(const double* sums, long nX, long nY, int pox, int poy, ep2d_record& r) {
+  r.sigma[0] = sums[A_S0] / (nX * nY);
+  r.sigma[1] = sums[A_S1] / (nX * nY);
+  r.sigma[2] = sums[A_S2] / ((nX - 1) * (nY - 1));
   r.sigma[3] = 0.0 / (nX * nY);
   const long perArea = (nX - 2 * pox) * (nY - 2 * poy);
-  r.sigmaPer[0] = hAvg_[A_SP0] / perArea;
-  r.sigmaPer[1] = hAvg_[A_SP1] / perArea;
-  r.sigmaPer[2] = hAvg_[A_SP2] / perArea;
-  r.sigmaPer[3] = hAvg_[A_SP3] / perArea;
-  r.epsilonPer[0] = hAvg_[A_E0] / perArea;
-  r.epsilonPer[1] = hAvg_[A_E1] / perArea;
-  r.epsilonPer[2] = hAvg_[A_E2] / perArea;
-  r.sigmaXYclean = hAvg_[A_SXY] / (double)((nX - 1) * (nY - 1));
+  r.sigmaPer[0] = sums[A_SP0] / perArea;
+  r.sigmaPer[1] = sums[A_SP1] / perArea;
+  r.sigmaPer[2] = sums[A_SP2] / perArea;
+  r.sigmaPer[3] = sums[A_SP3] / perArea;
+  r.epsilonPer[0] = sums[A_E0] / perArea;
+  r.epsilonPer[1] = sums[A_E1] / perArea;
+  r.epsilonPer[2] = sums[A_E2] / perArea;
+  r.sigmaXYclean = sums[A_SXY] / (double)((nX - 1) * (nY - 1));
 }
 
 // getDeltaP_honest (:825-853), getDeltaP_periodic (:854-882),
@@ -1266,6 +1273,29 @@ void Solver::getField(int field, double* host, size_t count) {
       return;
     }
   }
+}
+
+// Device buffer of a field in the current state (stress fields finalised
+// first); the stream is synchronised, so the buffer is ready for any stream.
+const void* Solver::fieldDevice(int field) {
+  EP2D_HIP(hipSetDevice(device_));
+  const void* p = nullptr;
+  switch (field) {
+    case EP2D_FIELD_P: ensureStress(); p = f_.P; break;
+    case EP2D_FIELD_TAUXX: ensureStress(); p = f_.tauXX; break;
+    case EP2D_FIELD_TAUYY: ensureStress(); p = f_.tauYY; break;
+    case EP2D_FIELD_TAUXY: ensureStress(); p = f_.tauXY; break;
+    case EP2D_FIELD_TAUXYAV: ensureStress(); p = f_.tauXYav; break;
+    case EP2D_FIELD_J2: ensureStress(); p = f_.J2; break;
+    case EP2D_FIELD_J2XY: ensureStress(); p = f_.J2XY; break;
+    case EP2D_FIELD_UX: p = r_.Ux[cur_]; break;
+    case EP2D_FIELD_UY: p = r_.Uy[cur_]; break;
+    case EP2D_FIELD_VX: p = r_.Vx; break;
+    case EP2D_FIELD_VY: p = r_.Vy; break;
+    default: throw ArgError("K and G are not stored on the device (the mask selects them)");
+  }
+  EP2D_HIP(hipStreamSynchronize(stream_));
+  return p;
 }
 
 void Solver::setField(int field, const double* host, size_t count) {

@@ -22,6 +22,9 @@ struct StatusError : std::runtime_error {
 
 // Derivation of pa[] from the physical inputs (get_sigma_2D.m:3-32).
 void derive_params(const ep2d_physics& ph, double pa[EP2D_NPARS], int* nX);
+// Volume averages of a record from the NAVG device sums (EffPlast2D.cu:491-503);
+// pox, poy: the periodic window offsets nX / nPores, nY / nPores.
+void finish_averages(const double* sums, long nX, long nY, int pox, int poy, ep2d_record& r);
 
 // Device and pinned host blocks kept for reuse, by exact size.  A batch
 // engine's solvers take their buffers from it and give them back when they
@@ -110,6 +113,7 @@ class Solver {
   // host U zeroing test of the reference (EffPlast2D.cu:517-547)
   bool uNodeInPore(bool isUy, long i, long j) const;
   void setField(int field, const double* host, size_t count);
+  const void* fieldDevice(int field);
   void getMask(unsigned char* host, size_t count);
 
   void setTiming(long sampleEvery);

@@ -50,7 +50,7 @@ class EffPlast2D:
     def __init__(self, pa, nX, nTimeSteps=1, nTasks=3, nIter=100_000, eIter=1e-10,
                  checkInterval=10_000, mode="fp64", device=0, shearLoadType=None,
                  verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=0,
-                 adaptiveCheck=False):
+                 adaptiveCheck=False, stream=None):
         self.pa = np.ascontiguousarray(pa, dtype=np.float64)
         if self.pa.shape != (L.NPARS,):
             raise ValueError("pa must hold 11 values")
@@ -65,8 +65,9 @@ class EffPlast2D:
                                      itersPerLaunch=itersPerLaunch,
                                      adaptiveCheck=1 if adaptiveCheck else 0)
         h = C.c_void_p()
-        check(lib().ep2d_create(_dptr(self.pa), self.nX, self.nY, C.byref(self._opts),
-                                C.byref(h)))
+        # stream: a hipStream_t (as an integer) the handle orders its work on
+        check(lib().ep2d_create_on_stream(_dptr(self.pa), self.nX, self.nY, C.byref(self._opts),
+                                          C.c_void_p(stream) if stream else None, C.byref(h)))
         self._h = h
         L.track(self)
 
@@ -135,6 +136,12 @@ class EffPlast2D:
         out = np.empty(m * n, dtype=np.float64)
         check(lib().ep2d_get_field(self._h, L.FIELD_ID[name], _dptr(out), out.size))
         return out
+
+    def device_ptr(self, name):
+        """Device address of a field of the current state (ep2d_field_device)."""
+        p = C.c_void_p()
+        check(lib().ep2d_field_device(self._h, L.FIELD_ID[name], C.byref(p)))
+        return p.value
 
     def set(self, name, values):
         m, n = self.field_shape(name)

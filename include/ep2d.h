@@ -173,6 +173,14 @@ void ep2d_default_options(ep2d_options* opt);
  * field under 4 GiB). */
 int ep2d_create(const double pa[EP2D_NPARS], int nX, int nY,
                 const ep2d_options* opt, ep2d_handle** out);
+/* The same, with every piece of the handle's device work ordered on the
+ * caller's stream (a hipStream_t of device opt->device, passed as void*),
+ * which the handle uses but does not own or destroy: the reference's
+ * kernel-launch boundary (EffPlast2D.cu:390-399) with an explicit stream
+ * instead of the legacy default stream and its device-wide synchronisation.
+ * Calls that return host results synchronise that stream. */
+int ep2d_create_on_stream(const double pa[EP2D_NPARS], int nX, int nY,
+                          const ep2d_options* opt, void* stream, ep2d_handle** out);
 /* Convenience: derive pa[] from phys, then ep2d_create. */
 int ep2d_create_from_physics(const ep2d_physics* phys, const ep2d_options* opt,
                              ep2d_handle** out);
@@ -201,6 +209,12 @@ int ep2d_get_field(ep2d_handle* h, int field, double* host, size_t count);
 int ep2d_set_field(ep2d_handle* h, int field, const double* host, size_t count);
 /* SaveMatrix (EffPlast2D.cu:795-799): raw FP64 dump of one field. */
 int ep2d_save_field(ep2d_handle* h, int field, const char* path);
+/* Device buffer of a field of the current state in the layout above (FP64;
+ * Vx, Vy FP32 in EP2D_MODE_SPLIT), valid until the handle's next call that
+ * iterates or is destroyed: the reference's raw device fields (the kernel
+ * arguments of EffPlast2D.cu:3-177).  Stress fields are finalised first and
+ * the handle's stream is synchronised.  K and G are not stored (EP2D_ERR_ARG). */
+int ep2d_field_device(ep2d_handle* h, int field, const void** ptr);
 /* Pore masks: bit0 SetMaterials pore (K, G reduced, :708-728), bit1
  * ComputeStress cell zeroing (:77-87), bit2 vertex zeroing (:88-95). */
 int ep2d_get_mask(ep2d_handle* h, unsigned char* host, size_t count);

@@ -1,20 +1,28 @@
-"""GPU: the batched multi-RVE engine against the single-RVE path.
+"""GPU: the batched multi-RVE engine against the single-RVE path and the
+CPU oracle.
 
 An RVE's arithmetic in the batched launch is the single-RVE kernel's, so its
 result must not depend on the other RVEs it shares launches with: moduli,
 per-step records and iteration counts are compared for exact equality with
 the single-RVE solve of the same request and check interval ("alone vs in the
-mix"), with and without graph-captured chunks, and under an admission budget
-that forces retirement and backfill.  The fp64 batch is also checked against
-the CPU oracle (relative 1e-8 on the moduli, iteration counts equal).
+mix"), with and without graph-captured chunks, and under admission caps (a
+count, and a device-memory budget in bytes with requests arriving while
+others run) that force retirement and backfill.  Seeded sampled requests
+(mixed sizes, porosities, pore counts, yield stresses, nTasks 1 to 3 and
+preloads along random directions in (Exx, Eyy, Exy), so the shear term of
+the affine increment is exercised) are checked against the CPU oracle: the
+FP64 batch within 1e-8 relative on the moduli with equal iteration counts,
+the split batch within 1e-4 relative of the FP64 oracle on Kd, Kphi and G
+with iteration counts within 10%.
 """
 import sys
 
 import numpy as np
 import pytest
 
-from effplast2d_amd import Batch, EffPlast2D
-from tests.pairs import make_pa
+from effplast2d_amd import Batch, EffPlast2D, request_bytes
+from effplast2d_amd.workloads import mixed_requests
+from tests.pairs import make_pa, oracle_state
 
 pytestmark = pytest.mark.gpu
 
@@ -249,3 +257,131 @@ def test_chains_full_size_equal_tiles(mode):
         assert st["kernel"] == (6 if "tripleRows" in shape else 7)
         _same(got["result"], ref["result"])
         assert got["result"]["totalIters"] == 900
+
+
+# ---- sampled requests against the oracle --------------------------------------------
+def _sampled():
+    """Eight seeded requests at 128^2 and 192^2, nTasks 1 to 3, preload along a
+    random unit direction in (Exx, Eyy, Exy)."""
+    return mixed_requests(8, 31, sizes=(128, 192), tasks=(1, 2, 3))
+
+
+@pytest.fixture(scope="module")
+def sampled_oracle():
+    out = {}
+    for rq in _sampled():
+        s = oracle_state(rq["nX"], rq["pa"], rq["nTasks"], None, nIter=rq["nIter"],
+                         eIter=rq["eIter"], outputStep=K)
+        res, recs = s.compute_eff_moduli(rq["initLoad"], rq["loadType"], 1)
+        out[rq["tag"]] = (res, [r for st in recs for r in st])
+    return out
+
+
+def _run_sampled(mode, **kw):
+    out = {}
+    with Batch(mode=mode, checkInterval=K, **kw) as b:
+        for rq in _sampled():
+            b.submit(**rq)
+        b.run()
+        for r in b.poll():
+            out[r["tag"]] = r
+    return out
+
+
+def test_sampled_requests_have_shear_preloads():
+    reqs = _sampled()
+    assert sum(abs(rq["loadType"][2]) > 0.1 for rq in reqs) >= 4
+    assert {rq["nTasks"] for rq in reqs} == {1, 2, 3}
+    assert len({rq["nX"] for rq in reqs}) == 2
+
+
+def test_fp64_batch_matches_oracle(sampled_oracle):
+    got = _run_sampled("fp64")
+    for tag, (res_o, recs_o) in sampled_oracle.items():
+        r = got[tag]
+        assert r["status"] == 0, r["message"]
+        res_g = r["result"]
+        _note(f"fp64 batch tag={tag} iters={res_g['totalIters']}")
+        assert res_g["totalIters"] == sum(int(x["iters"]) for x in recs_o)
+        for grp in ("num", "num_per", "an"):
+            for k in ("Kphi", "Kd", "G"):
+                a, b = res_g[grp][k], res_o[grp][k]
+                if np.isfinite(b) and b != 0:
+                    assert a == pytest.approx(b, rel=1e-8), (tag, grp, k)
+
+
+def test_split_batch_close_to_fp64_oracle(sampled_oracle):
+    """Converged requests: Kd, Kphi and G within 1e-4 of the FP64 oracle;
+    iteration counts within 10% (plus one check interval)."""
+    got = _run_sampled("split")
+    checked = 0
+    for tag, (res_o, recs_o) in sampled_oracle.items():
+        res_g = got[tag]["result"]
+        it_o = sum(int(x["iters"]) for x in recs_o)
+        assert abs(res_g["totalIters"] - it_o) <= 0.1 * it_o + K, tag
+        if not (res_g["converged"] and all(x["error"] < 1e-8 for x in recs_o)):
+            continue
+        checked += 1
+        for k in ("Kd", "Kphi", "G"):
+            a, b = res_g["num"][k], res_o["num"][k]
+            if np.isfinite(b) and b != 0:
+                e = abs(a - b) / abs(b)
+                _note(f"split batch tag={tag} {k}: rel {e:.2e}")
+                assert e <= 1e-4, (tag, k)
+    assert checked >= 4
+
+
+def test_serving_byte_budget_admission_and_backfill():
+    """Continuous batching under a device-memory budget in bytes that holds two
+    RVEs: two requests start, two more arrive after two chunks while the first
+    two run; the late ones are admitted only as memory frees up, the engine
+    never holds more than the budget, and every result equals the request's
+    solve alone."""
+    reqs = mixed_requests(4, 77, sizes=(96, 128), tasks=(1, 2, 3))
+    need = [request_bytes("split", rq["pa"], rq["nX"]) for rq in reqs]
+    budget = sorted(need)[-1] + sorted(need)[-2] + min(need) // 2
+    _note(f"serving budget {budget} bytes, needs {need}")
+    done, peak = {}, 0
+    with Batch(mode="split", checkInterval=K, memoryBudget=budget) as b:
+        for rq in reqs[:2]:
+            b.submit(**rq)
+        steps = 0
+        while len(done) < len(reqs):
+            if steps == 2:
+                for rq in reqs[2:]:
+                    b.submit(**rq)
+            active, queued = b.step()
+            st = b.stats()
+            assert st["deviceBytes"] <= budget
+            assert active <= 2
+            peak = max(peak, active)
+            for r in b.poll():
+                done[r["tag"]] = r
+            steps += 1
+            assert steps < 10_000
+    assert peak == 2
+    first_done = min(done[0]["tDone"], done[1]["tDone"])
+    for t in (2, 3):
+        assert done[t]["tSubmit"] > done[0]["tAdmit"]       # arrived mid-flight
+        assert done[t]["tAdmit"] >= first_done               # waited for memory
+    for rq in reqs:
+        alone = _run_alone(rq)
+        _same(done[rq["tag"]]["result"], alone)
+
+
+def _run_alone(rq, mode="split"):
+    with EffPlast2D(rq["pa"], rq["nX"], nTasks=rq["nTasks"], nIter=rq["nIter"],
+                    eIter=rq["eIter"], checkInterval=K, mode=mode) as g:
+        return g.ComputeEffModuli(rq["initLoad"], rq["loadType"])
+
+
+def test_admission_estimate_is_what_an_rve_holds():
+    """The bytes counted at admission are the bytes an admitted RVE holds."""
+    reqs = mixed_requests(3, 5, sizes=(64, 96, 160))
+    with Batch(mode="fp64", checkInterval=K) as b:
+        for rq in reqs:
+            b.submit(**rq)
+        b.step()
+        st = b.stats()
+        assert st["active"] == 3
+        assert st["deviceBytes"] == sum(request_bytes("fp64", rq["pa"], rq["nX"]) for rq in reqs)

@@ -1,0 +1,230 @@
+"""GPU parity at the workload sizes: the 4096^2 RVE with 16 pores, a
+converged 1024^2 solve, ComputeEffDamping, and the closed-form moduli of a
+uniform no-pore state, through the C ABI against the CPU oracle (or against
+closed forms that do not go through the oracle).
+
+Tolerances (per test):
+  * FP64 schemes (plain, tiles, marching, wave pairs and chains): fields
+    after a fixed number of iterations <= 1e-12 relative (max-norm, bit for
+    bit in practice); converged moduli <= 1e-8 relative; iteration counts
+    equal.
+  * split scheme against the FP64 oracle: displacements <= 1e-6 relative
+    (max-norm), velocity and stress fields <= 1e-4 relative L2 after a fixed
+    number of iterations; converged Kd, Kphi and G <= 1e-4 relative, field
+    relative L2 <= 1e-3, iteration counts within 10%.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from effplast2d_amd import EffPlast2D
+from oracle import ora
+from tests.pairs import (PLASTIC_FIELDS, STATE_FIELDS, STRESS_FIELDS, l2_rel, make_pa,
+                         oracle_state, random_state, rel_err)
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# ---- the 4096^2 RVE with 4 x 4 pores, porosity 0.05, Y = 2.5e-3 ---------------------
+N4096 = 4096
+
+
+def _rve4096_start():
+    pa = make_pa(N4096, nPores=4, porosity=0.05, Y=2.5e-3)
+    init = random_state(N4096, 4096, pa=pa, boundary_v=True)
+    return pa, init
+
+
+@pytest.fixture(scope="module")
+def rve4096_oracle():
+    """30 plastic iterations of the oracle from a noisy affine compression."""
+    pa, init = _rve4096_start()
+    s = oracle_state(N4096, pa, 3, init, nThreads=THREADS)
+    s.iterate(30)
+    # the return mapping must have been exercised
+    assert np.count_nonzero(s["J2"] > pa[ora.PA_COH] * (1 - 1e-12)) > 1000
+    out = {k: s[k].copy() for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS}
+    del s
+    return pa, init, out
+
+
+@pytest.mark.parametrize("kw", [{}, {"tileRows": 8}, {"marchRows": 56, "itersPerLaunch": 3}],
+                         ids=["auto", "tiles8", "chain3x56"])
+def test_rve4096_fp64_matches_oracle(rve4096_oracle, kw):
+    """FP64 at 4096^2 with 16 pores: 30 plastic iterations, every field against
+    the oracle at <= 1e-12 relative."""
+    pa, init, ref = rve4096_oracle
+    _note(f"rve4096 fp64 {kw}")
+    with EffPlast2D(pa, N4096, nTasks=3, mode="fp64", **kw) as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(30)
+        for k in STATE_FIELDS + STRESS_FIELDS + PLASTIC_FIELDS:
+            got = g.get(k)
+            e = rel_err(got, ref[k])
+            assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != ref[k])} differ)"
+
+
+def test_rve4096_split_fields_close_to_oracle():
+    """Split at 4096^2 (automatic shape: three-wave chains): after 30 plastic
+    iterations from the same start (V rounded to FP32 for both), U within
+    1e-6 relative and V and the stresses within 1e-4 relative L2 of FP64."""
+    pa, init = _rve4096_start()
+    _note("rve4096 split")
+    for k in ("Vx", "Vy"):
+        init[k] = init[k].astype(np.float32).astype(np.float64)
+    s = oracle_state(N4096, pa, 3, init, nThreads=THREADS)
+    s.iterate(30)
+    with EffPlast2D(pa, N4096, nTasks=3, mode="split") as g:
+        for k in STATE_FIELDS:
+            g.set(k, init[k])
+        g.iterate(30)
+        for k in ("Ux", "Uy"):
+            e = rel_err(g.get(k), s[k])
+            _note(f"  {k}: max rel {e:.2e}")
+            assert e <= 1e-6, k
+        for k in ("Vx", "Vy") + tuple(STRESS_FIELDS):
+            e = l2_rel(g.get(k), s[k])
+            _note(f"  {k}: L2 rel {e:.2e}")
+            assert e <= 1e-4, k
+
+
+# ---- a converged 1024^2 solve (the three-task elastoplastic RVE) ---------------------
+HEAD = dict(nX=1024, nPores=1, porosity=0.01, Y=2.5e-3, load=-3e-3, lt=[1.0, 1.0, 0.0],
+            shear=(0.0, 0.0, 1.0), check=500)
+
+
+@pytest.fixture(scope="module")
+def converged_1024_oracle():
+    h = HEAD
+    pa = make_pa(h["nX"], nPores=h["nPores"], porosity=h["porosity"], Y=h["Y"])
+    s = oracle_state(h["nX"], pa, 3, None, nIter=100_000, eIter=1e-8, outputStep=h["check"],
+                     shearLoadType=h["shear"], nThreads=THREADS)
+    res, recs = s.compute_eff_moduli(h["load"], h["lt"], 1)
+    fields = {k: s[k].copy() for k in ("P", "tauXX", "Ux", "Uy")}
+    return pa, res, [r for st in recs for r in st], fields
+
+
+def _solve_1024(pa, mode, **kw):
+    h = HEAD
+    with EffPlast2D(pa, h["nX"], nTasks=3, nIter=100_000, eIter=1e-8, checkInterval=h["check"],
+                    mode=mode, shearLoadType=h["shear"], **kw) as g:
+        res = g.ComputeEffModuli(h["load"], h["lt"])
+        recs = g.records()
+        fields = {k: g.get(k) for k in ("P", "tauXX", "Ux", "Uy")}
+    return res, recs, fields
+
+
+@pytest.mark.parametrize("kw", [{}, {"marchRows": 26, "itersPerLaunch": 2}],
+                         ids=["auto", "pairs26"])
+def test_converged_1024_fp64_matches_oracle(converged_1024_oracle, kw):
+    """The 1024^2 three-task solve run to eIter = 1e-8 (checks every 500
+    iterations): per-step iteration counts equal, records and moduli within
+    1e-8 of the oracle, final fields within 1e-12."""
+    pa, res_o, recs_o, f_o = converged_1024_oracle
+    _note(f"converged 1024 fp64 {kw}")
+    res_g, recs_g, f_g = _solve_1024(pa, "fp64", **kw)
+    assert [r["iters"] for r in recs_g] == [r["iters"] for r in recs_o]
+    assert all(r["error"] < 1e-8 for r in recs_o)
+    for rg, ro in zip(recs_g, recs_o):
+        for k in ["deltaP", "deltaPper", "tauInfty", "dPhi", "holeAreaPi", "sigmaXYclean"]:
+            assert rg[k] == pytest.approx(ro[k], rel=1e-10, abs=1e-300), k
+    for grp in ["num", "an"]:
+        for k in ["Kphi", "Kd"]:
+            assert res_g[grp][k] == pytest.approx(res_o[grp][k], rel=1e-8), (grp, k)
+    assert res_g["Gxy"] == pytest.approx(res_o["Gxy"], rel=1e-8)
+    for k in f_o:
+        assert rel_err(f_g[k], f_o[k]) <= 1e-12, k
+
+
+def test_converged_1024_split_within_tolerance(converged_1024_oracle):
+    """The split scheme on the same solve: Kd, Kphi and G (Gxy, the shear task
+    being a pure xy shear) within 1e-4 of the FP64 oracle, iteration counts
+    within 10%, fields within 1e-3 relative L2."""
+    pa, res_o, recs_o, f_o = converged_1024_oracle
+    _note("converged 1024 split")
+    res_g, recs_g, f_g = _solve_1024(pa, "split")
+    for rg, ro in zip(recs_g, recs_o):
+        assert abs(rg["iters"] - ro["iters"]) <= 0.1 * ro["iters"]
+    for k in ("Kphi", "Kd"):
+        e = abs(res_g["num"][k] - res_o["num"][k]) / abs(res_o["num"][k])
+        _note(f"  {k}: rel {e:.2e}")
+        assert e <= 1e-4, k
+    e = abs(res_g["Gxy"] - res_o["Gxy"]) / abs(res_o["Gxy"])
+    _note(f"  Gxy: rel {e:.2e}")
+    assert e <= 1e-4
+    for k in f_o:
+        assert l2_rel(f_g[k], f_o[k]) <= 1e-3, k
+
+
+# ---- ComputeEffDamping (EffPlast2D.cu:230-310) ---------------------------------------
+@pytest.mark.parametrize("mode", ["plain", "fp64"])
+@pytest.mark.parametrize("nl", [5, 9])
+def test_eff_damping_matches_oracle(mode, nl):
+    """NL - 1 sinusoidal load increments after the preload (loadValue = 1e-3):
+    per-step applied strain, averaged stress and strain, periodic averages and
+    iteration counts against the oracle."""
+    nX = 64
+    _note(f"damping {mode} nl={nl}")
+    pa = make_pa(nX, nPores=1, porosity=0.01, Y=2.5e-3)
+    s = oracle_state(nX, pa, nl, None, nIter=100_000, eIter=1e-8, outputStep=100)
+    res_o, recs_o = s.compute_eff_damping(-2e-3, [1.0, 1.0, 0.0], 1e-3, 1)
+    flat_o = [r for st in recs_o for r in st]
+    with EffPlast2D(pa, nX, nTasks=nl, nIter=100_000, eIter=1e-8, checkInterval=100,
+                    mode=mode) as g:
+        g.ComputeEffDamping(-2e-3, [1.0, 1.0, 0.0], 1e-3)
+        recs_g = g.records()
+    assert len(recs_g) == len(flat_o) == nl
+    # the increments follow sin(2 pi i / (NL-1)) (pi as 3.1415926): the strain
+    # returns to the preload after a full period
+    assert flat_o[-1]["epsilon"][0] == pytest.approx(flat_o[0]["epsilon"][0], rel=1e-6)
+    for rg, ro in zip(recs_g, flat_o):
+        assert rg["iters"] == ro["iters"]
+        for k in ("epsilon", "epsilonPer", "sigma", "sigmaPer"):
+            np.testing.assert_allclose(rg[k], ro[k], rtol=1e-10, atol=1e-18, err_msg=k)
+        for k in ("deltaP", "tauInfty", "dPhi"):
+            assert rg[k] == pytest.approx(ro[k], rel=1e-10, abs=1e-300), k
+
+
+# ---- uniform no-pore state: closed-form moduli on the product path -------------------
+@pytest.mark.parametrize("mode", ["plain", "fp64", "split"])
+def test_uniform_state_moduli_closed_form(mode):
+    """No pores, Y = 1 (no yield), 48^2 with nPores = 3 for the periodic
+    window: every load step is a uniform Hooke state, and the reference's
+    getters give (EffPlast2D.cu:453-503, 1483-1539), with r1 = (N-2)^2/N^2,
+    r2 = (N-2)^2/(N-1)^2 and nu0 = (1.5 K0 - G0)/(3 K0 + G0):
+      Kd = 0.33333333 (r1 (2 G0/3 + 2 K0) + 2 r2 nu0 (G0/3 + K0)),
+      KdPer = 0.33333333 (2 G0/3 + 2 K0),
+      G = r1 G0 and GPer = G0 under the shear task [1, -1, 0],
+      Gxy = GxyPer = G0 under [0, 0, 1]."""
+    n = 48
+    _note(f"uniform state {mode}")
+    pa = make_pa(n, nPores=3, porosity=0.0, Y=1.0)
+    K0, G0 = pa[ora.PA_K0], pa[ora.PA_G0]
+    nu0 = (1.5 * K0 - G0) / (3.0 * K0 + G0)
+    r1, r2 = (n - 2) ** 2 / n ** 2, (n - 2) ** 2 / (n - 1) ** 2
+    tol = 1e-9 if mode != "split" else 1e-5
+    res = {}
+    for shear in ((1.0, -1.0, 0.0), (0.0, 0.0, 1.0)):
+        with EffPlast2D(pa, n, nTasks=3, nIter=200, eIter=1e-8, checkInterval=10, mode=mode,
+                        shearLoadType=shear) as g:
+            res[shear] = g.ComputeEffModuli(-2e-3, [1.0, 1.0, 0.0])
+            assert all(r["iters"] == 10 for r in g.records())
+    r = res[(1.0, -1.0, 0.0)]
+    kd = 0.33333333 * (r1 * (2 * G0 / 3 + 2 * K0) + 2 * r2 * nu0 * (G0 / 3 + K0))
+    assert r["num"]["Kd"] == pytest.approx(kd, rel=tol)
+    assert r["num_per"]["Kd"] == pytest.approx(0.33333333 * (2 * G0 / 3 + 2 * K0), rel=tol)
+    assert r["num"]["G"] == pytest.approx(r1 * G0, rel=tol)
+    assert r["num_per"]["G"] == pytest.approx(G0, rel=tol)
+    r = res[(0.0, 0.0, 1.0)]
+    assert r["Gxy"] == pytest.approx(G0, rel=tol)
+    assert r["GxyPer"] == pytest.approx(G0, rel=tol)

@@ -8,9 +8,8 @@ Tolerances (written per test):
     bit for bit -- and the masks, which are integer, exactly.
   * split scheme (FP32 stress/velocity, FP64 displacement): fields after a
     fixed number of iterations within 1e-4 relative (max-norm); the converged
-    moduli within 1e-4 relative of the FP64 solve where the moduli are not
-    a small difference quotient (Kd, G), field relative L2 <= 1e-3, iteration
-    counts within 10%.
+    moduli Kd, Kphi and G within 1e-4 relative of the FP64 solve, field
+    relative L2 <= 1e-3, iteration counts within 10%.
 """
 import sys
 
@@ -285,6 +284,9 @@ MODULI_CASES = [
     (64, 1, 0.005, 2.5e-3, 1, -1e-5, [1.0, 1.0, 0.0], 100),
     (96, 3, 0.05, 2.5e-3, 3, -3e-3, [1.0, 0.6, 0.2], 200),
     (64, 2, 0.03, 3e-3, 2, -2e-3, [0.8, 1.0, 0.0], 100),
+    # the 128^2 elastic RVE with one pore of porosity 0.005 under -1e-5
+    # isotropic preload (nGrid = 4), checked every 500 iterations
+    (128, 1, 0.005, 2.5e-3, 1, -1e-5, [1.0, 1.0, 0.0], 500),
 ]
 
 
@@ -322,19 +324,19 @@ def test_moduli_match_oracle(mode, case):
         assert rel_err(fields[k], s[k]) <= 1e-12, k
 
 
-@pytest.mark.parametrize("case", MODULI_CASES[:3])
+@pytest.mark.parametrize("case", MODULI_CASES)
 def test_split_moduli_close_to_fp64_oracle(case):
     _note(f"split moduli case={case}")
     res_o, recs_o, res_g, recs_g, fields, s = _moduli_pair(*case, "split")
     flat_o = [r for st in recs_o for r in st]
     for rg, ro in zip(recs_g, flat_o):
         assert abs(rg["iters"] - ro["iters"]) <= 0.1 * ro["iters"] + case[-1]
-    for k in ["Kd", "G"]:
+    # relative error of Kd, Kphi and G against the FP64 oracle <= 1e-4
+    for k in ["Kd", "G", "Kphi"]:
         b = res_o["num"][k]
         if np.isfinite(b) and b != 0:
             assert res_g["num"][k] == pytest.approx(b, rel=1e-4), k
-    b = res_o["num"]["Kphi"]
-    assert res_g["num"]["Kphi"] == pytest.approx(b, rel=1e-3)
+            _note(f"  {k}: rel {abs(res_g['num'][k] - b) / abs(b):.2e}")
     for k in ["P", "tauXX", "Ux", "Uy"]:
         assert l2_rel(fields[k], s[k]) <= 1e-3, k
 
