@@ -18,10 +18,12 @@ Beside it, in the same run:
     plain four-kernel FP64 solver (the reference's structure);
   * the batched FP64 engine on the same K requests;
   * the batched sweep: 256 RVEs of seeded random size/porosity/pores/yield/
-    preload (sizes 128..1024), split scheme, one engine;
+    preload (sizes 128..1024), split scheme, one engine (all retired solves/s
+    and converged solves/s);
   * serving: Poisson arrivals of moduli queries of sizes 256..2048 under an
-    HBM budget, p50/p99 latency and solves/s, beside the sequential plain FP64
-    solver's figures modelled from its measured per-iteration time;
+    HBM budget at several arrival rates, p50/p99 latency, solves/s and the
+    highest sustainable rate, beside the sequential plain FP64 solver run on
+    the first requests of the same stream;
   * the CPU oracle on this host's cores on a bounded sample.
 
 Printed: ONE JSON line; value and roofline are the headline's, the roofline's
@@ -145,7 +147,8 @@ def percentile(xs, q):
 
 
 def shape_name(kind, rows):
-    names = {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair', 6: 'triple', 7: 'quad'}
+    names = {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair', 6: 'triple', 7: 'quad',
+             8: 'wide3x', 9: 'wide4x'}
     return f"{names[kind]}{rows}"
 
 
@@ -169,7 +172,7 @@ def batch_stats_summary(st, mode):
     instead: the HBM rate a one-iteration kernel would need to match the
     launch's iteration rate (above the peak for n > 1)."""
     nan = float("nan")
-    ipl = {4: 2, 5: 2, 6: 3, 7: 4}.get(st["kernel"], 1)   # iterations per launch
+    ipl = {4: 2, 5: 2, 6: 3, 7: 4, 8: 3, 9: 4}.get(st["kernel"], 1)   # iterations per launch
     it_disp = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else nan
     it_graph = st["graphMs"] / st["graphLaunches"] if st["graphLaunches"] else None
     # timedBytes counts the algorithmic bytes of each iteration
@@ -179,7 +182,7 @@ def batch_stats_summary(st, mode):
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
             3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>",
             5: f"k_march2p<{ct}, {ct}>", 6: f"k_chain<{ct}, {ct}, 3>",
-            7: f"k_chain<{ct}, {ct}, 4>"}[st["kernel"]]
+            7: f"k_chain<{ct}, {ct}, 4>", 8: "k_wide<3>", 9: "k_wide<4>"}[st["kernel"]]
     return {"kernel": kern, "iterations_per_launch": ipl,
             "kernel_avg_us": kms * 1e3,
             "kernel_timing": "graph replays" if it_graph is not None else "dispatch events",
@@ -193,7 +196,9 @@ def batch_stats_summary(st, mode):
             "graph_avg_us_per_launch": ipl * 1e3 * it_graph if it_graph is not None else None,
             "kernel_shape": shape_name(st["kernel"], st["kernelRows"]),
             "tuning_us": cand_us(st),
-            "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"]}
+            "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"],
+            # time steps the split scheme finished in the FP64 arithmetic
+            "fp64_finishes": st["fp64Finishes"]}
 
 
 def run_batch(reqs, mode, check, timing_every=1, **opts):
@@ -231,6 +236,7 @@ def run_batched_headline(pa, nx, mode, steps, warmup, check, graphs=True):
     tot_iters = sum(r["result"]["totalIters"] for r in res.values())
     r0 = res[0]["result"]
     log(f"[bench] batched {mode} x{steps}: {dt:.2f} s, {steps / dt:.2f} solves/s, "
+        f"{st['fp64Finishes']} FP64 finishes, tunings {st['tunings']}, chunks {st['chunks']}, "
         f"kernel {shape_name(st['kernel'], st['kernelRows'])} "
         f"{ {k: round(v, 1) for k, v in cand_us(st).items()} }")
     out = {"mode": mode, "batched": steps, "solves_per_s": steps / dt, "elapsed_s": dt,
@@ -284,42 +290,33 @@ def run_mixed_batch(n, seed, check):
     for d in by_size.values():
         d["mean_iters"] = sum(d["iters"]) / len(d["iters"])
         del d["iters"]
+    conv = sum(bool(r["result"]["converged"]) for r in res.values())
     out = {"workload": f"{n} RVEs, sizes U{{128,256,512,1024}}, porosity U[0.005,0.05], "
                        f"nPores U{{1,2,3}}, Y U[1e-3,5e-3], preload U[1e-3,5e-3] x random unit "
                        f"(Exx,Eyy,Exy), nTasks=3, seed {seed}, split, one engine",
-           "solves": n, "elapsed_s": dt, "solves_per_s": n / dt, "cell_iter_per_s": cells / dt,
+           "solves": n, "elapsed_s": dt,
+           # every request retired (converged or at its iteration cap) / the
+           # converged ones only
+           "solves_per_s": n / dt, "converged": conv, "capped": n - conv,
+           "converged_solves_per_s": conv / dt, "cell_iter_per_s": cells / dt,
            "hbm_frac_whole_run": st["bytesAlgorithmic"] / dt / 1e9 / HBM_PEAK_GBS,
            "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
-           "converged": sum(bool(r["result"]["converged"]) for r in res.values()),
            "by_size": by_size, "iteration_cap": f"{ITER_CAP_PER_N} N per load step"}
     out.update(batch_stats_summary(st, "split"))
-    log(f"[bench] mixed batch {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s, "
-        f"{cells / dt / 1e9:.1f} G cell-iter/s, kernel {out['achieved_GBs']:.0f} GB/s")
+    log(f"[bench] mixed batch {n} RVEs: {dt:.2f} s, {n / dt:.2f} solves/s ({conv} converged, "
+        f"{conv / dt:.2f} converged solves/s), {cells / dt / 1e9:.1f} G cell-iter/s, kernel "
+        f"{out['achieved_GBs']:.0f} GB/s")
     return out, res
 
 
-def plain_iteration_ms(nx, iters=200):
-    """Per-iteration time of the plain four-kernel FP64 solver at size nx
-    (sampled dispatch events, all four kernels)."""
-    from effplast2d_amd import EffPlast2D
-    pa, _ = make_pa(nx)
-    with EffPlast2D(pa, nx, nTasks=3, mode="plain") as g:
-        g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
-        g.iterate(20)
-        g.sync()
-        t0 = time.perf_counter()
-        g.iterate(iters)
-        g.sync()
-        return 1e3 * (time.perf_counter() - t0) / iters
-
-
-def run_serving(rate, duration, seed, check, budget_gib):
+def serve_stream(stream, check, budget_gib):
     """Continuous batching under Poisson arrivals: requests are submitted at
     their arrival times between chunks; each chunk admits what the HBM budget
     allows, runs every admitted RVE to its next residual check, retires the
-    finished ones and backfills."""
+    finished ones and backfills.  Returns results by tag, engine statistics,
+    the peak number of admitted RVEs and the stream's start on the batch
+    clock."""
     from effplast2d_amd import Batch
-    stream = serving_stream(rate, duration, seed)
     done = {}
     with Batch(mode="split", checkInterval=check, timingEvery=1,
                memoryBudget=int(budget_gib * (1 << 30)), adaptiveCheck=ADAPTIVE_CHECK) as b:
@@ -341,37 +338,82 @@ def run_serving(rate, duration, seed, check, budget_gib):
     bad = [r for r in done.values() if r["status"] != 0]
     if bad:
         raise RuntimeError(f"served request failed: {bad[0]['message']}")
+    return done, st, peak, t0
+
+
+def run_serving(rate, duration, seed, check, budget_gib):
+    """One serving run at `rate` arrivals per second: p50/p99 latency (submit
+    to retirement), solves/s over the span from the first arrival to the last
+    retirement, converged counts, and the backlog left when arrivals end."""
+    stream = serving_stream(rate, duration, seed)
+    done, st, peak, t0 = serve_stream(stream, check, budget_gib)
     lat = [r["tDone"] - r["tSubmit"] for r in done.values()]
     span = max(r["tDone"] for r in done.values()) - t0 - stream[0][0]
-    # sequential plain FP64 solver on the same stream: service time of each
-    # request = its iteration count x the plain solver's measured per-iteration
-    # time at its size; FIFO queue over the same arrival times
-    per_iter = {n: plain_iteration_ms(n) for n in sorted({rq["nX"] for _, rq in stream})}
-    t_free, lat_seq = 0.0, []
-    for (ta, rq) in stream:
-        svc = 1e-3 * per_iter[rq["nX"]] * done[rq["tag"]]["result"]["totalIters"]
-        start = max(ta, t_free)
-        t_free = start + svc
-        lat_seq.append(t_free - ta)
-    span_seq = t_free - stream[0][0]
-    out = {"workload": f"Poisson arrivals {rate}/s for {duration} s (seed {seed}), sizes "
+    conv = sum(bool(r["result"]["converged"]) for r in done.values())
+    backlog = sum(1 for r in done.values() if r["tDone"] - t0 > duration)
+    out = {"rate_per_s": rate,
+           "workload": f"Poisson arrivals {rate}/s for {duration} s (seed {seed}), sizes "
                        f"{{256,512,1024,2048}} w [.4,.3,.2,.1], nTasks U{{1,2,3}}, other "
                        f"parameters as the batched sweep, iteration cap {ITER_CAP_PER_N} N per "
                        f"load step, split, HBM budget {budget_gib} GiB",
-           "requests": len(stream), "solves_per_s": len(stream) / span,
+           "requests": len(stream), "converged": conv, "solves_per_s": len(stream) / span,
+           "converged_solves_per_s": conv / span, "span_s": span,
            "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
-           "peak_active": peak, "cell_iter_per_s": st["cellIterations"] / span,
-           "sequential_plain_fp64_modelled": {
-               "solves_per_s": len(stream) / span_seq,
-               "p50_ms": 1e3 * percentile(lat_seq, 0.5), "p99_ms": 1e3 * percentile(lat_seq, 0.99),
-               "per_iteration_ms": per_iter,
-               "how": "each request's GPU iteration count x the plain FP64 solver's measured "
-                      "per-iteration time at its size, FIFO over the same arrivals"}}
+           "peak_active": peak, "unfinished_when_arrivals_end": backlog,
+           "cell_iter_per_s": st["cellIterations"] / span}
     out.update(batch_stats_summary(st, "split"))
-    log(f"[bench] serving: {len(stream)} requests, {out['solves_per_s']:.2f} solves/s, "
-        f"p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms (sequential plain FP64 "
-        f"{out['sequential_plain_fp64_modelled']['solves_per_s']:.2f} solves/s, "
-        f"p99 {out['sequential_plain_fp64_modelled']['p99_ms']:.0f} ms)")
+    log(f"[bench] serving {rate}/s: {len(stream)} requests ({conv} converged), "
+        f"{out['solves_per_s']:.2f} solves/s, p50 {out['p50_ms']:.0f} ms p99 "
+        f"{out['p99_ms']:.0f} ms, peak {peak} active, {backlog} unfinished at {duration} s")
+    return out, stream, done
+
+
+def run_sequential_plain(stream, results, count):
+    """The plain FP64 solver (the reference's four kernels), one request at a
+    time in arrival order, on the first `count` requests of a served stream:
+    measured service times, FIFO queueing over the same arrival times."""
+    from effplast2d_amd import EffPlast2D
+    t_free, lat, svc_total, iters_equal = 0.0, [], 0.0, 0
+    for (ta, rq) in stream[:count]:
+        t0 = time.perf_counter()
+        with EffPlast2D(rq["pa"], rq["nX"], nTasks=rq["nTasks"], nIter=rq["nIter"],
+                        eIter=rq["eIter"], checkInterval=500, mode="plain") as g:
+            res = g.ComputeEffModuli(rq["initLoad"], rq["loadType"])
+        svc = time.perf_counter() - t0
+        svc_total += svc
+        start = max(ta, t_free)
+        t_free = start + svc
+        lat.append(t_free - ta)
+        iters_equal += int(res["totalIters"] == results[rq["tag"]]["result"]["totalIters"])
+    span = t_free - stream[0][0]
+    out = {"requests": count, "solves_per_s": count / span, "service_s_total": svc_total,
+           "p50_ms": 1e3 * percentile(lat, 0.5), "p99_ms": 1e3 * percentile(lat, 0.99),
+           "same_iterations_as_served_split": iters_equal,
+           "how": f"measured: the first {count} requests of the served stream, plain FP64 "
+                  "single-RVE solver, one at a time, FIFO over their arrival times"}
+    served = sorted(results[rq["tag"]]["tDone"] - results[rq["tag"]]["tSubmit"]
+                    for _, rq in stream[:count])
+    out["served_split_same_requests_p50_ms"] = 1e3 * percentile(served, 0.5)
+    log(f"[bench] sequential plain FP64, first {count} requests: {out['solves_per_s']:.2f} "
+        f"solves/s, p50 {out['p50_ms']:.0f} ms p99 {out['p99_ms']:.0f} ms")
+    return out
+
+
+def run_serving_sweep(rates, duration, seed, check, budget_gib, seq_count):
+    """Serving at several arrival rates (same seed): the sustainable rate is
+    the highest rate whose arrivals are all retired within 1.5 x the
+    arrival window and whose p99 stays within 3 x the lowest rate's p99."""
+    runs, first = [], None
+    for rate in rates:
+        out, stream, done = run_serving(rate, duration, seed, check, budget_gib)
+        runs.append(out)
+        if first is None:
+            first = (stream, done)
+    base = runs[0]["p99_ms"]
+    ok = [r for r in runs if r["span_s"] <= 1.5 * duration and r["p99_ms"] <= 3 * base]
+    out = {"runs": runs,
+           "sustainable_rate_per_s": max((r["rate_per_s"] for r in ok), default=None),
+           "sequential_plain_fp64": run_sequential_plain(first[0], first[1], seq_count)}
     return out
 
 
@@ -417,39 +459,39 @@ def cpu_baseline(pa, nx, iters_per_solve, seconds):
             "cell_iter_per_s": rate}
 
 
-def probe(mode, nx, iters, copy_mib, batch=0, tile=0, march=0, march2=0, pair=0, triple=0,
-          quad=0):
+def probe(mode, nx, iters, copy_mib, batch=0, shape="tiles8"):
     """Fixed workload for counter runs: `iters` iterations of one scheme on the
-    preloaded RVE (or, with batch > 0, `iters` launches of the batched kernel
-    over `batch` headline RVEs with a fixed tile height of 8), then 5 copies of
-    copy_mib MiB."""
+    preloaded RVE (or, with batch > 0, `iters` iterations of the batched
+    kernel of `shape` (a name as shape_name gives it, e.g. wide3x64) over
+    `batch` headline RVEs, launched one by one), then 5 copies of copy_mib
+    MiB (the calibration kernel of the traffic counters)."""
     import ctypes as C
 
     from effplast2d_amd import EffPlast2D
     from effplast2d_amd._lib import check, lib
     pa, _ = make_pa(nx)
+    kind = shape.rstrip("0123456789")
+    rows = int(shape[len(kind):])
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
-        shape = ({"quadRows": quad} if quad else {"tripleRows": triple} if triple else
-                 {"pairRows": pair} if pair else
-                 {"march2Rows": march2} if march2 else
-                 {"marchRows": march} if march else {"tileRows": tile or 8})
-        run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False, **shape)
+        run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False,
+                  **{SHAPE_OPTION[kind]: rows})
     else:
-        with EffPlast2D(pa, nx, nTasks=3, mode=mode, tileRows=tile, marchRows=march) as g:
+        kw = {"tiles": {"tileRows": rows}, "march": {"marchRows": rows}}[kind]
+        with EffPlast2D(pa, nx, nTasks=3, mode=mode, **kw) as g:
             g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
             g.iterate(iters)
             g.sync()
     ms = C.c_double()
     check(lib().ep2d_diag_copy(0, copy_mib << 20, 5, C.byref(ms)))
-    log(f"[probe] {mode} {nx} batch {batch}: {iters} iterations; copy {copy_mib} MiB: "
+    log(f"[probe] {mode} {nx} batch {batch} {shape}: {iters} iterations; copy {copy_mib} MiB: "
         f"{ms.value:.3f} ms ({2 * (copy_mib << 20) / ms.value / 1e6:.0f} GB/s)")
 
 
 SHAPE_OPTION = {"tiles": "tileRows", "march": "marchRows", "persist": "persistRows",
                 "march2x": "march2Rows", "pair": "pairRows", "triple": "tripleRows",
-                "quad": "quadRows"}
+                "quad": "quadRows", "wide3x": "wide3Rows", "wide4x": "wide4Rows"}
 
 
 def probe_shapes(mode, nx, iters, batch, shapes):
@@ -468,29 +510,42 @@ def probe_shapes(mode, nx, iters, batch, shapes):
         log(f"[probe] {mode} {nx} batch {batch}: {name}, {iters} iterations")
 
 
+# single-RVE kernel shapes of the sweep: name -> EffPlast2D options
+def single_shape(name):
+    kind = name.rstrip("0123456789")
+    n = int(name[len(kind):])
+    return {"tiles": {"tileRows": n}, "march": {"marchRows": n},
+            "pair": {"marchRows": n, "itersPerLaunch": 2},
+            "triple": {"marchRows": n, "itersPerLaunch": 3},
+            "quad": {"marchRows": n, "itersPerLaunch": 4},
+            "wide3x": {"marchRows": n, "itersPerLaunch": 3, "laneColumns": 2},
+            "wide4x": {"marchRows": n, "itersPerLaunch": 4, "laneColumns": 2}}[kind]
+
+
 def sweep(sizes, modes, shapes, iters):
-    """Average fused-kernel duration per (scheme, kernel shape, grid size) on
-    the preloaded single RVE, every launch timed with dispatch events."""
+    """Average kernel duration per iteration for each (scheme, kernel shape,
+    grid size) on the preloaded single RVE, every launch timed with dispatch
+    events (a multi-iteration launch counts as its iterations)."""
     from effplast2d_amd import EffPlast2D
     rows = []
     for nx in sizes:
         pa, _ = make_pa(nx)
         for mode in modes:
-            for kind, n in shapes:
-                kw = {"marchRows": n} if kind == "march" else {"tileRows": n}
+            for shape in shapes:
+                kw = single_shape(shape)
+                if mode != "split" and "laneColumns" in kw:
+                    continue
                 with EffPlast2D(pa, nx, nTasks=3, mode=mode, **kw) as g:
                     g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
-                    g.iterate(20)
+                    g.iterate(24)
                     g.set_timing(1)
                     g.iterate(iters)
                     ms, n_, name = g.timing()
                 b = BYTES_PER_POINT[mode] * (nx + 1) * nx
-                row = {"nx": nx, "mode": mode, "shape": f"{kind}{n}", "kernel_us": ms * 1e3,
-                       "GBs": b / (ms * 1e-3) / 1e9, "frac": b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "kernel": name}
+                row = {"nx": nx, "mode": mode, "shape": shape, "kernel_us": ms * 1e3,
+                       "GBs_per_iteration": b / (ms * 1e-3) / 1e9, "kernel": name}
                 rows.append(row)
-                log(f"[sweep] nx={nx} {mode} {kind}{n}: {ms * 1e3:.2f} us, "
-                    f"{row['GBs']:.0f} GB/s ({100 * row['frac']:.1f}%)")
+                log(f"[sweep] nx={nx} {mode} {shape}: {ms * 1e3:.2f} us per iteration")
     print(json.dumps({"sweep": rows}), flush=True)
 
 
@@ -577,6 +632,32 @@ def pmc_traffic(fetch_dir, write_dir, kernel_key, copy_mib, dest, command):
     print(text)
 
 
+def sq_summary(sq_dir, kernel_key, dest, command):
+    """VALU issue of the kernel whose name contains kernel_key, from one
+    counter run of VALUBusy (rocprofv3's derived metric: SQ_ACTIVE_INST_VALU
+    summed over the SIMDs / CUs / GRBM_GUI_ACTIVE, the percentage of cycles
+    a SIMD issues VALU instructions), SQ_INSTS_VALU and SQ_WAVES."""
+    path = glob.glob(os.path.join(sq_dir, "**", "*counter_collection.csv"), recursive=True)[0]
+    vals = {}
+    for c in ("VALUBusy", "SQ_INSTS_VALU", "SQ_WAVES"):
+        m = _counter_means(path, c)
+        vals[c] = next((v for k, v in m.items() if kernel_key in k), None)
+    busy = vals["VALUBusy"]
+    lines = [f"# rocprofv3 SQ summary: {kernel_key}",
+             f"# command: {command}",
+             f"VALUBusy (percent, mean over dispatches) = {busy}",
+             f"SQ_INSTS_VALU per dispatch = {vals['SQ_INSTS_VALU']}",
+             f"SQ_WAVES per dispatch = {vals['SQ_WAVES']}",
+             f"valu_insts_per_wave = "
+             f"{vals['SQ_INSTS_VALU'] / vals['SQ_WAVES'] if vals['SQ_WAVES'] else None}",
+             f"valu_busy_frac = {busy / 100.0 if busy is not None else None}"]
+    text = "\n".join(lines) + "\n"
+    if dest:
+        with open(dest, "w") as fh:
+            fh.write(text)
+    print(text)
+
+
 def run_rve4096(check, nIter=100_000):
     """4096^2, 16 pores (nPores=4), porosity 0.05, Y=2.5e-3, nTasks=3: the
     split scheme's moduli (including the periodic internal-cell ones) against
@@ -590,8 +671,9 @@ def run_rve4096(check, nIter=100_000):
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
         # automatic single-RVE shape: three-wave chains (split), wave pairs (FP64)
+        # verbose: the reference's console lines, one per residual check
         with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
-                        mode=mode, shearLoadType=LOAD["shearLoadType"]) as g:
+                        mode=mode, shearLoadType=LOAD["shearLoadType"], verbose=True) as g:
             g.set_timing(64)
             res = g.ComputeEffModuli(LOAD["initLoad"], LOAD["loadType"])
             ms, n, name = g.timing()
@@ -603,7 +685,8 @@ def run_rve4096(check, nIter=100_000):
                      "converged": bool(res["converged"]),
                      "num": res["num"], "num_per": res["num_per"], "Gxy": res["Gxy"],
                      "GxyPer": res["GxyPer"]}
-        log(f"[bench] rve4096 {mode}: {dt:.1f} s, iters {out[mode]['iters']}")
+        log(f"[bench] rve4096 {mode}: {dt:.1f} s, iters {out[mode]['iters']}, errors "
+            f"{['%.2e' % e for e in out[mode]['errors']]}, converged {out[mode]['converged']}")
     err = {}
     for grp in ("num", "num_per"):
         for k in ("Kphi", "Kd", "G"):
@@ -636,8 +719,12 @@ def main():
     ap.add_argument("--single-steps", type=int, default=2)
     ap.add_argument("--mixed-rves", type=int, default=256)
     ap.add_argument("--mixed-seed", type=int, default=2024)
-    ap.add_argument("--serve-rate", type=float, default=1.5)
-    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-rates", default="1.5,3,6",
+                    help="arrival rates (per second) of the serving runs")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-sequential", type=int, default=6,
+                    help="requests of the first serving stream also run on the sequential "
+                         "plain FP64 solver")
     ap.add_argument("--serve-seed", type=int, default=7)
     ap.add_argument("--serve-budget-gib", type=float, default=2.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -650,31 +737,39 @@ def main():
     ap.add_argument("--probe-iters", type=int, default=200)
     ap.add_argument("--probe-batch", type=int, default=0,
                     help="probe the batched kernel with this many headline RVEs")
-    ap.add_argument("--probe-tile", type=int, default=0, help="probe: tile height")
-    ap.add_argument("--probe-march", type=int, default=0, help="probe: marching segment rows")
-    ap.add_argument("--probe-march2", type=int, default=0,
-                    help="probe: two-iteration marching segment rows (batched only)")
-    ap.add_argument("--probe-pair", type=int, default=0,
-                    help="probe: two-iteration marching on wave pairs, segment rows (batched only)")
-    ap.add_argument("--probe-triple", type=int, default=0,
-                    help="probe: three-iteration marching on wave chains, segment rows (batched only)")
-    ap.add_argument("--probe-quad", type=int, default=0,
-                    help="probe: four-iteration marching on wave chains, segment rows (batched only)")
+    ap.add_argument("--probe-shape", default="tiles8",
+                    help="probe: kernel shape, e.g. tiles8, march32 (one RVE) or pair26, "
+                         "triple42, wide3x64 (batched)")
+    # the same as --probe-shape <kind><rows> (the forms the older summaries in
+    # profiles/ were made with)
+    for kind in ("tile", "march", "march2", "pair", "triple", "quad"):
+        ap.add_argument(f"--probe-{kind}", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-mib", type=int, default=512)
     ap.add_argument("--summarize", default=None)
     ap.add_argument("--summary-out", default=None)
     ap.add_argument("--sweep", action="store_true")
     ap.add_argument("--sweep-sizes", default="256,1024,4096")
     ap.add_argument("--sweep-iters", type=int, default=300)
+    ap.add_argument("--sweep-modes", default="fp64,split")
+    ap.add_argument("--sweep-shapes", default="tiles8,tiles14,march16,march32,pair26,triple42,"
+                                               "quad16,wide3x32,wide4x16")
     ap.add_argument("--rve4096", action="store_true")
+    ap.add_argument("--rve4096-iters", type=int, default=100_000,
+                    help="iteration cap per load step of the 4096^2 run (the reference's "
+                         "NITER is 100000)")
     ap.add_argument("--pmc-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None)
     ap.add_argument("--pmc-kernel", default="k_fused<double")
+    ap.add_argument("--sq-summary", default=None, metavar="SQ_DIR",
+                    help="write the VALU issue summary of --pmc-kernel from a counter run")
     ap.add_argument("--pmc-command", default="")
     a = ap.parse_args()
 
     if a.pmc_traffic:
         pmc_traffic(a.pmc_traffic[0], a.pmc_traffic[1], a.pmc_kernel, a.copy_mib,
                     a.summary_out, a.pmc_command)
+        return 0
+    if a.sq_summary:
+        sq_summary(a.sq_summary, a.pmc_kernel, a.summary_out, a.pmc_command)
         return 0
     if a.summarize:
         summarize(a.summarize, a.summary_out, a.pmc_command)
@@ -689,18 +784,22 @@ def main():
                      a.probe_shapes.split(","))
         return 0
     if a.probe:
-        probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, a.probe_tile,
-              a.probe_march, a.probe_march2, a.probe_pair, a.probe_triple, a.probe_quad)
+        shape = a.probe_shape
+        for kind, name in (("tile", "tiles"), ("march", "march"), ("march2", "march2x"),
+                           ("pair", "pair"), ("triple", "triple"), ("quad", "quad")):
+            rows = getattr(a, f"probe_{kind}")
+            if rows:
+                shape = f"{name}{rows}"
+        probe(a.probe_mode, a.nx, a.probe_iters, a.copy_mib, a.probe_batch, shape)
         return 0
     if a.sweep:
-        shapes = [("tiles", t) for t in (8, 14)] + [("march", m) for m in (4, 8, 16, 32, 64, 128)]
-        sweep([int(x) for x in a.sweep_sizes.split(",")], ["fp64", "split"], shapes,
-              a.sweep_iters)
+        sweep([int(x) for x in a.sweep_sizes.split(",")], a.sweep_modes.split(","),
+              a.sweep_shapes.split(","), a.sweep_iters)
         return 0
     global ADAPTIVE_CHECK
     ADAPTIVE_CHECK = a.adaptive_check
     if a.rve4096:
-        run_rve4096(a.check_interval)
+        run_rve4096(a.check_interval, a.rve4096_iters)
         return 0
 
     from effplast2d_amd import device_count
@@ -727,8 +826,9 @@ def main():
             legs["single_plain"]["solves_per_s"]
         legs["elastic_128_plain"] = run_elastic_128()
         legs["mixed_batch"], _ = run_mixed_batch(a.mixed_rves, a.mixed_seed, a.check_interval)
-        legs["serving"] = run_serving(a.serve_rate, a.serve_seconds, a.serve_seed,
-                                              a.check_interval, a.serve_budget_gib)
+        legs["serving"] = run_serving_sweep([float(x) for x in a.serve_rates.split(",")],
+                                            a.serve_seconds, a.serve_seed, a.check_interval,
+                                            a.serve_budget_gib, a.serve_sequential)
     frac = head["achieved_GBs"] / HBM_PEAK_GBS
     # HBM traffic and instruction counters of this kernel shape at this batch
     # size, from the committed rocprofv3 summaries (None when none matches)

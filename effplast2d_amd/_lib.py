@@ -46,7 +46,8 @@ class Options(C.Structure):
                 ("nTimeSteps", C.c_int), ("useShearLoadType", C.c_int),
                 ("shearLoadType", C.c_double * 3), ("verbose", C.c_int),
                 ("logPath", C.c_char_p), ("tileRows", C.c_int), ("marchRows", C.c_int),
-                ("itersPerLaunch", C.c_int), ("adaptiveCheck", C.c_int)]
+                ("itersPerLaunch", C.c_int), ("adaptiveCheck", C.c_int),
+                ("laneColumns", C.c_int), ("keepFp32Stress", C.c_int)]
 
 
 class Moduli(C.Structure):
@@ -96,7 +97,8 @@ class BatchOptions(C.Structure):
                 ("marchRows", C.c_int), ("persistRows", C.c_int), ("march2Rows", C.c_int),
                 ("pairRows", C.c_int), ("tripleRows", C.c_int), ("quadRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
-                ("useGraphs", C.c_int), ("timingEvery", C.c_long), ("adaptiveCheck", C.c_int)]
+                ("useGraphs", C.c_int), ("timingEvery", C.c_long), ("adaptiveCheck", C.c_int),
+                ("wide3Rows", C.c_int), ("wide4Rows", C.c_int), ("keepFp32Stress", C.c_int)]
 
 
 class Request(C.Structure):
@@ -120,7 +122,7 @@ class BatchResult(C.Structure):
                 "message": self.message.decode(errors="replace")}
 
 
-MAX_CAND = 24  # EP2D_MAX_CAND
+MAX_CAND = 32  # EP2D_MAX_CAND
 
 
 class BatchStats(C.Structure):
@@ -132,7 +134,8 @@ class BatchStats(C.Structure):
                 ("active", C.c_int), ("queued", C.c_int), ("deviceBytes", C.c_size_t),
                 ("kernel", C.c_int), ("kernelRows", C.c_int), ("tunings", C.c_int),
                 ("nCand", C.c_int), ("candKind", C.c_int * MAX_CAND),
-                ("candRows", C.c_int * MAX_CAND), ("candUs", C.c_double * MAX_CAND)]
+                ("candRows", C.c_int * MAX_CAND), ("candUs", C.c_double * MAX_CAND),
+                ("fp64Finishes", C.c_longlong)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -21,7 +21,8 @@ class Batch:
     def __init__(self, mode="split", checkInterval=500, tileRows=0, marchRows=0, persistRows=0,
                  march2Rows=0, pairRows=0, tripleRows=0, quadRows=0, device=0, memoryBudget=0,
                  maxActive=0, useGraphs=True,
-                 timingEvery=0, adaptiveCheck=False):
+                 timingEvery=0, adaptiveCheck=False, wide3Rows=0, wide4Rows=0,
+                 keepFp32Stress=False):
         o = L.BatchOptions()
         lib().ep2d_batch_default_options(C.byref(o))
         o.mode = L.MODES[mode] if isinstance(mode, str) else int(mode)
@@ -33,6 +34,9 @@ class Batch:
         o.pairRows = int(pairRows)
         o.tripleRows = int(tripleRows)
         o.quadRows = int(quadRows)
+        o.wide3Rows = int(wide3Rows)
+        o.wide4Rows = int(wide4Rows)
+        o.keepFp32Stress = 1 if keepFp32Stress else 0
         o.device = int(device)
         o.memoryBudget = int(memoryBudget)
         o.maxActive = int(maxActive)

@@ -22,16 +22,16 @@ namespace {
 constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
-// Kernel shapes the tuner times: the fused tile kernel at three tile
-// heights, the row-marching and persistent tile kernels at one each, the
-// two-iteration marching kernel on one wave at one segment length and on wave
-// pairs at nine, and the chains of three and four waves at four and three.
+// Kernel shapes the tuner times: the fused tile kernel at two tile heights,
+// the row-marching kernel at one segment height, the two-iteration wave
+// pairs at four (the short segments give a batch of one or two RVEs enough
+// waves), the chains of three and four waves at two and one, and (split
+// scheme) the wide chains of three and four waves at four and two.  Two
+// timed rounds of every shape fit a 200-iteration chunk.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8},  {kTiles, 14}, {kTiles, 30}, {kMarch, 32}, {kPersist, 14},
-    {kMarch2, 40}, {kPair, 8},   {kPair, 14},  {kPair, 20},  {kPair, 24},
-    {kPair, 26},  {kPair, 28},  {kPair, 32},  {kPair, 40},  {kPair, 52},
-    {kTriple, 28}, {kTriple, 42}, {kTriple, 56}, {kTriple, 86}, {kQuad, 32},
-    {kQuad, 64},  {kQuad, 128}};
+    {kTiles, 8},   {kTiles, 14},  {kMarch, 32},  {kPair, 8},    {kPair, 14},  {kPair, 26},
+    {kPair, 40},   {kTriple, 42}, {kTriple, 86}, {kQuad, 64},   {kWide3, 32}, {kWide3, 64},
+    {kWide3, 96},  {kWide3, 128}, {kWide4, 64},  {kWide4, 128}};
 
 namespace {
 
@@ -50,12 +50,16 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.pairRows < 0) throw ArgError("pairRows must be >= 0");
   if (opt_.tripleRows < 0) throw ArgError("tripleRows must be >= 0");
   if (opt_.quadRows < 0) throw ArgError("quadRows must be >= 0");
+  if (opt_.wide3Rows < 0) throw ArgError("wide3Rows must be >= 0");
+  if (opt_.wide4Rows < 0) throw ArgError("wide4Rows must be >= 0");
+  if ((opt_.wide3Rows != 0 || opt_.wide4Rows != 0) && opt_.mode != EP2D_MODE_SPLIT)
+    throw ArgError("wide3Rows and wide4Rows (wide chains) need the split scheme");
   const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +
                      (opt_.march2Rows != 0) + (opt_.pairRows != 0) + (opt_.tripleRows != 0) +
-                     (opt_.quadRows != 0);
+                     (opt_.quadRows != 0) + (opt_.wide3Rows != 0) + (opt_.wide4Rows != 0);
   if (nfixed > 1)
     throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows, pairRows, "
-                   "tripleRows, quadRows");
+                   "tripleRows, quadRows, wide3Rows, wide4Rows");
   fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
@@ -64,6 +68,8 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   else if (opt_.pairRows) shape_ = {kPair, opt_.pairRows};
   else if (opt_.tripleRows) shape_ = {kTriple, opt_.tripleRows};
   else if (opt_.quadRows) shape_ = {kQuad, opt_.quadRows};
+  else if (opt_.wide3Rows) shape_ = {kWide3, opt_.wide3Rows};
+  else if (opt_.wide4Rows) shape_ = {kWide4, opt_.wide4Rows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -107,7 +113,8 @@ long long Batch::submit(const ep2d_request& rq) {
   if (rq.checkInterval < 0) throw ArgError("request: checkInterval must be >= 0");
   if (fixed_ && multiIter(shape_.kind) && !multi_iter_fits(rq.nX, rq.nX))
     throw ArgError(
-        "request: march2Rows, pairRows, tripleRows and quadRows need every field under 2 GiB "
+        "request: march2Rows, pairRows, tripleRows, quadRows and the wide chains need every field "
+        "under 2 GiB "
         "(nX <= 16382)");
   queue_.push_back({nextId_, rq, clock()});
   st_.submitted++;
@@ -147,6 +154,7 @@ void Batch::admit() {
     o.tileRows = 0;
     o.itersPerLaunch = 1;   // the engine launches its own kernels
     o.adaptiveCheck = opt_.adaptiveCheck;
+    o.keepFp32Stress = opt_.keepFp32Stress;
     try {
       sl.s = std::make_unique<Solver>(jb.rq.pa, jb.rq.nX, jb.rq.nX, o, stream_, &pool_);
       sl.bytes = sl.s->deviceBytes();
@@ -193,6 +201,7 @@ void Batch::finish(Slot& sl) {
   r.tag = sl.tag;
   r.status = EP2D_OK;
   sl.s->jobFinish(&r.result);
+  st_.fp64Finishes += sl.s->escalations();
   r.tSubmit = sl.tSubmit;
   r.tAdmit = sl.tAdmit;
   r.tDone = clock();
@@ -243,6 +252,9 @@ int Batch::buildWork(Shape sh, int4* out) const {
   int t = 0;
   for (size_t k = 0; k < active_.size(); k++) {
     const Solver& s = *active_[k].s;
+    // the RVEs finishing a time step in the FP64 arithmetic are in the
+    // escalated list only (tiles of 8 rows), the others in every other list
+    if (s.escalated() != (sh.kind == kEsc)) continue;
     if (sh.kind == kMarch) {
       t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
@@ -251,15 +263,20 @@ int Batch::buildWork(Shape sh, int4* out) const {
       t += chain_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
+    if (wide(sh.kind)) {
+      t += wide_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
+      continue;
+    }
     if (multiIter(sh.kind)) {
       t += march2_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
     }
+    const int rows = sh.kind == kEsc ? 8 : sh.rows;
     const int ntx = (s.nX() + kTileX - 1) / kTileX;
-    const int nty = (s.nY() + sh.rows - 1) / sh.rows;
+    const int nty = (s.nY() + rows - 1) / rows;
     for (int j = 0; j < nty; j++)
       for (int i = 0; i < ntx; i++) {
-        if (out) out[t] = make_int4((int)k, i * kTileX, j * sh.rows, 0);
+        if (out) out[t] = make_int4((int)k, i * kTileX, j * rows, 0);
         t++;
       }
   }
@@ -277,6 +294,8 @@ void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long 
   if (chain(sh.kind) && dbl)
     launch_chain(ipl(sh.kind), dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_,
                  e0, e1);
+  else if (wide(sh.kind) && dbl)
+    launch_wide(ipl(sh.kind), dRves_, work, n, (int)(l & 1), (int)(d & 1), stream_, e0, e1);
   else if (multiIter(sh.kind) && dbl)
     launch_march2(dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_, e0, e1,
                   sh.kind == kPair);
@@ -358,7 +377,10 @@ Batch::Seq Batch::launchChunk(long c) {
   double bytes = 0.0, cells = 0.0;
   bool multiOk = true;  // every field under the multi-iteration kernels' bound
   for (auto& sl : active_) {
-    ids.push_back(sl.id);
+    // an RVE finishing a time step in the FP64 arithmetic is stepped on its
+    // own (Batch::step): not in the work lists, the statistics or the tuning
+    ids.push_back(2 * sl.id + (sl.s->escalated() ? 1 : 0));
+    if (sl.s->escalated()) continue;
     const double n = sl.s->nX();
     bytes += (opt_.mode == EP2D_MODE_SPLIT ? 48.0 : 64.0) * (n + 1) * n;
     cells += n * n;
@@ -370,7 +392,10 @@ Batch::Seq Batch::launchChunk(long c) {
     shape_ = Shape{kTiles, 14};
     tuneDue_ = true;
   }
-  auto allowed = [&](int k) { return multiOk || !multiIter(kCand[k].kind); };
+  auto allowed = [&](int k) {
+    return (multiOk || !multiIter(kCand[k].kind)) &&
+           (opt_.mode == EP2D_MODE_SPLIT || !wide(kCand[k].kind));
+  };
   // a tuning chunk runs a warm-up round and up to kTuneEach timed rounds of
   // every candidate, as many as the chunk holds (at least one timed round)
   // (leaving room, when the chunk is long enough for two rounds besides, for
@@ -401,6 +426,7 @@ Batch::Seq Batch::launchChunk(long c) {
                             hipMemcpyHostToDevice, stream_));
     curs_ = curs;
   }
+  if (nTiles_ == 0) return Seq{0, 0};  // every RVE finishes in the FP64 arithmetic
   long l = 0, d = 0, it = 0;
   if (tuning) {
     // a warm-up round, then the timed rounds, candidates interleaved; a
@@ -489,7 +515,8 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
   if (ids != key_) {
     dropGraphs();
     const Shape aux{kMarch, shape_.rows};
-    size_t need = (size_t)buildWork(shape_, nullptr);
+    const Shape esc{kEsc, 8};
+    size_t need = (size_t)buildWork(shape_, nullptr) + (size_t)buildWork(esc, nullptr);
     if (multiIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
     if (tuning)
       for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
@@ -502,6 +529,9 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
       auxCnt_ = buildWork(aux, hTiles_ + off);
       off += auxCnt_;
     }
+    escOff_ = off;
+    escCnt_ = buildWork(esc, hTiles_ + off);
+    off += escCnt_;
     if (tuning)
       for (int k = 0; k < kNCand; k++) {
         candOff_[k] = off;
@@ -646,8 +676,19 @@ void Batch::step(int* nActive, int* nQueued) {
     for (auto& sl : active_) c = std::min(c, sl.s->tsChunk());
     c = std::max<long>(c, 1);
     const Seq q = launchChunk(c);
+    // the RVEs finishing a time step in the FP64 arithmetic: c one-iteration
+    // launches of the tile kernel over their own list (U flips each launch,
+    // V is updated in place)
+    for (long k = 0; k < c && escCnt_ > 0; k++)
+      launch_fused_batch(dRves_, dTiles_ + escOff_, escCnt_, kModeSplitStress64, 8,
+                         (int)(k & 1), stream_);
     for (auto& sl : active_) {
-      sl.s->noteBatchIterations(c, q.launches, q.doubles);
+      if (sl.s->escalated()) {
+        sl.s->noteBatchIterations(c, c, 0);
+        st_.cellIterations += (long long)c * sl.s->nX() * sl.s->nY();
+      } else {
+        sl.s->noteBatchIterations(c, q.launches, q.doubles);
+      }
       if (sl.s->tsAtCheck()) sl.s->tsCheckLaunch();
     }
     EP2D_HIP(hipStreamSynchronize(stream_));
@@ -719,6 +760,7 @@ void Batch::stats(ep2d_batch_stats* s) const {
   s->deviceBytes = used_;
   s->kernel = shape_.kind;
   s->kernelRows = shape_.rows;
+  for (const auto& sl : active_) s->fp64Finishes += sl.s->escalations();
   s->nCand = kNCand;
   for (int k = 0; k < kNCand; k++) {
     s->candKind[k] = kCand[k].kind;

@@ -46,22 +46,28 @@ class Batch {
     bool graph = false;  // a whole graph replay (launches counted in graphLaunches)
   };
   static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4, kPair = 5,
-                       kTriple = 6, kQuad = 7;
+                       kTriple = 6, kQuad = 7, kWide3 = 8, kWide4 = 9;
+  // the list of the RVEs finishing a time step in the FP64 arithmetic (tiles
+  // of 8 rows for the batched tile kernel); not a shape the tuner times
+  static constexpr int kEsc = 10;
   // iterations per launch of a kernel kind; a kind with more than one runs
   // the chunk's last iterations through the one-iteration marching kernel
   static int ipl(int kind) {
-    return kind == kQuad ? 4 : kind == kTriple ? 3 : (kind == kMarch2 || kind == kPair ? 2 : 1);
+    return kind == kQuad || kind == kWide4 ? 4
+           : kind == kTriple || kind == kWide3 ? 3
+           : (kind == kMarch2 || kind == kPair ? 2 : 1);
   }
   static bool chain(int kind) { return kind == kTriple || kind == kQuad; }
+  static bool wide(int kind) { return kind == kWide3 || kind == kWide4; }
   static bool multiIter(int kind) { return ipl(kind) > 1; }
   struct Shape {
     int kind = kTiles;  // fused tile, row-marching, persistent tile, two-iteration
                         // marching on one wave or on a pair of waves,
                         // three- or four-iteration marching on a chain of
-                        // three or four waves
+                        // three or four waves, with one or two columns per lane
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 22;
+  static constexpr int kNCand = 16;
   // refinement of the tuning: up to kRefCands candidates within 3% of the
   // best, kRefRounds rounds of kRefLaunches back-to-back launches each
   static constexpr int kRefCands = 3, kRefRounds = 4, kRefLaunches = 4;
@@ -123,6 +129,7 @@ class Batch {
   std::vector<int> curs_;         // current buffers in the uploaded RVE table
   int candOff_[kNCand] = {}, candCnt_[kNCand] = {};
   int auxOff_ = 0, auxCnt_ = 0;   // single-iteration segments of a two-iteration shape
+  int escOff_ = 0, escCnt_ = 0;   // tiles of the RVEs in the FP64 arithmetic
 
   // kernel-shape tuning on real iterations
   bool tuneDue_ = true;

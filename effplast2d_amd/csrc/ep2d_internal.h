@@ -55,7 +55,17 @@ struct RveDev {
   double coh2lo;            // below coh^2: a smaller squared J2 cannot exceed coh
   float coh2lof;            // split scheme: coh^2 in FP32, the yield test of its FP32 stress
   double gtab[16];          // vertex shear modulus by vertex class (mask bits 3..6)
+  const unsigned char* mmap;  // block map of the mask: one byte per kMapRows x kMapCols
+                              // cells, nonzero where a mask byte of the block is
 };
+
+// Blocks of the mask map (RveDev::mmap), row-major, ceil(nX / kMapCols)
+// blocks per row.
+constexpr int kMapRows = 8;
+constexpr int kMapCols = 32;
+inline size_t mask_map_bytes(int nX, int nY) {
+  return (size_t)((nX + kMapCols - 1) / kMapCols) * (size_t)((nY + kMapRows - 1) / kMapRows);
+}
 
 // Stress fields in HBM (plain scheme; also the finalised fields of the fused
 // schemes).
@@ -93,6 +103,8 @@ constexpr int AVG_TY = 4;
 // ---- launchers (ep2d_kernels.hip) ------------------------------------------
 void launch_build_mask(unsigned char* mask, int nX, int nY, double dX, double dY,
                        double rad, double nPores, hipStream_t s);
+void launch_mask_map(const unsigned char* mask, int nX, int nY, unsigned char* map,
+                     hipStream_t s);
 void launch_fill(double* p, size_t n, double v, hipStream_t s);
 void launch_fill_f32(float* p, size_t n, float v, hipStream_t s);
 void launch_affine(double* Ux, double* Uy, int nX, int nY, double dX, double dY,
@@ -106,6 +118,10 @@ void launch_plain_disp(const RveDev& r, double* Ux, double* Uy, const StressDev&
                        hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // fused schemes (mode 1: FP64, mode 2: split): reads U[cur], writes U[cur ^ 1],
 // updates V in place.  e0/e1 (optional) receive the dispatch's start/end.
+// kModeSplitStress64: the split scheme's state (FP64 U, FP32 V) advanced
+// with the FP64 arithmetic (tiles of 8 rows; the finish of a split solve
+// whose FP32 stress cannot resolve the residual target).
+constexpr int kModeSplitStress64 = 3;
 FusedPlan plan_fused(int mode, bool plastic, int ty, int nX, int nY, int device);
 void launch_fused(const RveDev& r, int cur, int mode, bool plastic, const FusedPlan& pl,
                   hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
@@ -145,6 +161,13 @@ constexpr int chain_cols(int ns) { return 64 - 4 * ns; }
 int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out);
 void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// Wide chains (split scheme only): the chains with two columns per lane,
+// strips of 128 - 4 ns output columns; same segments order, U and V buffers
+// and results as launch_chain.
+constexpr int wide_cols(int ns) { return 128 - 4 * ns; }
+int wide_segments(int ns, int nX, int nY, int rows, int rve, int4* out);
+void launch_wide(int ns, const RveDev* rves, const int4* segs, int nsegs, int parity, int vpar,
+                 hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // batched fused iteration over many RVEs: tiles[k] = (rve, i0, j0, parity
 // offset); RVE k reads U[parity ^ offset].  All RVEs share mode and tile height.
 void launch_fused_batch(const RveDev* rves, const int4* tiles, int ntiles, int mode, int ty,

@@ -1047,6 +1047,11 @@ int dispatch_fused(int mode, bool plastic, int ty, A&&... a) {
   }
   if (mode == 2) {
     if (plastic) { EP2D_TY_CASES(float, float, true) } else { EP2D_TY_CASES(float, float, false) }
+  } else if (mode == kModeSplitStress64) {
+    // the split scheme's FP32 velocity storage with the FP64 arithmetic
+    // (tiles of 8 rows only)
+    if (plastic) return Fn<double, float, true, 8>::call(std::forward<A>(a)...);
+    return Fn<double, float, false, 8>::call(std::forward<A>(a)...);
   } else {
     if (plastic) { EP2D_TY_CASES(double, double, true) } else { EP2D_TY_CASES(double, double, false) }
   }
@@ -1420,14 +1425,18 @@ __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
 // old velocities).  Valid outputs shrink by two lanes per stage: lanes 4..59,
 // strips of 56 columns.  The arithmetic of each stage is the single-iteration
 // kernels', so two stages give the same bits as two launches.
-template <typename CT, typename VT, bool EDGE>
+//
+// CLEAN stages (segments whose mask bytes are all 0, chain kernels) read no
+// mask: matrix K and G, no pore zeroing and the vertex modulus of four matrix
+// cells, which is what the mask path computes from zero mask bytes.
+template <typename CT, typename VT, bool EDGE, bool CLEAN = false>
 struct MarchStage {
   using M = March<CT, VT>;
   const RveDev* r;
   const CT* gt;
   int nX, nY, i;
   bool colC, colV, leftEdge, rightEdge, inX, plastic;
-  CT coh, c2lo, K0, Kp, G2_0, G2_p, P0, vdampX, vdampY, dTrho;
+  CT coh, c2lo, K0, Kp, G2_0, G2_p, P0, vdampX, vdampY, dTrho, g0;
   double dT;
   // grid steps held in registers: the stores of a step may alias the RVE
   // table, so values read from it would be read again after every store
@@ -1463,6 +1472,7 @@ struct MarchStage {
     K0 = (CT)rr.K0;
     Kp = (CT)rr.Kp;
     G2_0 = (CT)2.0 * (CT)rr.G0;  // 2G, exact
+    g0 = (CT)rr.gtab[0];
     G2_p = (CT)2.0 * (CT)rr.Gp;
     P0 = (CT)rr.P0;
     dXd = rr.dX;
@@ -1506,8 +1516,8 @@ struct MarchStage {
     const double uxs_n = lane_next(ux_n);
     CT tP_n, tXX_n, tYY_n;
     {
-      const CT K = (m_n & M_CELL_MAT) ? Kp : K0;
-      const CT G2 = (m_n & M_CELL_MAT) ? G2_p : G2_0;
+      const CT K = !CLEAN && (m_n & M_CELL_MAT) ? Kp : K0;
+      const CT G2 = !CLEAN && (m_n & M_CELL_MAT) ? G2_p : G2_0;
       const CT exx = dxs(uxs_n - ux_n);
       const CT eyy = dys(uy_nn - uy_n);
       CT pp, txx, tyy;
@@ -1519,7 +1529,7 @@ struct MarchStage {
         split_cell_trial(exx, eyy, K, G2, P0, pp, txx, tyy);
       }
       const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
-      if ((m_n & M_CELL_ZERO) || !ok) {
+      if ((!CLEAN && (m_n & M_CELL_ZERO)) || !ok) {
         pp = 0;
         txx = 0;
         tyy = 0;
@@ -1533,26 +1543,45 @@ struct MarchStage {
     const double uys_n = lane_next(uy_n);
     CT tXY_c;
     {
-      const CT g = gt[(m_c >> M_VCLASS_SHIFT) & 15];
+      const CT g = CLEAN ? g0 : gt[(m_c >> M_VCLASS_SHIFT) & 15];
       CT txy;
       if constexpr (sizeof(CT) == 8) txy = g * (dys(uxs_n - uxs_c) + dxs(uys_n - uy_n));
       else txy = split_vertex_trial(g, (CT)(uxs_n - uxs_c), rdYc, (CT)(uys_n - uy_n), rdXc);
       const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
-      if ((m_c & M_VERT_ZERO) || !ok) txy = 0;
+      if ((!CLEAN && (m_c & M_VERT_ZERO)) || !ok) txy = 0;
       tXY_c = txy;
     }
     // return mapping
     CT cXY_c = tXY_c, cP_c = tP_c, cXX_c = tXX_c, cYY_c = tYY_c;
-    if (plastic) {
+    if (plastic && !EDGE) {
+      // M::vertex_rm and M::cell_rm, with the scaling skipped where no lane
+      // of the wave yields (the squared invariants decide, as there)
+      const CT sxx = tXX_c + sXX_c + tXX_n + sXX_n;
+      const CT syy = tYY_c + sYY_c + tYY_n + sYY_n;
+      CT s2v, s2c;
+      if constexpr (sizeof(CT) == 8)
+        s2v = (CT)0.0625 * sxx * sxx + (CT)0.0625 * syy * syy + (CT)2.0 * tXY_c * tXY_c;
+      else
+        s2v = split_s2_vertex(sxx, syy, tXY_c);
+      const CT pr_p = lane_prev(tXY_p), pr_c = lane_prev(tXY_c);
+      const CT av = (CT)0.25 * (pr_p + tXY_p + pr_c + tXY_c);
+      if constexpr (sizeof(CT) == 8) s2c = cXX_c * cXX_c + cYY_c * cYY_c + (CT)2.0 * av * av;
+      else s2c = split_s2_cell(cXX_c, cYY_c, av);
+      if (__any(s2v > c2lo || s2c > c2lo)) {
+        CT sc;
+        if (rm_scale(s2v, coh, c2lo, sc)) cXY_c *= sc;
+        if (rm_scale(s2c, coh, c2lo, sc)) {
+          cXX_c = cXX_c * sc;
+          cYY_c = cYY_c * sc;
+        }
+      }
+    } else if (plastic) {
       const bool vok = !EDGE || (colV && j >= 0 && j < nY - 1);
       const CT sxx = tXX_c + sXX_c + tXX_n + sXX_n;
       const CT syy = tYY_c + sYY_c + tYY_n + sYY_n;
       if (vok) cXY_c = M::vertex_rm(tXY_c, sxx, syy, coh, c2lo);
       const CT pr_p = lane_prev(tXY_p), pr_c = lane_prev(tXY_c);
-      if (!EDGE) {
-        const CT av = (CT)0.25 * (pr_p + tXY_p + pr_c + tXY_c);
-        M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
-      } else {
+      {
         const CT nx_p = lane_next(tXY_p), nx_c = lane_next(tXY_c), nx_pp = lane_next(tXY_pp);
         const CT pr_pp = lane_prev(tXY_pp);
         const CT pp_p = lane_prev(pr_p), pp_c = lane_prev(pr_c), pp_pp = lane_prev(pr_pp);
@@ -1639,7 +1668,7 @@ struct MarchStage {
 // segments bound each resource to its field: lanes outside the field's
 // columns carry the offset kOob, and a row outside the field reads through a
 // resource of bound 0, so every such load returns 0 without a branch.
-template <typename CT, typename VT, bool EDGE>
+template <typename CT, typename VT, bool EDGE, bool CLEAN = false>
 struct StageInput {
   using Ld = typename March<CT, VT>::Ld;
   const double* Uxi;
@@ -1682,18 +1711,18 @@ struct StageInput {
   }
 
   // Ux row jb, Uy row jb+1, mask row jb into the stage
-  __device__ __forceinline__ void start(MarchStage<CT, VT, EDGE>& st, int jb) const {
+  __device__ __forceinline__ void start(MarchStage<CT, VT, EDGE, CLEAN>& st, int jb) const {
     double ux = 0.0, uy = 0.0;
     unsigned m = 0;
     const bool colC = !EDGE || (i >= 0 && i < nX);
     if (!EDGE) {
       ux = gld(Uxi, (unsigned)(jb * sx + i));
       uy = gld(Uyi, (unsigned)((jb + 1) * nX + i));
-      m = gld(mask, (unsigned)(jb * nX + i));
+      if (!CLEAN) m = gld(mask, (unsigned)(jb * nX + i));
     } else {
       if (i >= 0 && i <= nX && jb >= 0 && jb < nY) ux = gld(Uxi, (unsigned)(jb * sx + i));
       if (colC && jb + 1 >= 0 && jb + 1 <= nY) uy = gld(Uyi, (unsigned)((jb + 1) * nX + i));
-      if (colC && jb >= 0 && jb < nY) m = gld(mask, (unsigned)(jb * nX + i));
+      if (!CLEAN && colC && jb >= 0 && jb < nY) m = gld(mask, (unsigned)(jb * nX + i));
     }
     st.start(ux, uy, m);
   }
@@ -1704,7 +1733,7 @@ struct StageInput {
     if (!EDGE) {
       d.ux = bld<double>(bUx, o8, (unsigned)(t + 1) * (unsigned)sx * 8u);
       d.uy = bld<double>(bUy, o8, (unsigned)(t + 2) * (unsigned)nX * 8u);
-      d.m = bld<unsigned char>(bM, (unsigned)i, (unsigned)(t + 1) * (unsigned)nX);
+      d.m = CLEAN ? 0u : bld<unsigned char>(bM, (unsigned)i, (unsigned)(t + 1) * (unsigned)nX);
       d.vx = bld<VT>(bVx, oV, (unsigned)t * (unsigned)sx * (unsigned)sizeof(VT));
       d.vy = bld<VT>(bVy, oV, (unsigned)t * (unsigned)nX * (unsigned)sizeof(VT));
     } else {
@@ -1717,7 +1746,7 @@ struct StageInput {
                      s0 = r0 ? (unsigned)t : 0u;
       d.ux = bld<double>(r1 ? bUx : none, oUx, s1 * (unsigned)sx * 8u);
       d.uy = bld<double>(r2 ? bUy : none, oC8, s2 * (unsigned)nX * 8u);
-      d.m = bld<unsigned char>(r1 ? bM : none, oM, s1 * (unsigned)nX);
+      d.m = CLEAN ? 0u : bld<unsigned char>(r1 ? bM : none, oM, s1 * (unsigned)nX);
       d.vx = bld<VT>(r0 ? bVx : none, oVc, s0 * (unsigned)sx * (unsigned)sizeof(VT));
       d.vy = bld<VT>(r0 ? bVy : none, oVc, s0 * (unsigned)nX * (unsigned)sizeof(VT));
     }
@@ -1968,6 +1997,39 @@ __global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves
     march2p_run<CT, VT, true>(r, cur, vpar, s, j0, j1, gt[wave], stageB, ring[wave >> 1], nSteps);
 }
 
+// Block map of the mask: one byte per kMapRows x kMapCols block of cells,
+// nonzero when a mask byte of the block is.
+__global__ void k_mask_map(const unsigned char* __restrict__ mask, int nX, int nY,
+                           unsigned char* __restrict__ map, int nbx, int nby) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbx * nby) return;
+  const int bx = b % nbx, by = b / nbx;
+  const int i0 = bx * kMapCols, j0 = by * kMapRows;
+  const int i1 = min(i0 + kMapCols, nX), j1 = min(j0 + kMapRows, nY);
+  unsigned any = 0;
+  for (int j = j0; j < j1; j++)
+    for (int i = i0; i < i1; i++) any |= mask[(size_t)j * nX + i];
+  map[b] = any ? 1 : 0;
+}
+
+// Whether the mask is 0 over rows [jlo, jhi] and columns [ilo, ihi] (clipped
+// to the grid), from the block map; the same answer in every lane.
+__device__ __forceinline__ bool mask_clear(const RveDev& r, int ilo, int ihi, int jlo, int jhi) {
+  ilo = max(ilo, 0);
+  jlo = max(jlo, 0);
+  ihi = min(ihi, r.nX - 1);
+  jhi = min(jhi, r.nY - 1);
+  if (ilo > ihi || jlo > jhi) return true;
+  const int bx0 = ilo / kMapCols, bx1 = ihi / kMapCols;
+  const int by0 = jlo / kMapRows, by1 = jhi / kMapRows;
+  const int nbx = (r.nX + kMapCols - 1) / kMapCols;
+  const int w = bx1 - bx0 + 1, n = w * (by1 - by0 + 1);
+  unsigned any = 0;
+  for (int q = (int)(threadIdx.x & 63); q < n; q += 64)
+    any |= r.mmap[(by0 + q / w) * nbx + bx0 + q % w];
+  return !__any(any != 0);
+}
+
 // NS iterations per launch on a chain of NS waves (one workgroup).  Stage 0
 // reads the state from HBM and runs the first iteration, stage k the (k+1)-th
 // from the rows of stage k-1, and the last stage writes the result.  Each
@@ -1984,44 +2046,67 @@ __global__ __launch_bounds__(256) void k_march2p(const RveDev* __restrict__ rves
 // row j+1 (slot q-3) of its ring; a middle stage writes its row and that mask
 // row to slot q of the next ring.  Reads and the write of one step never share
 // a slot, and the barrier after each step orders a write before its reads.
-template <typename CT, typename VT, bool EDGE, int NS>
+template <typename CT, typename VT, bool EDGE, bool CLEAN, int NS>
 __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, int s, int j0,
                                           int j1, const CT* gt, int role, RowRing<CT>* ring,
                                           int nSteps) {
   using M = March<CT, VT>;
   using Ld = typename M::Ld;
+  using Stage = MarchStage<CT, VT, EDGE, CLEAN>;
   const int nX = r.nX, nY = r.nY, sx = nX + 1;
   const int L = threadIdx.x & 63;
   const int i = s + L;
   const int jbA = j0 - 2 * NS, tEnd = j1 + 2 * NS - 3;
-  MarchStage<CT, VT, EDGE> st;
+  Stage st;
   st.setup(r, gt, i);
+  // Each stage runs its lockstep steps in three loops -- idle steps before
+  // its first row, its rows, idle steps after its last -- so the loop over
+  // its rows carries only the stage's state, and the ring slot of a step
+  // (q & 3) is known at compile time in a loop unrolled by four.
+  auto idle = [&](int q0, int q1) {
+    for (int q = q0; q < q1; q++) lds_barrier();
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
   if (role == 0) {
-    // stage 0: rows jbA..tEnd from HBM into ring 0 (slot q & 3)
+    // stage 0: rows jbA..tEnd (steps 0 .. tEnd-jbA) from HBM into ring 0
     RowRing<CT>& out = ring[0];
-    const StageInput<CT, VT, EDGE> in(r, cur, vpar, i, tEnd);
+    const StageInput<CT, VT, EDGE, CLEAN> in(r, cur, vpar, i, tEnd);
     in.start(st, jbA);
-    auto fetch = [&](int t) { return in.fetch(t); };
-    auto stepA = [&](int q, const Ld& ld, int sl) {
-      const int t = jbA + q;
-      if (t <= tEnd) {
-        const auto o = st.step(t, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
-        out.ux[sl][L] = o.ux;
-        out.uy[sl][L] = o.uy;
-        out.vx[sl][L] = o.vx;
-        out.vy[sl][L] = o.vy;
-        out.m[sl][L] = ld.m;
-      }
+    const int nAct = tEnd - jbA + 1;
+    auto stepA = [&](int q, const Ld& ld, auto sl) {
+      const auto o = st.step(jbA + q, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
+      out.ux[sl][L] = o.ux;
+      out.uy[sl][L] = o.uy;
+      out.vx[sl][L] = o.vx;
+      out.vy[sl][L] = o.vy;
+      if (!CLEAN) out.m[sl][L] = ld.m;
       lds_barrier();
     };
-    Ld P = fetch(jbA);
-    for (int q = 0; q < nSteps; q += 2) {
-      const Ld Q = fetch(jbA + q + 1);
-      stepA(q, P, q & 3);
-      if (q + 1 >= nSteps) break;
-      P = fetch(jbA + q + 2);
-      stepA(q + 1, Q, (q + 1) & 3);
+    Ld P = in.fetch(jbA), Q;
+    int q = 0;
+    for (; q + 4 <= nAct; q += 4) {
+      Q = in.fetch(jbA + q + 1);
+      stepA(q, P, I0{});
+      P = in.fetch(jbA + q + 2);
+      stepA(q + 1, Q, I1{});
+      Q = in.fetch(jbA + q + 3);
+      stepA(q + 2, P, I2{});
+      P = in.fetch(jbA + q + 4);
+      stepA(q + 3, Q, I3{});
     }
+    if (q < nAct) {
+      Q = in.fetch(jbA + q + 1);
+      stepA(q, P, I0{});
+      if (q + 1 < nAct) {
+        P = in.fetch(jbA + q + 2);
+        stepA(q + 1, Q, I1{});
+        if (q + 2 < nAct) stepA(q + 2, P, I2{});
+      }
+    }
+    idle(nAct, nSteps);
     return;
   }
   // every later stage is its own loop with its stage number K known at
@@ -2030,7 +2115,9 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   auto march = [&](auto K) {
     constexpr int k = decltype(K)::value, lag = 3 * k;
     constexpr bool last = k == NS - 1;
-    const int jb = j0 - 2 * (NS - k), tLast = tEnd + k;
+    // first step (rows in from the ring) and last step of this stage
+    constexpr int qStart = 5 * k - 1;
+    const int qLast = tEnd + k - jbA;
     RowRing<CT>& in = ring[k - 1];
     RowRing<CT>& out = ring[last ? k - 1 : k];  // not written by the last stage
     double* __restrict__ Uxo = r.Ux[cur ^ 1];
@@ -2046,50 +2133,65 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
     const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo),
                                  bVx = EDGE ? brsrc_n(Vxo, vxb) : brsrc(Vxo),
                                  bVy = EDGE ? brsrc_n(Vyo, vyb) : brsrc(Vyo);
-    // slot phase ph = q & 3 known at compile time (the loop is unrolled by
-    // four): ring addresses are immediate offsets
+    // step q reads Ux' of row j+1 (slot q-2), Uy' of row j+2 (slot q-1), V'
+    // of row j and mask row j+1 (slot q-3); a middle stage writes slot q
     auto stepK = [&](int q, auto ph) {
       constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;  // q-1, q-2, q-3
-      const int t = jbA + q;
-      if (t == jb + lag - 1) {
-        st.start(in.ux[s2][L], in.uy[s1][L], in.m[s3][L]);
-      } else if (t >= jb + lag && t <= tLast) {
-        const int j = t - lag;
-        const double uxn = in.ux[s2][L], uynn = in.uy[s1][L];
-        const unsigned mn = in.m[s3][L];
-        const CT vxi = in.vx[s3][L], vyi = in.vy[s3][L];
-        const double uxsK = st.uxs_c;  // Ux(i+1, j) of this stage's input, before the step
-        const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
-        if constexpr (!last) {
-          out.ux[ph][L] = o.ux;
-          out.uy[ph][L] = o.uy;
-          out.vx[ph][L] = o.vx;
-          out.vy[ph][L] = o.vy;
-          out.m[ph][L] = mn;
-        } else if (j >= j0 && outLane) {
-          const unsigned ovx = EDGE && !o.updX ? kOob : oV, ovy = EDGE && !o.updY ? kOob : oV;
-          bst(bVx, ovx, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
-          bst(bVy, ovy, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
-          bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
-          bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
-          if (EDGE) {
-            // nodes ComputeDisp never updates: Ux column nX, Uy row nY
-            if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK);
-            if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
-          }
+      const int j = jbA + q - lag;
+      const double uxn = in.ux[s2][L], uynn = in.uy[s1][L];
+      const unsigned mn = CLEAN ? 0u : in.m[s3][L];
+      const CT vxi = in.vx[s3][L], vyi = in.vy[s3][L];
+      const double uxsK = st.uxs_c;  // Ux(i+1, j) of this stage's input, before the step
+      const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
+      if constexpr (!last) {
+        out.ux[ph][L] = o.ux;
+        out.uy[ph][L] = o.uy;
+        out.vx[ph][L] = o.vx;
+        out.vy[ph][L] = o.vy;
+        if (!CLEAN) out.m[ph][L] = mn;
+      } else if (j >= j0 && outLane) {
+        const unsigned ovx = EDGE && !o.updX ? kOob : oV, ovy = EDGE && !o.updY ? kOob : oV;
+        bst(bVx, ovx, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
+        bst(bVy, ovy, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
+        bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
+        bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
+        if (EDGE) {
+          // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+          if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK);
+          if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
         }
       }
       lds_barrier();
     };
-    for (int q = 0; q < nSteps; q += 4) {
-      stepK(q, std::integral_constant<int, 0>{});
-      if (q + 1 >= nSteps) break;
-      stepK(q + 1, std::integral_constant<int, 1>{});
-      if (q + 2 >= nSteps) break;
-      stepK(q + 2, std::integral_constant<int, 2>{});
-      if (q + 3 >= nSteps) break;
-      stepK(q + 3, std::integral_constant<int, 3>{});
+    idle(0, qStart);
+    {
+      // first rows: Ux' row jb (slot qStart-2), Uy' row jb+1 (slot
+      // qStart-1), mask row jb (slot qStart-3)
+      constexpr int ph = qStart & 3;
+      constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;
+      st.start(in.ux[s2][L], in.uy[s1][L], CLEAN ? 0u : in.m[s3][L]);
+      lds_barrier();
     }
+    constexpr int P0 = (qStart + 1) & 3;
+    using Q0 = std::integral_constant<int, P0>;
+    using Q1 = std::integral_constant<int, (P0 + 1) & 3>;
+    using Q2 = std::integral_constant<int, (P0 + 2) & 3>;
+    using Q3 = std::integral_constant<int, (P0 + 3) & 3>;
+    int q = qStart + 1;
+    for (; q + 3 <= qLast; q += 4) {
+      stepK(q, Q0{});
+      stepK(q + 1, Q1{});
+      stepK(q + 2, Q2{});
+      stepK(q + 3, Q3{});
+    }
+    if (q <= qLast) {
+      stepK(q, Q0{});
+      if (q + 1 <= qLast) {
+        stepK(q + 1, Q1{});
+        if (q + 2 <= qLast) stepK(q + 2, Q2{});
+      }
+    }
+    idle(qLast + 1, nSteps);
   };
   static_assert(NS == 3 || NS == 4, "chains of three or four stages");
   if (role == 1) {
@@ -2101,9 +2203,11 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   }
 }
 
-// One segment per workgroup of NS waves (stage k on wave k).
+// One segment per workgroup of NS waves (stage k on wave k).  The split
+// scheme's register budget holds seven waves per SIMD.
 template <typename CT, typename VT, int NS>
-__global__ __launch_bounds__(64 * NS) void k_chain(const RveDev* __restrict__ rves,
+__global__ __launch_bounds__(64 * NS)
+__attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? 7 : 1))) void k_chain(const RveDev* __restrict__ rves,
                                                    const int4* __restrict__ segs, int nsegs,
                                                    int parity, int vpar) {
   __shared__ CT gt[NS][16];
@@ -2121,10 +2225,701 @@ __global__ __launch_bounds__(64 * NS) void k_chain(const RveDev* __restrict__ rv
   const int s = sg.y, j0 = sg.z, j1 = sg.w;
   const int nSteps = j1 - j0 + 5 * NS - 3;
   const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1);
-  if (interior)
-    chain_run<CT, VT, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+  // mask rows the segment reads: j0-2NS .. j1+2NS-2, columns s .. s+63
+  const bool clean = interior && mask_clear(r, s, s + 63, j0 - 2 * NS, j1 + 2 * NS - 2);
+  if (clean)
+    chain_run<CT, VT, false, true, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+  else if (interior)
+    chain_run<CT, VT, false, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
   else
-    chain_run<CT, VT, true, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+    chain_run<CT, VT, true, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+}
+
+// ---------------------------------------------------------------------------
+// Wide chains (split scheme): the chain kernels with two columns per lane.
+// Lane L holds columns i0 = s+2L (element 0) and i0+1 (element 1) of every
+// row, so one wave spans 128 columns: a stage's four halo columns take 4 of
+// 128 lanes' columns instead of 4 of 64, and of the two x-neighbours of a
+// lane's columns one is the lane's other column, so one lane shift serves
+// both.  Strips carry 128 - 4 NS output columns (lanes NS .. 63-NS).  Rows,
+// stage lags, the order of ring slots and the U and V buffers are those of
+// the chain kernels, and each element is the split scheme's arithmetic in the
+// same order, so the results equal the chain and tile kernels' bit for bit.
+//
+// Segments are classified as a workgroup starts.  EX: the strip touches the
+// left or right boundary (per-column boundary tests); EY: the segment's rows
+// reach the top or bottom boundary (per-row tests, uniform over the wave).
+// CLEAN: every mask byte the segment reads is 0 (from the RVE's block map of
+// the mask), so the stages read no mask and use the matrix K and G, no pore
+// zeroing and the vertex modulus of four matrix cells -- exactly what the
+// mask path computes from zero mask bytes.
+
+// two doubles / two floats at consecutive columns through a buffer resource,
+// one access per element: a row of Ux or Vx (nX+1 elements) starts at an odd
+// element on every other row, so a pair is aligned to one element only
+__device__ __forceinline__ void bld2d(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                      double (&v)[2]) {
+  v[0] = bld<double>(r, voff, soff);
+  v[1] = bld<double>(r, voff + 8u, soff);
+}
+__device__ __forceinline__ void bld2f(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                      float (&v)[2]) {
+  v[0] = bld<float>(r, voff, soff);
+  v[1] = bld<float>(r, voff + 4u, soff);
+}
+__device__ __forceinline__ void bst2d(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                      const double (&v)[2]) {
+  bst(r, voff, soff, v[0]);
+  bst(r, voff + 8u, soff, v[1]);
+}
+__device__ __forceinline__ void bst2f(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff,
+                                      const float (&v)[2]) {
+  bst(r, voff, soff, v[0]);
+  bst(r, voff + 4u, soff, v[1]);
+}
+
+// One wide stage: MarchStage<float, float, EDGE> on two columns per lane.
+template <bool EX, bool EY, bool CLEAN>
+struct WideStage {
+  static constexpr bool EDGE = EX || EY;
+  int nX, nY, i0;
+  bool colC[2], colV[2], leftE[2], rightE[2];
+  bool plastic;
+  float coh, c2lo, K0, Kp, G2_0, G2_p, P0, vdampX, vdampY, dTrho, rdX, rdY, g0;
+  double dT;
+  const float* gt;
+  // carried rows, element c at column i0 + c: U rows j and j+1, mask row j,
+  // trial cells j (and j-1), trial vertices j-1 and j-2, corrected row j-1;
+  // the values at column i0+2 (next lane) and i0-1 (previous lane) that a
+  // step needs again are carried too
+  double ux_c[2], uy_c[2], uy_n[2], uxs_c1;
+  unsigned m_c[2];
+  float tP_c[2], tXX_c[2], tYY_c[2], sXX_c1, sYY_c1;
+  float tP_p[2], tXX_p[2], tYY_p[2];
+  float tXY_p[2], tXY_pp[2], prXY_p0;
+  float cP_p[2], cYY_p[2], cXY_p[2], rXY_p0;
+
+  struct Out {
+    double ux[2], uy[2];
+    float vx[2], vy[2];
+    bool updX[2], updY[2];
+  };
+
+  __device__ __forceinline__ void setup(const RveDev& rr, const float* g, int col0) {
+    gt = g;
+    nX = rr.nX;
+    nY = rr.nY;
+    i0 = col0;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int i = col0 + c;
+      colC[c] = !EX || (i >= 0 && i < nX);
+      colV[c] = !EX || (i >= 0 && i < nX - 1);
+      leftE[c] = EX && i == 0;
+      rightE[c] = EX && i == nX - 1;
+    }
+    plastic = rr.plastic != 0;
+    coh = (float)rr.coh;
+    c2lo = rr.coh2lof;
+    K0 = (float)rr.K0;
+    Kp = (float)rr.Kp;
+    G2_0 = 2.0f * (float)rr.G0;
+    G2_p = 2.0f * (float)rr.Gp;
+    P0 = (float)rr.P0;
+    rdX = (float)rr.rdX;
+    rdY = (float)rr.rdY;
+    vdampX = (float)rr.vdampX;
+    vdampY = (float)rr.vdampY;
+    dTrho = (float)rr.dTrho;
+    dT = rr.dT;
+    g0 = (float)rr.gtab[0];
+  }
+
+  // Ux row jb, Uy row jb+1, mask row jb (jb = first step)
+  __device__ __forceinline__ void start(const double (&ux)[2], const double (&uy1)[2],
+                                        const unsigned (&m)[2]) {
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      ux_c[c] = ux[c];
+      uy_c[c] = 0.0;
+      uy_n[c] = uy1[c];
+      m_c[c] = m[c];
+      tP_c[c] = tXX_c[c] = tYY_c[c] = 0;
+      tP_p[c] = tXX_p[c] = tYY_p[c] = 0;
+      tXY_p[c] = tXY_pp[c] = 0;
+      cP_p[c] = cYY_p[c] = cXY_p[c] = 0;
+    }
+    uxs_c1 = lane_next(ux[0]);
+    sXX_c1 = sYY_c1 = 0;
+    prXY_p0 = rXY_p0 = 0;
+  }
+
+  __device__ __forceinline__ Out step(const int j, const double (&ux_n)[2],
+                                      const double (&uy_nn)[2], const unsigned (&m_n)[2],
+                                      const float (&vx_in)[2], const float (&vy_in)[2]) {
+    using M = March<float, float>;
+    // trial cell row j+1
+    const double uxs_n[2] = {ux_n[1], lane_next(ux_n[0])};
+    float tP_n[2], tXX_n[2], tYY_n[2];
+    const bool rowC = !EY || (j + 1 >= 0 && j + 1 < nY);
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      float K = K0, G2 = G2_0;
+      if constexpr (!CLEAN) {
+        K = (m_n[c] & M_CELL_MAT) ? Kp : K0;
+        G2 = (m_n[c] & M_CELL_MAT) ? G2_p : G2_0;
+      }
+      const float exx = (float)(uxs_n[c] - ux_n[c]) * rdX;
+      const float eyy = (float)(uy_nn[c] - uy_n[c]) * rdY;
+      float pp, txx, tyy;
+      split_cell_trial(exx, eyy, K, G2, P0, pp, txx, tyy);
+      const bool ok = rowC && colC[c];
+      if ((!CLEAN && (m_n[c] & M_CELL_ZERO)) || !ok) {
+        pp = 0;
+        txx = 0;
+        tyy = 0;
+      }
+      tP_n[c] = pp;
+      tXX_n[c] = txx;
+      tYY_n[c] = tyy;
+    }
+    const float sXX_n[2] = {tXX_n[1], lane_next(tXX_n[0])};
+    const float sYY_n[2] = {tYY_n[1], lane_next(tYY_n[0])};
+    // trial vertex row j
+    const double uys_n[2] = {uy_n[1], lane_next(uy_n[0])};
+    const double uxs_c[2] = {ux_c[1], uxs_c1};
+    const bool rowV = !EY || (j >= 0 && j < nY - 1);
+    float tXY_c[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      float g = g0;
+      if constexpr (!CLEAN) g = gt[(m_c[c] >> M_VCLASS_SHIFT) & 15];
+      float txy = split_vertex_trial(g, (float)(uxs_n[c] - uxs_c[c]), rdY,
+                                     (float)(uys_n[c] - uy_n[c]), rdX);
+      const bool ok = rowV && colV[c];
+      if ((!CLEAN && (m_c[c] & M_VERT_ZERO)) || !ok) txy = 0;
+      tXY_c[c] = txy;
+    }
+    // return mapping
+    float cXY_c[2], cP_c[2], cXX_c[2], cYY_c[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      cXY_c[c] = tXY_c[c];
+      cP_c[c] = tP_c[c];
+      cXX_c[c] = tXX_c[c];
+      cYY_c[c] = tYY_c[c];
+    }
+    float prc0 = 0;
+    if (plastic) {
+      const float sXX_c[2] = {tXX_c[1], sXX_c1}, sYY_c[2] = {tYY_c[1], sYY_c1};
+      prc0 = lane_prev(tXY_c[1]);
+      const float pr_p[2] = {prXY_p0, tXY_p[0]}, pr_c[2] = {prc0, tXY_c[0]};
+      if constexpr (!EDGE) {
+        // M::vertex_rm and M::cell_rm, with the scaling skipped where no lane
+        // of the wave yields (the squared invariants decide, as there)
+        float s2v[2], s2c[2];
+        bool yield = false;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const float sxx = tXX_c[c] + sXX_c[c] + tXX_n[c] + sXX_n[c];
+          const float syy = tYY_c[c] + sYY_c[c] + tYY_n[c] + sYY_n[c];
+          s2v[c] = split_s2_vertex(sxx, syy, tXY_c[c]);
+          const float av = 0.25f * (pr_p[c] + tXY_p[c] + pr_c[c] + tXY_c[c]);
+          s2c[c] = split_s2_cell(cXX_c[c], cYY_c[c], av);
+          yield = yield || s2v[c] > c2lo || s2c[c] > c2lo;
+        }
+        if (__any(yield)) {
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            float sc;
+            if (rm_scale(s2v[c], coh, c2lo, sc)) cXY_c[c] *= sc;
+            if (rm_scale(s2c[c], coh, c2lo, sc)) {
+              cXX_c[c] = cXX_c[c] * sc;
+              cYY_c[c] = cYY_c[c] * sc;
+            }
+          }
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          const float sxx = tXX_c[c] + sXX_c[c] + tXX_n[c] + sXX_n[c];
+          const float syy = tYY_c[c] + sYY_c[c] + tYY_n[c] + sYY_n[c];
+          if (rowV && colV[c]) cXY_c[c] = M::vertex_rm(tXY_c[c], sxx, syy, coh, c2lo);
+        }
+        // the clamped vertex blocks of the edge cells (first/second as in
+        // MarchStage): one column right and two left of each element
+        float nx_p[2], nx_c[2], nx_pp[2], pr_pp[2], pp_p[2], pp_c[2], pp_pp[2];
+        if constexpr (EX) {
+          nx_p[0] = tXY_p[1];
+          nx_p[1] = lane_next(tXY_p[0]);
+          nx_c[0] = tXY_c[1];
+          nx_c[1] = lane_next(tXY_c[0]);
+          nx_pp[0] = tXY_pp[1];
+          nx_pp[1] = lane_next(tXY_pp[0]);
+          pp_p[0] = lane_prev(tXY_p[0]);
+          pp_p[1] = pr_p[0];
+          pp_c[0] = lane_prev(tXY_c[0]);
+          pp_c[1] = prc0;
+          pr_pp[0] = lane_prev(tXY_pp[1]);
+          pr_pp[1] = tXY_pp[0];
+          pp_pp[0] = lane_prev(tXY_pp[0]);
+          pp_pp[1] = pr_pp[0];
+        } else {
+#pragma unroll
+          for (int c = 0; c < 2; c++) nx_p[c] = nx_c[c] = nx_pp[c] = pp_p[c] = pp_c[c] = pp_pp[c] = 0;
+          pr_pp[0] = lane_prev(tXY_pp[1]);
+          pr_pp[1] = tXY_pp[0];
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          auto first = [&](float t, float pr, float pp) {
+            return leftE[c] ? t : (rightE[c] ? pp : pr);
+          };
+          auto second = [&](float t, float pr, float nxv) {
+            return leftE[c] ? nxv : (rightE[c] ? pr : t);
+          };
+          const bool cc = colC[c];
+          if (cc && (!EY || (j >= 1 && j < nY - 1))) {
+            const float av = 0.25f * (first(tXY_p[c], pr_p[c], pp_p[c]) +
+                                      second(tXY_p[c], pr_p[c], nx_p[c]) +
+                                      first(tXY_c[c], pr_c[c], pp_c[c]) +
+                                      second(tXY_c[c], pr_c[c], nx_c[c]));
+            M::cell_rm(cXX_c[c], cYY_c[c], av, coh, c2lo);
+          } else if (EY && cc && j == nY - 1 && j >= 1) {
+            const float av = 0.25f * (first(tXY_pp[c], pr_pp[c], pp_pp[c]) +
+                                      second(tXY_pp[c], pr_pp[c], nx_pp[c]) +
+                                      first(tXY_p[c], pr_p[c], pp_p[c]) +
+                                      second(tXY_p[c], pr_p[c], nx_p[c]));
+            M::cell_rm(cXX_c[c], cYY_c[c], av, coh, c2lo);
+          }
+          if (EY && j == 1) {
+            float xx = tXX_p[c], yy = tYY_p[c];
+            if (cc) {
+              const float av = 0.25f * (first(tXY_p[c], pr_p[c], pp_p[c]) +
+                                        second(tXY_p[c], pr_p[c], nx_p[c]) +
+                                        first(tXY_c[c], pr_c[c], pp_c[c]) +
+                                        second(tXY_c[c], pr_c[c], nx_c[c]));
+              M::cell_rm(xx, yy, av, coh, c2lo);
+            }
+            cP_p[c] = tP_p[c];
+            cYY_p[c] = yy;
+          }
+        }
+      }
+    }
+    // velocity and displacement of row j
+    const float rP[2] = {lane_prev(cP_c[1]), cP_c[0]};
+    const float rXX[2] = {lane_prev(cXX_c[1]), cXX_c[0]};
+    const float rxc0 = lane_prev(cXY_c[1]);
+    const float rXY_c[2] = {rxc0, cXY_c[0]}, rXY_p[2] = {rXY_p0, cXY_p[0]};
+    Out o;
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const int i = i0 + c;
+      float vx = vx_in[c], vy = vy_in[c];
+      o.updX[c] = (!EX || (i > 0 && i < nX)) && (!EY || (j > 0 && j < nY - 1));
+      o.updY[c] = (!EX || (i > 0 && i < nX - 1)) && (!EY || (j > 0 && j < nY));
+      if (o.updX[c])
+        vx = split_velocity(vx, vdampX, dTrho, -cP_c[c] + rP[c] + cXX_c[c] - rXX[c], rdX,
+                            rXY_c[c] - rXY_p[c], rdY);
+      if (o.updY[c])
+        vy = split_velocity(vy, vdampY, dTrho, -cP_c[c] + cP_p[c] + cYY_c[c] - cYY_p[c], rdY,
+                            cXY_p[c] - rXY_p[c], rdX);
+      const bool in = colC[c] && (!EY || (j >= 0 && j < nY));
+      o.vx[c] = vx;
+      o.vy[c] = vy;
+      o.ux[c] = in ? split_disp(ux_c[c], vx, dT) : ux_c[c];
+      o.uy[c] = in ? split_disp(uy_c[c], vy, dT) : uy_c[c];
+    }
+    // rotate
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      ux_c[c] = ux_n[c];
+      uy_c[c] = uy_n[c];
+      uy_n[c] = uy_nn[c];
+      m_c[c] = m_n[c];
+      tP_p[c] = tP_c[c];
+      tXX_p[c] = tXX_c[c];
+      tYY_p[c] = tYY_c[c];
+      tP_c[c] = tP_n[c];
+      tXX_c[c] = tXX_n[c];
+      tYY_c[c] = tYY_n[c];
+      tXY_pp[c] = tXY_p[c];
+      tXY_p[c] = tXY_c[c];
+      cP_p[c] = cP_c[c];
+      cYY_p[c] = cYY_c[c];
+      cXY_p[c] = cXY_c[c];
+    }
+    uxs_c1 = uxs_n[1];
+    sXX_c1 = sXX_n[1];
+    sYY_c1 = sYY_n[1];
+    prXY_p0 = prc0;
+    rXY_p0 = rxc0;
+    return o;
+  }
+};
+
+// Stage 0's inputs of a wide segment: StageInput on two columns per lane.
+// Interior strips (not EX) read the lane's two columns at consecutive
+// offsets; EX strips read each column through a resource bounded to its
+// field (lane offsets of columns outside the field are kOob, loads return 0).
+template <bool EX, bool EY, bool CLEAN>
+struct WideInput {
+  struct Ld {
+    double ux[2], uy[2];   // Ux row t+1, Uy row t+2
+    unsigned m[2];         // mask row t+1
+    float vx[2], vy[2];    // V row t
+  };
+  int nX, nY, sx, i0, tEnd;
+  unsigned o8, o4;
+  __amdgpu_buffer_rsrc_t bUx, bUy, bVx, bVy, bM, none;
+  unsigned oUx[2], oC8[2], oM[2], oVc[2];
+
+  __device__ __forceinline__ WideInput(const RveDev& r, int cur, int vpar, int col0, int tEnd_)
+      : nX(r.nX), nY(r.nY), sx(r.nX + 1), i0(col0), tEnd(tEnd_), o8((unsigned)col0 * 8u),
+        o4((unsigned)col0 * 4u) {
+    const double* Ux = r.Ux[cur];
+    const double* Uy = r.Uy[cur];
+    const float* Vx = static_cast<const float*>(vpar ? r.Vx2 : r.Vx);
+    const float* Vy = static_cast<const float*>(vpar ? r.Vy2 : r.Vy);
+    if (!EDGE()) {
+      bUx = brsrc(Ux);
+      bUy = brsrc(Uy);
+      bVx = brsrc(Vx);
+      bVy = brsrc(Vy);
+      bM = brsrc(r.mask);
+      none = bUx;
+    } else {
+      const unsigned ux = (unsigned)nY * (unsigned)sx, uy = (unsigned)(nY + 1) * (unsigned)nX;
+      bUx = brsrc_n(Ux, ux * 8u);
+      bUy = brsrc_n(Uy, uy * 8u);
+      bVx = brsrc_n(Vx, ux * 4u);
+      bVy = brsrc_n(Vy, uy * 4u);
+      bM = brsrc_n(r.mask, (unsigned)nY * (unsigned)nX);
+      none = brsrc_n(Ux, 0u);
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        const int i = col0 + c;
+        const bool cx = i >= 0 && i <= nX, cc = i >= 0 && i < nX;
+        oUx[c] = cx ? (unsigned)i * 8u : kOob;
+        oC8[c] = cc ? (unsigned)i * 8u : kOob;
+        oM[c] = cc ? (unsigned)i : kOob;
+        oVc[c] = cc ? (unsigned)i * 4u : kOob;
+      }
+    }
+  }
+  static constexpr bool EDGE() { return EX || EY; }
+
+  // one row of a field: two columns at element offsets in (EX: each through
+  // its own offset)
+  __device__ __forceinline__ void rowd(__amdgpu_buffer_rsrc_t b, const unsigned (&oc)[2],
+                                       unsigned soff, double (&v)[2]) const {
+    if constexpr (!EX) {
+      bld2d(b, o8, soff, v);
+    } else {
+      v[0] = bld<double>(b, oc[0], soff);
+      v[1] = bld<double>(b, oc[1], soff);
+    }
+  }
+  __device__ __forceinline__ void rowf(__amdgpu_buffer_rsrc_t b, unsigned soff,
+                                       float (&v)[2]) const {
+    if constexpr (!EX) {
+      bld2f(b, o4, soff, v);
+    } else {
+      v[0] = bld<float>(b, oVc[0], soff);
+      v[1] = bld<float>(b, oVc[1], soff);
+    }
+  }
+  __device__ __forceinline__ void rowm(__amdgpu_buffer_rsrc_t b, unsigned soff,
+                                       unsigned (&m)[2]) const {
+    if constexpr (CLEAN) {
+      m[0] = m[1] = 0;
+    } else if constexpr (!EX) {
+      m[0] = bld<unsigned char>(b, (unsigned)i0, soff);
+      m[1] = bld<unsigned char>(b, (unsigned)i0 + 1u, soff);
+    } else {
+      m[0] = bld<unsigned char>(b, oM[0], soff);
+      m[1] = bld<unsigned char>(b, oM[1], soff);
+    }
+  }
+
+  // Ux row jb, Uy row jb+1, mask row jb into the stage
+  __device__ __forceinline__ void start(WideStage<EX, EY, CLEAN>& st, int jb) const {
+    double ux[2], uy[2];
+    unsigned m[2];
+    const bool rx = !EY || (jb >= 0 && jb < nY), ry = !EY || (jb + 1 >= 0 && jb + 1 <= nY);
+    const unsigned sx0 = rx ? (unsigned)jb : 0u, sy1 = ry ? (unsigned)(jb + 1) : 0u;
+    rowd(rx ? bUx : none, oUx, sx0 * (unsigned)sx * 8u, ux);
+    rowd(ry ? bUy : none, oC8, sy1 * (unsigned)nX * 8u, uy);
+    rowm(rx ? bM : none, sx0 * (unsigned)nX, m);
+    st.start(ux, uy, m);
+  }
+
+  __device__ __forceinline__ Ld fetch(int t0) const {
+    const int t = EY ? t0 : min(t0, tEnd);
+    Ld d;
+    // rows outside a field read through a resource of bound 0 (EY only; the
+    // row tests are uniform: scalar selects)
+    const bool r1 = !EY || (t + 1 >= 0 && t + 1 < nY), r2 = !EY || (t + 2 >= 0 && t + 2 <= nY),
+               r0 = !EY || (t >= 0 && t < nY);
+    const unsigned s1 = r1 ? (unsigned)(t + 1) : 0u, s2 = r2 ? (unsigned)(t + 2) : 0u,
+                   s0 = r0 ? (unsigned)t : 0u;
+    rowd(r1 ? bUx : none, oUx, s1 * (unsigned)sx * 8u, d.ux);
+    rowd(r2 ? bUy : none, oC8, s2 * (unsigned)nX * 8u, d.uy);
+    rowm(r1 ? bM : none, s1 * (unsigned)nX, d.m);
+    rowf(r0 ? bVx : none, s0 * (unsigned)sx * 4u, d.vx);
+    rowf(r0 ? bVy : none, s0 * (unsigned)nX * 4u, d.vy);
+    return d;
+  }
+};
+
+// The four-row LDS ring of the wide chains: U and V of both columns of every
+// lane, and the two mask bytes (dirty segments).
+struct WideRing {
+  double ux[4][128];
+  double uy[4][128];
+  float vx[4][128];
+  float vy[4][128];
+  unsigned m[4][64];
+};
+
+template <bool EX, bool EY, bool CLEAN, int NS>
+__device__ __forceinline__ void wide_run(const RveDev& r, int cur, int vpar, int s, int j0,
+                                         int j1, const float* gt, int role, WideRing* ring,
+                                         int nSteps) {
+  using Stage = WideStage<EX, EY, CLEAN>;
+  using In = WideInput<EX, EY, CLEAN>;
+  using Ld = typename In::Ld;
+  const int nX = r.nX, nY = r.nY, sx = nX + 1;
+  const int L = threadIdx.x & 63;
+  const int i0 = s + 2 * L;
+  const int jbA = j0 - 2 * NS, tEnd = j1 + 2 * NS - 3;
+  Stage st;
+  st.setup(r, gt, i0);
+  auto put = [&](WideRing& w, int sl, const typename Stage::Out& o, const unsigned (&m)[2]) {
+    *reinterpret_cast<double2*>(&w.ux[sl][2 * L]) = make_double2(o.ux[0], o.ux[1]);
+    *reinterpret_cast<double2*>(&w.uy[sl][2 * L]) = make_double2(o.uy[0], o.uy[1]);
+    *reinterpret_cast<float2*>(&w.vx[sl][2 * L]) = make_float2(o.vx[0], o.vx[1]);
+    *reinterpret_cast<float2*>(&w.vy[sl][2 * L]) = make_float2(o.vy[0], o.vy[1]);
+    if constexpr (!CLEAN) w.m[sl][L] = m[0] | (m[1] << 8);
+  };
+  // Lockstep step q (0 <= q < nSteps, one barrier each) is row jbA+q of
+  // stage 0 and row jbA+q-3k of stage k.  Each stage runs its steps in three
+  // loops -- idle steps before its first row, its rows, idle steps after its
+  // last -- so the loop over its rows carries only the stage's state, and
+  // the slot of a step (q & 3) is known at compile time in a loop unrolled
+  // by four.
+  auto idle = [&](int q0, int q1) {
+    for (int q = q0; q < q1; q++) lds_barrier();
+  };
+  if (role == 0) {
+    // stage 0: rows jbA..tEnd (steps 0 .. tEnd-jbA) from HBM into ring 0
+    WideRing& out = ring[0];
+    const In in(r, cur, vpar, i0, tEnd);
+    in.start(st, jbA);
+    const int nAct = tEnd - jbA + 1;
+    auto stepA = [&](int q, const Ld& ld, auto sl) {
+      const auto o = st.step(jbA + q, ld.ux, ld.uy, ld.m, ld.vx, ld.vy);
+      put(out, sl, o, ld.m);
+      lds_barrier();
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    Ld P = in.fetch(jbA), Q;
+    int q = 0;
+    for (; q + 4 <= nAct; q += 4) {
+      Q = in.fetch(jbA + q + 1);
+      stepA(q, P, I0{});
+      P = in.fetch(jbA + q + 2);
+      stepA(q + 1, Q, I1{});
+      Q = in.fetch(jbA + q + 3);
+      stepA(q + 2, P, I2{});
+      P = in.fetch(jbA + q + 4);
+      stepA(q + 3, Q, I3{});
+    }
+    if (q < nAct) {
+      Q = in.fetch(jbA + q + 1);
+      stepA(q, P, I0{});
+      if (q + 1 < nAct) {
+        P = in.fetch(jbA + q + 2);
+        stepA(q + 1, Q, I1{});
+        if (q + 2 < nAct) stepA(q + 2, P, I2{});
+      }
+    }
+    idle(nAct, nSteps);
+    return;
+  }
+  auto march = [&](auto K) {
+    constexpr int k = decltype(K)::value, lag = 3 * k;
+    constexpr bool last = k == NS - 1;
+    // first step (rows in from the ring) and last step of this stage
+    constexpr int qStart = 5 * k - 1;
+    const int qLast = tEnd + k - jbA;
+    WideRing& in = ring[k - 1];
+    WideRing& out = ring[last ? k - 1 : k];  // not written by the last stage
+    double* __restrict__ Uxo = r.Ux[cur ^ 1];
+    double* __restrict__ Uyo = r.Uy[cur ^ 1];
+    float* Vxo = static_cast<float*>(vpar ? r.Vx : r.Vx2);
+    float* Vyo = static_cast<float*>(vpar ? r.Vy : r.Vy2);
+    const bool outLane = L >= NS && L < 64 - NS;
+    const unsigned o8 = (unsigned)i0 * 8u, o4 = (unsigned)i0 * 4u;
+    const unsigned vxb = (unsigned)nY * (unsigned)sx * 4u,
+                   vyb = (unsigned)(nY + 1) * (unsigned)nX * 4u;
+    const __amdgpu_buffer_rsrc_t bUx = brsrc(Uxo), bUy = brsrc(Uyo),
+                                 bVx = EX ? brsrc_n(Vxo, vxb) : brsrc(Vxo),
+                                 bVy = EX ? brsrc_n(Vyo, vyb) : brsrc(Vyo);
+    // step q reads Ux' of row j+1 (slot q-2), Uy' of row j+2 (slot q-1), V'
+    // of row j and mask row j+1 (slot q-3); a middle stage writes slot q
+    auto stepK = [&](int q, auto ph) {
+      constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;  // q-1, q-2, q-3
+      const int j = jbA + q - lag;
+      double uxn[2], uynn[2];
+      unsigned mn[2];
+      float vxi[2], vyi[2];
+      {
+        const double2 a = *reinterpret_cast<const double2*>(&in.ux[s2][2 * L]);
+        const double2 b = *reinterpret_cast<const double2*>(&in.uy[s1][2 * L]);
+        uxn[0] = a.x;
+        uxn[1] = a.y;
+        uynn[0] = b.x;
+        uynn[1] = b.y;
+        if constexpr (CLEAN) {
+          mn[0] = mn[1] = 0;
+        } else {
+          const unsigned mm = in.m[s3][L];
+          mn[0] = mm & 0xffu;
+          mn[1] = mm >> 8;
+        }
+        const float2 x = *reinterpret_cast<const float2*>(&in.vx[s3][2 * L]);
+        const float2 y = *reinterpret_cast<const float2*>(&in.vy[s3][2 * L]);
+        vxi[0] = x.x;
+        vxi[1] = x.y;
+        vyi[0] = y.x;
+        vyi[1] = y.y;
+      }
+      // Ux(i+1, j) of this stage's input, before the step
+      const double uxsK[2] = {st.ux_c[1], st.uxs_c1};
+      const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
+      if constexpr (!last) {
+        put(out, ph, o, mn);
+      } else if (j >= j0 && outLane) {
+        const unsigned rowx = (unsigned)j * (unsigned)sx, rowy = (unsigned)j * (unsigned)nX;
+        if constexpr (!EX) {
+          if (!EY || o.updX[0]) bst2f(bVx, o4, rowx * 4u, o.vx);
+          if (!EY || o.updY[0]) bst2f(bVy, o4, rowy * 4u, o.vy);
+          bst2d(bUx, o8, rowx * 8u, o.ux);
+          bst2d(bUy, o8, rowy * 8u, o.uy);
+          if (EY && j == nY - 1) {
+            // Uy row nY: ComputeDisp never updates it
+            const double uyl[2] = {st.uy_c[0], st.uy_c[1]};
+            bst2d(bUy, o8, (rowy + (unsigned)nX) * 8u, uyl);
+          }
+        } else {
+#pragma unroll
+          for (int c = 0; c < 2; c++) {
+            const int i = i0 + c;
+            if (i >= nX) continue;
+            const unsigned oi = (unsigned)i;
+            bst(bVx, o.updX[c] ? oi * 4u : kOob, rowx * 4u, o.vx[c]);
+            bst(bVy, o.updY[c] ? oi * 4u : kOob, rowy * 4u, o.vy[c]);
+            bst(bUx, oi * 8u, rowx * 8u, o.ux[c]);
+            bst(bUy, oi * 8u, rowy * 8u, o.uy[c]);
+            // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+            if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK[c]);
+            if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c[c]);
+          }
+        }
+      }
+      lds_barrier();
+    };
+    idle(0, qStart);
+    {
+      // first rows: Ux' row jb (slot qStart-2), Uy' row jb+1 (slot
+      // qStart-1), mask row jb (slot qStart-3)
+      constexpr int ph = qStart & 3;
+      constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;
+      const double2 a = *reinterpret_cast<const double2*>(&in.ux[s2][2 * L]);
+      const double2 b = *reinterpret_cast<const double2*>(&in.uy[s1][2 * L]);
+      const double ux[2] = {a.x, a.y}, uy[2] = {b.x, b.y};
+      unsigned m[2] = {0u, 0u};
+      if constexpr (!CLEAN) {
+        const unsigned mm = in.m[s3][L];
+        m[0] = mm & 0xffu;
+        m[1] = mm >> 8;
+      }
+      st.start(ux, uy, m);
+      lds_barrier();
+    }
+    constexpr int P0 = (qStart + 1) & 3;
+    using Q0 = std::integral_constant<int, P0>;
+    using Q1 = std::integral_constant<int, (P0 + 1) & 3>;
+    using Q2 = std::integral_constant<int, (P0 + 2) & 3>;
+    using Q3 = std::integral_constant<int, (P0 + 3) & 3>;
+    int q = qStart + 1;
+    for (; q + 3 <= qLast; q += 4) {
+      stepK(q, Q0{});
+      stepK(q + 1, Q1{});
+      stepK(q + 2, Q2{});
+      stepK(q + 3, Q3{});
+    }
+    if (q <= qLast) {
+      stepK(q, Q0{});
+      if (q + 1 <= qLast) {
+        stepK(q + 1, Q1{});
+        if (q + 2 <= qLast) stepK(q + 2, Q2{});
+      }
+    }
+    idle(qLast + 1, nSteps);
+  };
+  static_assert(NS == 3 || NS == 4, "wide chains of three or four stages");
+  if (role == 1) {
+    march(std::integral_constant<int, 1>{});
+  } else if (role == 2) {
+    march(std::integral_constant<int, 2>{});
+  } else {
+    if constexpr (NS == 4) march(std::integral_constant<int, 3>{});
+  }
+}
+
+// One wide segment per workgroup of NS waves (stage k on wave k).  The
+// register budget holds four waves per SIMD (at most 128 VGPRs): the
+// boundary-strip variants need 127 without spills, and one kernel for every
+// variant keeps a launch a single dispatch.
+template <int NS>
+__global__ __launch_bounds__(64 * NS) __attribute__((amdgpu_waves_per_eu(4))) void k_wide(const RveDev* __restrict__ rves,
+                                                  const int4* __restrict__ segs, int nsegs,
+                                                  int parity, int vpar) {
+  __shared__ float gt[NS][16];
+  __shared__ WideRing ring[NS - 1];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int k = tile_of_block(blockIdx.x, gridDim.x);
+  if (k >= nsegs) return;  // the whole workgroup
+  const int4 sg = segs[k];
+  const RveDev& r = rves[sg.x];
+  if (lane < 16) gt[wave][lane] = (float)r.gtab[lane];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wave's table is in LDS
+  __builtin_amdgcn_wave_barrier();
+  const int cur = parity ^ r.cur;
+  const int s = sg.y, j0 = sg.z, j1 = sg.w;
+  const int nSteps = j1 - j0 + 5 * NS - 3;
+  const bool ex = !(s >= 1 && s + 127 <= r.nX - 2);
+  const bool ey = !(j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1));
+  // mask rows the segment reads: j0-2NS .. j1+2NS-2, columns s .. s+127
+  const bool clean = mask_clear(r, s, s + 127, j0 - 2 * NS, j1 + 2 * NS - 2);
+  const float* g = gt[wave];
+  if (!ex && !ey) {
+    if (clean) wide_run<false, false, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    else wide_run<false, false, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  } else if (!ex) {
+    if (clean) wide_run<false, true, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    else wide_run<false, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  } else {
+    if (clean) wide_run<true, true, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    else wide_run<true, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  }
 }
 
 }  // namespace
@@ -2310,6 +3105,43 @@ void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int m
   } else {
     throw std::invalid_argument("chain kernels run 3 or 4 iterations per launch");
   }
+  EP2D_HIP(hipGetLastError());
+}
+
+int wide_segments(int ns_, int nX, int nY, int rows, int rve, int4* out) {
+  if (rows < 1) rows = 1;
+  const int cols = wide_cols(ns_);
+  const int ns = (nX + cols - 1) / cols;
+  int n = 0;
+  for (int j0 = 0; j0 < nY; j0 += rows)
+    for (int k = 0; k < ns; k++) {
+      if (out) out[n] = make_int4(rve, cols * k - 2 * ns_, j0, std::min(j0 + rows, nY));
+      n++;
+    }
+  return n;
+}
+
+void launch_wide(int ns, const RveDev* rves, const int4* segs, int nsegs, int parity, int vpar,
+                 hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (nsegs <= 0) return;
+  const dim3 grid(nsegs), block(64 * ns);
+  auto go = [&](auto kern) {
+    if (e0)
+      hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
+    else
+      hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
+  };
+  if (ns == 3) go(k_wide<3>);
+  else if (ns == 4) go(k_wide<4>);
+  else throw std::invalid_argument("wide chains run 3 or 4 iterations per launch");
+  EP2D_HIP(hipGetLastError());
+}
+
+void launch_mask_map(const unsigned char* mask, int nX, int nY, unsigned char* map,
+                     hipStream_t s) {
+  const int n = (int)mask_map_bytes(nX, nY);
+  hipLaunchKernelGGL(k_mask_map, dim3((n + 255) / 256), dim3(256), 0, s, mask, nX, nY, map,
+                     (nX + kMapCols - 1) / kMapCols, (nY + kMapRows - 1) / kMapRows);
   EP2D_HIP(hipGetLastError());
 }
 

@@ -205,6 +205,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   r_.Vx2 = fused ? alloc(nUx * vbytes) : r_.Vx;
   r_.Vy2 = fused ? alloc(nUy * vbytes) : r_.Vy;
   r_.mask = (unsigned char*)alloc(nC);
+  r_.mmap = (unsigned char*)alloc(mask_map_bytes(nX, nY));
   f_.P = (double*)alloc(nC * 8);
   f_.tauXX = (double*)alloc(nC * 8);
   f_.tauYY = (double*)alloc(nC * 8);
@@ -252,6 +253,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
 
   // SetMaterials / SetMatrixZero (:708-794): masks, zero fields.
   launch_build_mask(const_cast<unsigned char*>(r_.mask), nX, nY, dX_, dY_, rad_, nPores_, stream_);
+  launch_mask_map(r_.mask, nX, nY, const_cast<unsigned char*>(r_.mmap), stream_);
   for (int k = 0; k < 2; k++) {
     launch_fill(r_.Ux[k], nUx, 0.0, stream_);
     launch_fill(r_.Uy[k], nUy, 0.0, stream_);
@@ -290,6 +292,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (ty != 0 && !valid_tile_rows(ty)) throw ArgError("tileRows must be 0, 6, 8, 14 or 30");
     if (ty == 0) ty = default_tile_rows(mode_, nX);
     fplan_ = plan_fused(mode_, opt_.nTasks > 1, ty, nX, nY, device_);
+    if (mode_ == EP2D_MODE_SPLIT)
+      fplan64_ = plan_fused(kModeSplitStress64, opt_.nTasks > 1, 8, nX, nY, device_);
     if (opt_.marchRows < 0) throw ArgError("marchRows must be >= 0");
     if (opt_.itersPerLaunch < 0 || opt_.itersPerLaunch > 4)
       throw ArgError("itersPerLaunch must be 1, 2, 3 or 4");
@@ -297,6 +301,11 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
     ipl_ = opt_.itersPerLaunch >= 2 ? opt_.itersPerLaunch : 1;
     multiIter_ = ipl_ > 1;
+    if (opt_.laneColumns < 0 || opt_.laneColumns > 2)
+      throw ArgError("laneColumns must be 0, 1 or 2");
+    if (opt_.laneColumns == 2 && (mode_ != EP2D_MODE_SPLIT || ipl_ < 3))
+      throw ArgError("laneColumns 2 (wide chains) needs the split scheme and itersPerLaunch 3 or 4");
+    wide_ = opt_.laneColumns == 2;
     if (opt_.marchRows > 0) {
       // row-marching kernel: the parameter table and the segment list live
       // on the device (the kernel reads the current buffer as parity ^ cur)
@@ -312,11 +321,14 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       EP2D_HIP(hipMemcpyAsync(dSegs_, segs.data(), segs.size() * sizeof(int4),
                               hipMemcpyHostToDevice, stream_));
       if (multiIter_) {
-        nSegs2_ = ipl_ == 2 ? march2_segments(nX, nY, marchRows_, 0, nullptr)
-                            : chain_segments(ipl_, nX, nY, marchRows_, 0, nullptr);
+        auto segsOf = [&](int4* out) {
+          if (ipl_ == 2) return march2_segments(nX, nY, marchRows_, 0, out);
+          if (wide_) return wide_segments(ipl_, nX, nY, marchRows_, 0, out);
+          return chain_segments(ipl_, nX, nY, marchRows_, 0, out);
+        };
+        nSegs2_ = segsOf(nullptr);
         std::vector<int4> s2((size_t)nSegs2_);
-        if (ipl_ == 2) march2_segments(nX, nY, marchRows_, 0, s2.data());
-        else chain_segments(ipl_, nX, nY, marchRows_, 0, s2.data());
+        segsOf(s2.data());
         dSegs2_ = (int4*)alloc(s2.size() * sizeof(int4));
         EP2D_HIP(hipMemcpyAsync(dSegs2_, s2.data(), s2.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, stream_));
@@ -396,7 +408,7 @@ size_t Solver::plannedDeviceBytes(int nX, int nY, int mode, double nPores) {
   if (fused) t += a(nUx * 8) + a(nUy * 8);
   t += a(nUx * vb) + a(nUy * vb);
   if (fused) t += a(nUx * vb) + a(nUy * vb);
-  t += a(nC) + 5 * a(nC * 8) + 2 * a(nV * 8);
+  t += a(nC) + a(mask_map_bytes(nX, nY)) + 5 * a(nC * 8) + 2 * a(nV * 8);
   t += a(4 * sizeof(unsigned long long)) + a((size_t)avg_partial_blocks(nX, nY) * NAVG * 8) +
        a(NAVG * 8);
   const size_t g = gather_entries(nX, nY, nPores);
@@ -500,7 +512,10 @@ void Solver::launchIteration() {
     launch_plain_disp(r_, r_.Ux[0], r_.Uy[0], f_, stream_);
     if (timed) EP2D_HIP(hipEventRecord(ev.second, stream_));
   } else {
-    if (marchRows_ > 0)
+    if (stress64_)
+      launch_fused(r_, cur_, kModeSplitStress64, plastic, fplan64_, stream_,
+                   timed ? ev.first : nullptr, timed ? ev.second : nullptr);
+    else if (marchRows_ > 0)
       launch_march(dRve_, dSegs_, nSegs_, mode_, cur_, vIdx_, stream_,
                    timed ? ev.first : nullptr, timed ? ev.second : nullptr);
     else
@@ -540,6 +555,9 @@ void Solver::launchMulti(int q) {
   if (ipl_ == 2)
     launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
                   timed ? ev.first : nullptr, timed ? ev.second : nullptr, true);
+  else if (wide_)
+    launch_wide(ipl_, dRve_, dSegs2_, nSegs2_, cur_, vIdx_ ^ (q & 1), stream_,
+                timed ? ev.first : nullptr, timed ? ev.second : nullptr);
   else
     launch_chain(ipl_, dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
                  timed ? ev.first : nullptr, timed ? ev.second : nullptr);
@@ -561,7 +579,7 @@ void Solver::flipV() {
 
 void Solver::iterate(long n) {
   EP2D_HIP(hipSetDevice(device_));
-  if (!multiIter_) {
+  if (!multiIter_ || stress64_) {
     for (long k = 0; k < n; k++) launchIteration();
     return;
   }
@@ -612,9 +630,10 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
     if (marchRows_ > 0 && multiIter_)
       nm = (ipl_ == 2 ? std::string(mode_ == EP2D_MODE_SPLIT ? "k_march2p<float, float>"
                                                               : "k_march2p<double, double>")
-                      : std::string(mode_ == EP2D_MODE_SPLIT ? "k_chain<float, float, "
-                                                              : "k_chain<double, double, ") +
-                            std::to_string(ipl_) + ">") +
+            : wide_ ? "k_wide<" + std::to_string(ipl_) + ">"
+                    : std::string(mode_ == EP2D_MODE_SPLIT ? "k_chain<float, float, "
+                                                           : "k_chain<double, double, ") +
+                          std::to_string(ipl_) + ">") +
            " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs2_) +
            " (per iteration)";
     else if (marchRows_ > 0)
@@ -878,6 +897,9 @@ void Solver::tsBegin() {
   nextCheck_ = opt_.checkInterval;
   lastCheckAt_ = -1;
   lastCheckErr_ = 0.0;
+  refErr_ = 0.0;
+  sinceHalving_ = 0;
+  stress64_ = false;
 }
 
 long Solver::tsChunk() const {
@@ -909,6 +931,31 @@ void Solver::scheduleNextCheck(double error) {
 
 bool Solver::tsOver() const { return tsConv_ || tsDone_ >= opt_.nIter; }
 
+// Split scheme: the FP32 stresses carry rounding noise of about 2^-24 of
+// their size, which puts a floor under the velocity residual (the floor
+// depends on the grid, the load and the residual's normaliser max(|e0|,
+// max(e1, e2)), EffPlast2D.cu:412, which can be far below the strain of the
+// load).  The pseudo-transient residual does not fall at every check (it
+// rises and falls by factors of two near eIter in FP64 too), but it halves
+// within a few checks; on the floor it stops halving.  When the residual is
+// within 1000 eIter of the target and has not halved over the last eight
+// checks, the rest of the time step runs the FP64 arithmetic on the same
+// state: FP64 U, the FP32 velocity storage.
+void Solver::noteStall(double error) {
+  if (mode_ != EP2D_MODE_SPLIT || opt_.keepFp32Stress || stress64_ || !(error > 0.0)) return;
+  if (!(refErr_ > 0.0) || error < 0.5 * refErr_) {
+    refErr_ = error;
+    sinceHalving_ = 0;
+    return;
+  }
+  if (++sinceHalving_ >= 8 && error < 1e3 * opt_.eIter) {
+    stress64_ = true;
+    escalations_++;
+    say("    Residual stalled under the FP32 stress: FP64 arithmetic from iteration ", tsDone_,
+        "\n");
+  }
+}
+
 // Residual of EffPlast2D.cu:407-412: launch the reduction and the copy of its
 // three words; tsCheckFinish reads them after the stream is synchronised.
 void Solver::tsCheckLaunch() {
@@ -935,6 +982,8 @@ bool Solver::tsCheckFinish() {
     tsConv_ = true;
   } else if (tsDone_ >= opt_.nIter) {
     say("WARNING: Maximum number of iterations reached!\nError is ", error, "\n\n");
+  } else {
+    noteStall(error);
   }
   scheduleNextCheck(error);
   return tsOver();

@@ -73,6 +73,7 @@ class Solver {
   bool jobDone() const { return task_ >= plan_.size(); }
   void tsBegin();                 // start a time step (affine increment)
   void scheduleNextCheck(double error);
+  void noteStall(double error);   // split: switch to the FP64 arithmetic on a stalled residual
   long tsChunk() const;           // iterations to the next check or the cap
   bool tsOver() const;            // converged or iteration cap reached
   long tsDoneIters() const { return tsDone_; }
@@ -88,6 +89,10 @@ class Solver {
   const RveDev& dev() const { return r_; }
   const FusedPlan& fusedPlan() const { return fplan_; }
   int cur() const { return cur_; }
+  // split scheme: this time step finishes in the FP64 arithmetic (the tile
+  // kernel; the batched engine runs such RVEs in a list of their own)
+  bool escalated() const { return stress64_; }
+  long escalations() const { return escalations_; }
   int vBuffer() const { return vIdx_; }
   int mode() const { return mode_; }
   bool plastic() const { return opt_.nTasks > 1; }
@@ -164,6 +169,7 @@ class Solver {
   int4* dSegs_ = nullptr;
   bool multiIter_ = false;          // more than one iteration per launch
   int ipl_ = 1;                     // iterations per launch: 1, 2 (pairs), 3 or 4 (chains)
+  bool wide_ = false;               // chains on two columns per lane (split scheme)
   int nSegs2_ = 0;
   int4* dSegs2_ = nullptr;
   int vIdx_ = 0;                    // velocity buffer holding the current V
@@ -216,6 +222,13 @@ class Solver {
   long nextCheck_ = 0;       // iteration of the next residual check in this time step
   long lastCheckAt_ = -1;    // the previous check and its residual (adaptive schedule)
   double lastCheckErr_ = 0.0;
+  double refErr_ = 0.0;      // the residual at the check where it last halved
+  int sinceHalving_ = 0;     // checks since then
+  long escalations_ = 0;     // time steps finished in the FP64 arithmetic
+  // split scheme: the rest of this time step runs the FP64 arithmetic (FP64
+  // U, FP32 V kept) because the FP32 stress stopped resolving the residual
+  bool stress64_ = false;
+  FusedPlan fplan64_;        // tile plan of that arithmetic
   double tsError_ = 0.0;
   bool tsConv_ = false;
   int jobKind_ = 0;

@@ -113,6 +113,15 @@ typedef struct ep2d_options {
                            residual's decay rate over the last two checks
                            predicts eIter (between checkInterval/64 and
                            checkInterval later)                           */
+  int laneColumns;      /* split scheme with itersPerLaunch 3 or 4: 2 runs the
+                           wide chains (two columns per lane, 128-column
+                           strips), 1 the chains of one column per lane;
+                           0: chosen with the shape (same results)        */
+  int keepFp32Stress;   /* split scheme, 0 (default): a time step whose residual,
+                           within 1000 eIter of the target, has not halved
+                           over eight checks (the FP32 stress no longer
+                           resolves it) finishes with the FP64 arithmetic on
+                           the same FP64 U and FP32 V; 1: never           */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
@@ -305,6 +314,13 @@ typedef struct ep2d_batch_options {
   long timingEvery;      /* time every n-th launch with dispatch events (0: off) */
   int adaptiveCheck;     /* residual checks scheduled by the decay rate
                             (ep2d_options.adaptiveCheck) for every request   */
+  int wide3Rows;         /* fixed three-iteration wide chains (two columns per
+                            lane; split scheme only), segment rows          */
+  int wide4Rows;         /* fixed four-iteration wide chains, segment rows   */
+  int keepFp32Stress;    /* ep2d_options.keepFp32Stress for every request: an
+                            RVE that finishes a time step in the FP64
+                            arithmetic leaves the batched launches for the
+                            rest of that step                                */
 } ep2d_batch_options;
 
 typedef struct ep2d_request {
@@ -331,7 +347,7 @@ typedef struct ep2d_batch_result {
   char message[160];     /* error text when status != EP2D_OK                  */
 } ep2d_batch_result;
 
-#define EP2D_MAX_CAND 24
+#define EP2D_MAX_CAND 32
 
 typedef struct ep2d_batch_stats {
   long long launches, chunks, tilesLaunched, submitted, finished, failed;
@@ -356,6 +372,8 @@ typedef struct ep2d_batch_stats {
   int candRows[EP2D_MAX_CAND];    /* candidate k: tile height / segment rows */
   double candUs[EP2D_MAX_CAND];   /* last tuning, mean us per iteration of
                                      candidate k (0 if not timed)             */
+  long long fp64Finishes;         /* time steps finished in the FP64 arithmetic
+                                     (ep2d_options.keepFp32Stress)            */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;

@@ -136,8 +136,15 @@ def test_converged_1024_fp64_matches_oracle(converged_1024_oracle, kw):
     assert [r["iters"] for r in recs_g] == [r["iters"] for r in recs_o]
     assert all(r["error"] < 1e-8 for r in recs_o)
     for rg, ro in zip(recs_g, recs_o):
-        for k in ["deltaP", "deltaPper", "tauInfty", "dPhi", "holeAreaPi", "sigmaXYclean"]:
+        for k in ["deltaP", "deltaPper", "tauInfty", "dPhi", "holeAreaPi"]:
             assert rg[k] == pytest.approx(ro[k], rel=1e-10, abs=1e-300), k
+        # the mean xy stress of the two isotropic steps is a sum of about a
+        # million values that cancel to rounding level (the oracle sums in
+        # row order, the device in a fixed tree): absolute tolerance 1e-12 of
+        # the stress scale
+        scale = float(np.max(np.abs(ro["sigma"])))
+        assert rg["sigmaXYclean"] == pytest.approx(ro["sigmaXYclean"], rel=1e-10,
+                                                   abs=1e-12 * scale)
     for grp in ["num", "an"]:
         for k in ["Kphi", "Kd"]:
             assert res_g[grp][k] == pytest.approx(res_o[grp][k], rel=1e-8), (grp, k)
