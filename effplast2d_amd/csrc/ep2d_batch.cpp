@@ -22,16 +22,19 @@ namespace {
 constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
-// Kernel shapes the tuner times: the fused tile kernel at two tile heights,
-// the row-marching kernel at one segment height, the two-iteration wave
-// pairs at four (the short segments give a batch of one or two RVEs enough
-// waves), the chains of three and four waves at two and one, and (split
-// scheme) the wide chains of three and four waves at four and two.  Two
-// timed rounds of every shape fit a 200-iteration chunk.
+// Kernel shapes the tuner times: the fused tile kernel, the two-iteration
+// wave pairs at two segment heights, the chains of three and four waves and
+// (split scheme) the wide chains at short and tall segments.  Every launch
+// of a chain kernel runs its workgroups in whole rounds of the resident ones
+// (all its segments take the same time), so the tall heights leave a big
+// batch one or two rounds with few halo rows (20 RVEs of 1024^2: 1760
+// four-wave workgroups of 256 rows fill one round of seven per CU), the
+// short ones give small batches enough workgroups.  Two timed rounds of
+// every shape fit a 200-iteration chunk.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8},   {kTiles, 14},  {kMarch, 32},  {kPair, 8},    {kPair, 14},  {kPair, 26},
-    {kPair, 40},   {kTriple, 42}, {kTriple, 86}, {kQuad, 64},   {kWide3, 32}, {kWide3, 64},
-    {kWide3, 96},  {kWide3, 128}, {kWide4, 64},  {kWide4, 128}};
+    {kTiles, 8},    {kPair, 14},    {kPair, 26},   {kTriple, 42}, {kTriple, 128},
+    {kQuad, 64},    {kQuad, 128},   {kQuad, 256},  {kWide3, 64},  {kWide4, 128},
+    {kWide4, 208}};
 
 namespace {
 

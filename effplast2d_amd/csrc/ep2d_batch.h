@@ -67,7 +67,7 @@ class Batch {
                         // three or four waves, with one or two columns per lane
     int rows = 8;       // tile height or segment rows
   };
-  static constexpr int kNCand = 16;
+  static constexpr int kNCand = 11;
   // refinement of the tuning: up to kRefCands candidates within 3% of the
   // best, kRefRounds rounds of kRefLaunches back-to-back launches each
   static constexpr int kRefCands = 3, kRefRounds = 4, kRefLaunches = 4;

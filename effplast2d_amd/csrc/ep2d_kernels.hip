@@ -1426,11 +1426,15 @@ __global__ __launch_bounds__(256) void k_march(const RveDev* __restrict__ rves,
 // strips of 56 columns.  The arithmetic of each stage is the single-iteration
 // kernels', so two stages give the same bits as two launches.
 //
-// CLEAN stages (segments whose mask bytes are all 0, chain kernels) read no
-// mask: matrix K and G, no pore zeroing and the vertex modulus of four matrix
-// cells, which is what the mask path computes from zero mask bytes.
-template <typename CT, typename VT, bool EDGE, bool CLEAN = false>
+// EX stages run on strips that touch the left or right boundary (per-column
+// tests), EY stages on segments whose rows reach the top or bottom boundary
+// (per-row tests, uniform over the wave); EDGE is either.  CLEAN stages
+// (segments whose mask bytes are all 0, chain kernels) read no mask: matrix
+// K and G, no pore zeroing and the vertex modulus of four matrix cells, which
+// is what the mask path computes from zero mask bytes.
+template <typename CT, typename VT, bool EX, bool EY = EX, bool CLEAN = false>
 struct MarchStage {
+  static constexpr bool EDGE = EX || EY;
   using M = March<CT, VT>;
   const RveDev* r;
   const CT* gt;
@@ -1460,11 +1464,11 @@ struct MarchStage {
     nX = rr.nX;
     nY = rr.nY;
     i = col;
-    colC = !EDGE || (i >= 0 && i < nX);
-    colV = !EDGE || (i >= 0 && i < nX - 1);
+    colC = !EX || (i >= 0 && i < nX);
+    colV = !EX || (i >= 0 && i < nX - 1);
     inX = colC;
-    leftEdge = EDGE && i == 0;
-    rightEdge = EDGE && i == nX - 1;
+    leftEdge = EX && i == 0;
+    rightEdge = EX && i == nX - 1;
     plastic = rr.plastic != 0;
     coh = (CT)rr.coh;
     if constexpr (sizeof(CT) == 8) c2lo = rr.coh2lo;
@@ -1528,7 +1532,7 @@ struct MarchStage {
       } else {
         split_cell_trial(exx, eyy, K, G2, P0, pp, txx, tyy);
       }
-      const bool ok = !EDGE || (colC && j + 1 >= 0 && j + 1 < nY);
+      const bool ok = (!EX || colC) && (!EY || (j + 1 >= 0 && j + 1 < nY));
       if ((!CLEAN && (m_n & M_CELL_ZERO)) || !ok) {
         pp = 0;
         txx = 0;
@@ -1547,7 +1551,7 @@ struct MarchStage {
       CT txy;
       if constexpr (sizeof(CT) == 8) txy = g * (dys(uxs_n - uxs_c) + dxs(uys_n - uy_n));
       else txy = split_vertex_trial(g, (CT)(uxs_n - uxs_c), rdYc, (CT)(uys_n - uy_n), rdXc);
-      const bool ok = !EDGE || (colV && j >= 0 && j < nY - 1);
+      const bool ok = (!EX || colV) && (!EY || (j >= 0 && j < nY - 1));
       if ((!CLEAN && (m_c & M_VERT_ZERO)) || !ok) txy = 0;
       tXY_c = txy;
     }
@@ -1576,27 +1580,36 @@ struct MarchStage {
         }
       }
     } else if (plastic) {
-      const bool vok = !EDGE || (colV && j >= 0 && j < nY - 1);
+      const bool vok = (!EX || colV) && (!EY || (j >= 0 && j < nY - 1));
       const CT sxx = tXX_c + sXX_c + tXX_n + sXX_n;
       const CT syy = tYY_c + sYY_c + tYY_n + sYY_n;
       if (vok) cXY_c = M::vertex_rm(tXY_c, sxx, syy, coh, c2lo);
       const CT pr_p = lane_prev(tXY_p), pr_c = lane_prev(tXY_c);
       {
-        const CT nx_p = lane_next(tXY_p), nx_c = lane_next(tXY_c), nx_pp = lane_next(tXY_pp);
+        // the clamped vertex blocks of the boundary columns (EX only): one
+        // column right and two left
+        CT nx_p = 0, nx_c = 0, nx_pp = 0, pp_p = 0, pp_c = 0, pp_pp = 0;
         const CT pr_pp = lane_prev(tXY_pp);
-        const CT pp_p = lane_prev(pr_p), pp_c = lane_prev(pr_c), pp_pp = lane_prev(pr_pp);
+        if constexpr (EX) {
+          nx_p = lane_next(tXY_p);
+          nx_c = lane_next(tXY_c);
+          nx_pp = lane_next(tXY_pp);
+          pp_p = lane_prev(pr_p);
+          pp_c = lane_prev(pr_c);
+          pp_pp = lane_prev(pr_pp);
+        }
         auto first = [&](CT t, CT pr, CT pp) { return leftEdge ? t : (rightEdge ? pp : pr); };
         auto second = [&](CT t, CT pr, CT nxv) { return leftEdge ? nxv : (rightEdge ? pr : t); };
-        if (colC && j >= 1 && j < nY - 1) {
+        if (colC && (!EY || (j >= 1 && j < nY - 1))) {
           const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
                                     first(tXY_c, pr_c, pp_c) + second(tXY_c, pr_c, nx_c));
           M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
-        } else if (colC && j == nY - 1 && j >= 1) {
+        } else if (EY && colC && j == nY - 1 && j >= 1) {
           const CT av = (CT)0.25 * (first(tXY_pp, pr_pp, pp_pp) + second(tXY_pp, pr_pp, nx_pp) +
                                     first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p));
           M::cell_rm(cXX_c, cYY_c, av, coh, c2lo);
         }
-        if (j == 1) {
+        if (EY && j == 1) {
           CT xx = tXX_p, yy = tYY_p;
           if (colC) {
             const CT av = (CT)0.25 * (first(tXY_p, pr_p, pp_p) + second(tXY_p, pr_p, nx_p) +
@@ -1613,8 +1626,8 @@ struct MarchStage {
     const CT rXY_c = lane_prev(cXY_c), rXY_p = lane_prev(cXY_p);
     Out o;
     CT vx = vx_in, vy = vy_in;
-    o.updX = !EDGE || (i > 0 && i < nX && j > 0 && j < nY - 1);
-    o.updY = !EDGE || (i > 0 && i < nX - 1 && j > 0 && j < nY);
+    o.updX = (!EX || (i > 0 && i < nX)) && (!EY || (j > 0 && j < nY - 1));
+    o.updY = (!EX || (i > 0 && i < nX - 1)) && (!EY || (j > 0 && j < nY));
     if constexpr (sizeof(CT) == 8) {
       if (o.updX)
         vx = (CT)(VT)(vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p)));
@@ -1628,7 +1641,7 @@ struct MarchStage {
         vy = (CT)(VT)split_velocity(vy, vdampY, dTrho, -cP_c + cP_p + cYY_c - cYY_p, rdYc,
                                     cXY_p - rXY_p, rdXc);
     }
-    const bool in = !EDGE || (inX && j >= 0 && j < nY);
+    const bool in = (!EX || inX) && (!EY || (j >= 0 && j < nY));
     o.vx = vx;
     o.vy = vy;
     if constexpr (sizeof(CT) == 8) {
@@ -1668,8 +1681,9 @@ struct MarchStage {
 // segments bound each resource to its field: lanes outside the field's
 // columns carry the offset kOob, and a row outside the field reads through a
 // resource of bound 0, so every such load returns 0 without a branch.
-template <typename CT, typename VT, bool EDGE, bool CLEAN = false>
+template <typename CT, typename VT, bool EX, bool EY = EX, bool CLEAN = false>
 struct StageInput {
+  static constexpr bool EDGE = EX || EY;
   using Ld = typename March<CT, VT>::Ld;
   const double* Uxi;
   const double* Uyi;
@@ -1711,7 +1725,7 @@ struct StageInput {
   }
 
   // Ux row jb, Uy row jb+1, mask row jb into the stage
-  __device__ __forceinline__ void start(MarchStage<CT, VT, EDGE, CLEAN>& st, int jb) const {
+  __device__ __forceinline__ void start(MarchStage<CT, VT, EX, EY, CLEAN>& st, int jb) const {
     double ux = 0.0, uy = 0.0;
     unsigned m = 0;
     const bool colC = !EDGE || (i >= 0 && i < nX);
@@ -2046,13 +2060,14 @@ __device__ __forceinline__ bool mask_clear(const RveDev& r, int ilo, int ihi, in
 // row j+1 (slot q-3) of its ring; a middle stage writes its row and that mask
 // row to slot q of the next ring.  Reads and the write of one step never share
 // a slot, and the barrier after each step orders a write before its reads.
-template <typename CT, typename VT, bool EDGE, bool CLEAN, int NS>
+template <typename CT, typename VT, bool EX, bool EY, bool CLEAN, int NS>
 __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, int s, int j0,
                                           int j1, const CT* gt, int role, RowRing<CT>* ring,
                                           int nSteps) {
+  constexpr bool EDGE = EX || EY;
   using M = March<CT, VT>;
   using Ld = typename M::Ld;
-  using Stage = MarchStage<CT, VT, EDGE, CLEAN>;
+  using Stage = MarchStage<CT, VT, EX, EY, CLEAN>;
   const int nX = r.nX, nY = r.nY, sx = nX + 1;
   const int L = threadIdx.x & 63;
   const int i = s + L;
@@ -2073,7 +2088,7 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   if (role == 0) {
     // stage 0: rows jbA..tEnd (steps 0 .. tEnd-jbA) from HBM into ring 0
     RowRing<CT>& out = ring[0];
-    const StageInput<CT, VT, EDGE, CLEAN> in(r, cur, vpar, i, tEnd);
+    const StageInput<CT, VT, EX, EY, CLEAN> in(r, cur, vpar, i, tEnd);
     in.start(st, jbA);
     const int nAct = tEnd - jbA + 1;
     auto stepA = [&](int q, const Ld& ld, auto sl) {
@@ -2124,7 +2139,7 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
     double* __restrict__ Uyo = r.Uy[cur ^ 1];
     VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
     VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
-    const bool outLane = L >= 2 * NS && L < 64 - 2 * NS && (!EDGE || i < nX);
+    const bool outLane = L >= 2 * NS && L < 64 - 2 * NS && (!EX || i < nX);
     const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
     // edge segments bound the V resources to their fields: a lane whose V
     // ComputeDisp leaves alone stores at kOob, which the hardware drops
@@ -2155,11 +2170,9 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
         bst(bVy, ovy, (unsigned)j * (unsigned)nX * (unsigned)sizeof(VT), (VT)o.vy);
         bst(bUx, o8, (unsigned)j * (unsigned)sx * 8u, o.ux);
         bst(bUy, o8, (unsigned)j * (unsigned)nX * 8u, o.uy);
-        if (EDGE) {
-          // nodes ComputeDisp never updates: Ux column nX, Uy row nY
-          if (i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK);
-          if (j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
-        }
+        // nodes ComputeDisp never updates: Ux column nX, Uy row nY
+        if (EX && i == nX - 1) gst(Uxo, (unsigned)(j * sx + i + 1), uxsK);
+        if (EY && j == nY - 1) gst(Uyo, (unsigned)((j + 1) * nX + i), st.uy_c);  // row j+1 after the step
       }
       lds_barrier();
     };
@@ -2224,15 +2237,23 @@ __attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? 7 : 1))) void k_chain(const
   const int cur = parity ^ r.cur;
   const int s = sg.y, j0 = sg.z, j1 = sg.w;
   const int nSteps = j1 - j0 + 5 * NS - 3;
-  const bool interior = s >= 1 && s + 63 <= r.nX - 2 && j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1);
-  // mask rows the segment reads: j0-2NS .. j1+2NS-2, columns s .. s+63
-  const bool clean = interior && mask_clear(r, s, s + 63, j0 - 2 * NS, j1 + 2 * NS - 2);
-  if (clean)
-    chain_run<CT, VT, false, true, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
-  else if (interior)
-    chain_run<CT, VT, false, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+  // column edge (strips touching the left or right boundary) and row edge
+  // (segments whose rows reach the top or bottom), and whether every mask
+  // byte the segment reads (rows j0-2NS .. j1+2NS-2, columns s .. s+63) is 0
+  const bool ex = !(s >= 1 && s + 63 <= r.nX - 2);
+  const bool ey = !(j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1));
+  const bool clean = !ex && mask_clear(r, s, s + 63, j0 - 2 * NS, j1 + 2 * NS - 2);
+  const CT* g = gt[wave];
+  if (ex)
+    chain_run<CT, VT, true, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  else if (ey && clean)
+    chain_run<CT, VT, false, true, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  else if (ey)
+    chain_run<CT, VT, false, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+  else if (clean)
+    chain_run<CT, VT, false, false, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
   else
-    chain_run<CT, VT, true, false, NS>(r, cur, vpar, s, j0, j1, gt[wave], wave, ring, nSteps);
+    chain_run<CT, VT, false, false, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
 }
 
 // ---------------------------------------------------------------------------

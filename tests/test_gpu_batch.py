@@ -259,7 +259,7 @@ def test_chains_full_size_equal_tiles(mode):
             return b.poll()[0], b.stats()
 
     ref, _ = run(tileRows=14)
-    for shape in ({"tripleRows": 86}, {"tripleRows": 42}, {"quadRows": 64}):
+    for shape in ({"tripleRows": 86}, {"tripleRows": 42}, {"quadRows": 64}, {"quadRows": 256}):
         got, st = run(**shape)
         assert st["kernel"] == (6 if "tripleRows" in shape else 7)
         _same(got["result"], ref["result"])
