@@ -192,6 +192,9 @@ SIGNATURES = {
     "ep2d_iterate": ([C.c_void_p, C.c_long], C.c_int),
     "ep2d_apply_affine": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double], C.c_int),
     "ep2d_residual": ([C.c_void_p, _dp, _dp, C.POINTER(C.c_long)], C.c_int),
+    "ep2d_residual_noalloc": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                               C.c_double, C.c_double, _dp, C.c_void_p, C.c_void_p,
+                               C.c_void_p], C.c_int),
     "ep2d_averages": ([C.c_void_p, C.POINTER(Record)], C.c_int),
     "ep2d_boundary": ([C.c_void_p, _dp, _dp, _dp], C.c_int),
     "ep2d_hole_area": ([C.c_void_p, C.POINTER(Record)], C.c_int),

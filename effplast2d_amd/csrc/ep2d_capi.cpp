@@ -58,6 +58,14 @@ int guarded(F&& f) {
   }
 }
 
+}  // namespace
+
+namespace ep2d {
+// The message ep2d_last_error() returns, for entry points outside this file.
+void record_error(const char* msg) { set_error(msg); }
+}  // namespace ep2d
+
+namespace {
 ep2d::Solver* S(ep2d_handle* h) {
   if (!h || !h->solver) throw ep2d::ArgError("null handle");
   return h->solver;

@@ -238,6 +238,21 @@ int ep2d_apply_affine(ep2d_handle* h, double dUxdx, double dUydy,
  * max(|e0|, max(e1, e2)); nonfinite receives the count of NaN/Inf values. */
 int ep2d_residual(ep2d_handle* h, const double curEffStrain[3], double* error,
                   long* nonfinite);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops), the residual of
+ * ep2d_residual on velocity fields the caller owns.  Vx is (nX+1)*nY and Vy
+ * nX*(nY+1) values (float when velocityF32, else double) in the reference
+ * layout; dX, dY, dT as in pa.  Enqueued on `stream` (a hipStream_t, NULL:
+ * the null stream): writes the residual to the device double *error and,
+ * when nonfinite is not NULL, the count of NaN/Inf values to the device
+ * word *nonfinite; nothing is allocated and nothing is synchronised.  The
+ * reduction's scratch is a fixed device array of 4 words per GPU (32 bytes,
+ * held until the process ends) that every call on that GPU shares: calls
+ * that overlap on one GPU race.  nX, nY in [3, EP2D_NOALLOC_MAX_SIDE]. */
+#define EP2D_NOALLOC_MAX_SIDE 32768
+int ep2d_residual_noalloc(const void* Vx, const void* Vy, int nX, int nY, int velocityF32,
+                          double dX, double dY, double dT, const double curEffStrain[3],
+                          double* error, unsigned long long* nonfinite, void* stream);
 /* Volume averages (:453-503) on the device in FP64, plus sigmaXYclean. */
 int ep2d_averages(ep2d_handle* h, ep2d_record* rec);
 /* getDeltaP_honest, getDeltaP_periodic, getTauInfty_honest (:825-882, :925-942). */
