@@ -632,14 +632,18 @@ def pmc_traffic(fetch_dir, write_dir, kernel_key, copy_mib, dest, command):
     print(text)
 
 
-def sq_summary(sq_dir, kernel_key, dest, command):
+def sq_summary(sq_dir, kernel_key, dest, command, points=None):
     """VALU issue of the kernel whose name contains kernel_key, from one
     counter run of VALUBusy (rocprofv3's derived metric: SQ_ACTIVE_INST_VALU
     summed over the SIMDs / CUs / GRBM_GUI_ACTIVE, the percentage of cycles
-    a SIMD issues VALU instructions), SQ_INSTS_VALU and SQ_WAVES."""
+    a SIMD issues VALU instructions), SQ_INSTS_VALU and SQ_WAVES, and, when
+    the run collected them, SQ_WAIT_ANY and SQ_WAVE_CYCLES (the share of a
+    wave's lifetime spent waiting on a counter or barrier).  points: point
+    iterations one dispatch computes (the lane instructions per point
+    iteration are then reported)."""
     path = glob.glob(os.path.join(sq_dir, "**", "*counter_collection.csv"), recursive=True)[0]
     vals = {}
-    for c in ("VALUBusy", "SQ_INSTS_VALU", "SQ_WAVES"):
+    for c in ("VALUBusy", "SQ_INSTS_VALU", "SQ_WAVES", "SQ_WAIT_ANY", "SQ_WAVE_CYCLES"):
         m = _counter_means(path, c)
         vals[c] = next((v for k, v in m.items() if kernel_key in k), None)
     busy = vals["VALUBusy"]
@@ -649,8 +653,15 @@ def sq_summary(sq_dir, kernel_key, dest, command):
              f"SQ_INSTS_VALU per dispatch = {vals['SQ_INSTS_VALU']}",
              f"SQ_WAVES per dispatch = {vals['SQ_WAVES']}",
              f"valu_insts_per_wave = "
-             f"{vals['SQ_INSTS_VALU'] / vals['SQ_WAVES'] if vals['SQ_WAVES'] else None}",
-             f"valu_busy_frac = {busy / 100.0 if busy is not None else None}"]
+             f"{vals['SQ_INSTS_VALU'] / vals['SQ_WAVES'] if vals['SQ_WAVES'] else None}"]
+    if vals["SQ_WAIT_ANY"] is not None and vals["SQ_WAVE_CYCLES"]:
+        lines.append(f"SQ_WAIT_ANY / SQ_WAVE_CYCLES (share of wave lifetime waiting) = "
+                     f"{vals['SQ_WAIT_ANY'] / vals['SQ_WAVE_CYCLES']:.4f}")
+    if points and vals["SQ_INSTS_VALU"]:
+        lines.append(f"point iterations per dispatch = {points}")
+        lines.append(f"valu_lane_insts_per_point_iteration = "
+                     f"{vals['SQ_INSTS_VALU'] * 64.0 / points:.1f}")
+    lines.append(f"valu_busy_frac = {busy / 100.0 if busy is not None else None}")
     text = "\n".join(lines) + "\n"
     if dest:
         with open(dest, "w") as fh:
@@ -670,7 +681,7 @@ def run_rve4096(check, nIter=100_000):
     out = {}
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
-        # automatic single-RVE shape: three-wave chains (split), wave pairs (FP64)
+        # automatic single-RVE shape: four-wave chains (split), wave pairs (FP64)
         # verbose: the reference's console lines, one per residual check
         with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
                         mode=mode, shearLoadType=LOAD["shearLoadType"], verbose=True) as g:
@@ -702,7 +713,7 @@ def run_rve4096(check, nIter=100_000):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8, help="requests in the timed batch")
+    ap.add_argument("--steps", type=int, default=20, help="requests in the timed batch")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="split", choices=["fp64", "split"],
                     help="scheme of the batched headline")
@@ -761,6 +772,8 @@ def main():
     ap.add_argument("--pmc-kernel", default="k_fused<double")
     ap.add_argument("--sq-summary", default=None, metavar="SQ_DIR",
                     help="write the VALU issue summary of --pmc-kernel from a counter run")
+    ap.add_argument("--sq-points", type=float, default=0,
+                    help="point iterations per dispatch of --pmc-kernel (for --sq-summary)")
     ap.add_argument("--pmc-command", default="")
     a = ap.parse_args()
 
@@ -769,7 +782,8 @@ def main():
                     a.summary_out, a.pmc_command)
         return 0
     if a.sq_summary:
-        sq_summary(a.sq_summary, a.pmc_kernel, a.summary_out, a.pmc_command)
+        sq_summary(a.sq_summary, a.pmc_kernel, a.summary_out, a.pmc_command,
+                   a.sq_points or None)
         return 0
     if a.summarize:
         summarize(a.summarize, a.summary_out, a.pmc_command)
