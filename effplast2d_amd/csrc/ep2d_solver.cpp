@@ -23,10 +23,10 @@ namespace {
 double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max semantics
 
 // Tile height of the fused kernels when the caller does not choose one.
+// FP64 from 1024^2 on: tiles of 14 rows (16.5 against 19.3 us per
+// iteration for 8 rows at 1024^2, one RVE); 8 rows otherwise.
 int default_tile_rows(int mode, int nX) {
-  (void)mode;
-  (void)nX;
-  return 8;
+  return mode == EP2D_MODE_FP64 && nX >= 1024 ? 14 : 8;
 }
 }  // namespace
 
@@ -274,14 +274,16 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   buildGatherTable();
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
     // automatic shape for one RVE, from one-RVE measurements: split from
-    // 1024^2 on chains of waves (four iterations per launch and 16-row
-    // segments at 1024^2, 32 at 2048^2; three and 56 rows beyond), FP64 from
-    // 2048^2 on wave pairs (26 rows); the tile kernel below those sizes.
-    // Segments stay short enough to give the GPU waves.
+    // 1024^2 on chains of four waves (four iterations per launch; 16-row
+    // segments at 1024^2, 32 beyond: 10.2, 23.4 and 76.3 us per iteration
+    // at 1024^2, 2048^2 and 4096^2, against 13.9, 37.3 and 177.7 for the
+    // tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the tile
+    // kernel below those sizes.  Segments stay short enough to give the GPU
+    // waves.
     const int n = std::max(nX, nY);
     if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 16 : (n <= 2048 ? 32 : 56);
-      opt_.itersPerLaunch = n <= 2048 ? 4 : 3;
+      opt_.marchRows = n <= 1024 ? 16 : 32;
+      opt_.itersPerLaunch = 4;
     } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {
       opt_.marchRows = 26;
       opt_.itersPerLaunch = 2;
