@@ -148,7 +148,7 @@ def percentile(xs, q):
 
 def shape_name(kind, rows):
     names = {1: 'tiles', 2: 'march', 3: 'persist', 4: 'march2x', 5: 'pair', 6: 'triple', 7: 'quad',
-             8: 'wide3x', 9: 'wide4x'}
+             8: 'wide3x', 9: 'wide4x', 11: 'triple2w', 12: 'quad2w'}
     return f"{names[kind]}{rows}"
 
 
@@ -172,7 +172,7 @@ def batch_stats_summary(st, mode):
     instead: the HBM rate a one-iteration kernel would need to match the
     launch's iteration rate (above the peak for n > 1)."""
     nan = float("nan")
-    ipl = {4: 2, 5: 2, 6: 3, 7: 4, 8: 3, 9: 4}.get(st["kernel"], 1)   # iterations per launch
+    ipl = {4: 2, 5: 2, 6: 3, 7: 4, 8: 3, 9: 4, 11: 3, 12: 4}.get(st["kernel"], 1)  # per launch
     it_disp = st["timedMs"] / st["timedLaunches"] if st["timedLaunches"] else nan
     it_graph = st["graphMs"] / st["graphLaunches"] if st["graphLaunches"] else None
     # timedBytes counts the algorithmic bytes of each iteration
@@ -181,8 +181,9 @@ def batch_stats_summary(st, mode):
     ct = "float" if mode == "split" else "double"
     kern = {1: f"k_fused_batch<{ct}, {ct}, TY>", 2: f"k_march<{ct}, {ct}>",
             3: f"k_fused_persist<{ct}, {ct}, TY>", 4: f"k_march2<{ct}, {ct}>",
-            5: f"k_march2p<{ct}, {ct}>", 6: f"k_chain<{ct}, {ct}, 3>",
-            7: f"k_chain<{ct}, {ct}, 4>", 8: "k_wide<3>", 9: "k_wide<4>"}[st["kernel"]]
+            5: f"k_march2p<{ct}, {ct}>", 6: f"k_chain<{ct}, {ct}, 3, 1>",
+            7: f"k_chain<{ct}, {ct}, 4, 1>", 8: "k_wide<3>", 9: "k_wide<4>",
+            11: f"k_chain<{ct}, {ct}, 3, 2>", 12: f"k_chain<{ct}, {ct}, 4, 2>"}[st["kernel"]]
     return {"kernel": kern, "iterations_per_launch": ipl,
             "kernel_avg_us": kms * 1e3,
             "kernel_timing": "graph replays" if it_graph is not None else "dispatch events",
@@ -491,7 +492,8 @@ def probe(mode, nx, iters, copy_mib, batch=0, shape="tiles8"):
 
 SHAPE_OPTION = {"tiles": "tileRows", "march": "marchRows", "persist": "persistRows",
                 "march2x": "march2Rows", "pair": "pairRows", "triple": "tripleRows",
-                "quad": "quadRows", "wide3x": "wide3Rows", "wide4x": "wide4Rows"}
+                "quad": "quadRows", "wide3x": "wide3Rows", "wide4x": "wide4Rows",
+                "triple2w": "triple2Rows", "quad2w": "quad2Rows"}
 
 
 def probe_shapes(mode, nx, iters, batch, shapes):
@@ -519,7 +521,9 @@ def single_shape(name):
             "triple": {"marchRows": n, "itersPerLaunch": 3},
             "quad": {"marchRows": n, "itersPerLaunch": 4},
             "wide3x": {"marchRows": n, "itersPerLaunch": 3, "laneColumns": 2},
-            "wide4x": {"marchRows": n, "itersPerLaunch": 4, "laneColumns": 2}}[kind]
+            "wide4x": {"marchRows": n, "itersPerLaunch": 4, "laneColumns": 2},
+            "triple2w": {"marchRows": n, "itersPerLaunch": 3, "stageWaves": 2},
+            "quad2w": {"marchRows": n, "itersPerLaunch": 4, "stageWaves": 2}}[kind]
 
 
 def sweep(sizes, modes, shapes, iters):
@@ -684,7 +688,8 @@ def run_rve4096(check, nIter=100_000, initLoad=None):
     out = {}
     for mode in ("split", "fp64"):
         t0 = time.perf_counter()
-        # automatic single-RVE shape: four-wave chains (split), wave pairs (FP64)
+        # automatic single-RVE shape: four-stage chains on two waves per stage
+        # (split), wave pairs (FP64)
         # verbose: the reference's console lines, one per residual check
         with EffPlast2D(pa, nx, nTasks=3, nIter=nIter, eIter=1e-8, checkInterval=check,
                         mode=mode, shearLoadType=LOAD["shearLoadType"], verbose=True) as g:

@@ -47,7 +47,7 @@ class Options(C.Structure):
                 ("shearLoadType", C.c_double * 3), ("verbose", C.c_int),
                 ("logPath", C.c_char_p), ("tileRows", C.c_int), ("marchRows", C.c_int),
                 ("itersPerLaunch", C.c_int), ("adaptiveCheck", C.c_int),
-                ("laneColumns", C.c_int), ("keepFp32Stress", C.c_int)]
+                ("laneColumns", C.c_int), ("keepFp32Stress", C.c_int), ("stageWaves", C.c_int)]
 
 
 class Moduli(C.Structure):
@@ -98,7 +98,8 @@ class BatchOptions(C.Structure):
                 ("pairRows", C.c_int), ("tripleRows", C.c_int), ("quadRows", C.c_int),
                 ("device", C.c_int), ("memoryBudget", C.c_size_t), ("maxActive", C.c_int),
                 ("useGraphs", C.c_int), ("timingEvery", C.c_long), ("adaptiveCheck", C.c_int),
-                ("wide3Rows", C.c_int), ("wide4Rows", C.c_int), ("keepFp32Stress", C.c_int)]
+                ("wide3Rows", C.c_int), ("wide4Rows", C.c_int), ("keepFp32Stress", C.c_int),
+                ("triple2Rows", C.c_int), ("quad2Rows", C.c_int)]
 
 
 class Request(C.Structure):

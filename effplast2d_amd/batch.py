@@ -22,7 +22,7 @@ class Batch:
                  march2Rows=0, pairRows=0, tripleRows=0, quadRows=0, device=0, memoryBudget=0,
                  maxActive=0, useGraphs=True,
                  timingEvery=0, adaptiveCheck=False, wide3Rows=0, wide4Rows=0,
-                 keepFp32Stress=False):
+                 keepFp32Stress=False, triple2Rows=0, quad2Rows=0):
         o = L.BatchOptions()
         lib().ep2d_batch_default_options(C.byref(o))
         o.mode = L.MODES[mode] if isinstance(mode, str) else int(mode)
@@ -37,6 +37,8 @@ class Batch:
         o.wide3Rows = int(wide3Rows)
         o.wide4Rows = int(wide4Rows)
         o.keepFp32Stress = 1 if keepFp32Stress else 0
+        o.triple2Rows = int(triple2Rows)
+        o.quad2Rows = int(quad2Rows)
         o.device = int(device)
         o.memoryBudget = int(memoryBudget)
         o.maxActive = int(maxActive)

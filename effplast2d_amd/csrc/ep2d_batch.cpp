@@ -23,18 +23,18 @@ constexpr int kTileX = 62;  // fused tile width (Fused::TX)
 }  // namespace
 
 // Kernel shapes the tuner times: the fused tile kernel, the two-iteration
-// wave pairs at two segment heights, the chains of three and four waves and
-// (split scheme) the wide chains at short and tall segments.  Every launch
-// of a chain kernel runs its workgroups in whole rounds of the resident ones
-// (all its segments take the same time), so the tall heights leave a big
-// batch one or two rounds with few halo rows (20 RVEs of 1024^2: 1760
-// four-wave workgroups of 256 rows fill one round of seven per CU), the
-// short ones give small batches enough workgroups.  Two timed rounds of
-// every shape fit a 200-iteration chunk.
+// wave pairs at two segment heights, the chains of three and four stages on
+// one wave per stage, and the four-stage chains on two waves per stage (fewer
+// halo columns per output, six waves per SIMD instead of seven).  Short
+// segments give small batches enough workgroups; tall ones have fewer halo
+// rows.  The wide chains (slower on the batches measured) and the three-
+// stage chains on two waves per stage (six-wave workgroups spread unevenly
+// over a CU's four SIMDs) stay fixed shapes.  Two timed rounds of every
+// shape fit a 200-iteration chunk.
 const Batch::Shape Batch::kCand[Batch::kNCand] = {
-    {kTiles, 8},    {kPair, 14},    {kPair, 26},   {kTriple, 42}, {kTriple, 128},
-    {kQuad, 64},    {kQuad, 128},   {kQuad, 256},  {kWide3, 64},  {kWide4, 128},
-    {kWide4, 208}};
+    {kTiles, 8},    {kPair, 14},    {kPair, 26},    {kTriple, 42},  {kTriple, 128},
+    {kQuad, 64},    {kQuad, 128},   {kQuad2, 48},   {kQuad2, 64},   {kQuad2, 96},
+    {kQuad2, 128}};
 
 namespace {
 
@@ -59,10 +59,11 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
     throw ArgError("wide3Rows and wide4Rows (wide chains) need the split scheme");
   const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +
                      (opt_.march2Rows != 0) + (opt_.pairRows != 0) + (opt_.tripleRows != 0) +
-                     (opt_.quadRows != 0) + (opt_.wide3Rows != 0) + (opt_.wide4Rows != 0);
+                     (opt_.quadRows != 0) + (opt_.wide3Rows != 0) + (opt_.wide4Rows != 0) +
+                     (opt_.triple2Rows != 0) + (opt_.quad2Rows != 0);
   if (nfixed > 1)
     throw ArgError("give at most one of tileRows, marchRows, persistRows, march2Rows, pairRows, "
-                   "tripleRows, quadRows, wide3Rows, wide4Rows");
+                   "tripleRows, quadRows, wide3Rows, wide4Rows, triple2Rows, quad2Rows");
   fixed_ = nfixed == 1;
   if (opt_.marchRows) shape_ = {kMarch, opt_.marchRows};
   else if (opt_.tileRows) shape_ = {kTiles, opt_.tileRows};
@@ -73,6 +74,8 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   else if (opt_.quadRows) shape_ = {kQuad, opt_.quadRows};
   else if (opt_.wide3Rows) shape_ = {kWide3, opt_.wide3Rows};
   else if (opt_.wide4Rows) shape_ = {kWide4, opt_.wide4Rows};
+  else if (opt_.triple2Rows) shape_ = {kTriple2, opt_.triple2Rows};
+  else if (opt_.quad2Rows) shape_ = {kQuad2, opt_.quad2Rows};
   device_ = opt_.device;
   int ndev = 0;
   EP2D_HIP(hipGetDeviceCount(&ndev));
@@ -263,7 +266,8 @@ int Batch::buildWork(Shape sh, int4* out) const {
       continue;
     }
     if (chain(sh.kind)) {
-      t += chain_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
+      t += chain_segments(ipl(sh.kind), s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr,
+                          stageWaves(sh.kind));
       continue;
     }
     if (wide(sh.kind)) {
@@ -296,7 +300,7 @@ void Batch::launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long 
                       hipEvent_t e1) {
   if (chain(sh.kind) && dbl)
     launch_chain(ipl(sh.kind), dRves_, work, n, opt_.mode, (int)(l & 1), (int)(d & 1), stream_,
-                 e0, e1);
+                 e0, e1, stageWaves(sh.kind));
   else if (wide(sh.kind) && dbl)
     launch_wide(ipl(sh.kind), dRves_, work, n, (int)(l & 1), (int)(d & 1), stream_, e0, e1);
   else if (multiIter(sh.kind) && dbl)

@@ -47,24 +47,31 @@ class Batch {
   };
   static constexpr int kTiles = 1, kMarch = 2, kPersist = 3, kMarch2 = 4, kPair = 5,
                        kTriple = 6, kQuad = 7, kWide3 = 8, kWide4 = 9;
+  // the chains with two waves side by side per stage
+  static constexpr int kTriple2 = 11, kQuad2 = 12;
   // the list of the RVEs finishing a time step in the FP64 arithmetic (tiles
   // of 8 rows for the batched tile kernel); not a shape the tuner times
   static constexpr int kEsc = 10;
   // iterations per launch of a kernel kind; a kind with more than one runs
   // the chunk's last iterations through the one-iteration marching kernel
   static int ipl(int kind) {
-    return kind == kQuad || kind == kWide4 ? 4
-           : kind == kTriple || kind == kWide3 ? 3
+    return kind == kQuad || kind == kWide4 || kind == kQuad2 ? 4
+           : kind == kTriple || kind == kWide3 || kind == kTriple2 ? 3
            : (kind == kMarch2 || kind == kPair ? 2 : 1);
   }
-  static bool chain(int kind) { return kind == kTriple || kind == kQuad; }
+  static bool chain(int kind) {
+    return kind == kTriple || kind == kQuad || kind == kTriple2 || kind == kQuad2;
+  }
+  // waves per stage of a chain kind
+  static int stageWaves(int kind) { return kind == kTriple2 || kind == kQuad2 ? 2 : 1; }
   static bool wide(int kind) { return kind == kWide3 || kind == kWide4; }
   static bool multiIter(int kind) { return ipl(kind) > 1; }
   struct Shape {
     int kind = kTiles;  // fused tile, row-marching, persistent tile, two-iteration
                         // marching on one wave or on a pair of waves,
                         // three- or four-iteration marching on a chain of
-                        // three or four waves, with one or two columns per lane
+                        // three or four stages, each on one or two waves, or
+                        // on one wave with two columns per lane
     int rows = 8;       // tile height or segment rows
   };
   static constexpr int kNCand = 11;

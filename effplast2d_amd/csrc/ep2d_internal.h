@@ -154,13 +154,15 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out);
 void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                    int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                    bool pairs = false);
-// ns = 3 or 4 iterations per launch on chains of ns waves (one per
-// iteration, rows handed over in LDS): strips of 64 - 4 ns output columns; U
-// and V buffers as for launch_march2.
-constexpr int chain_cols(int ns) { return 64 - 4 * ns; }
-int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out);
+// ns = 3 or 4 iterations per launch on chains of ns stages (one per
+// iteration, rows handed over in LDS), each stage on w = 1 or 2 waves side
+// by side: strips of 60 w + 4 - 4 ns output columns; U and V buffers as for
+// launch_march2.
+constexpr int chain_cols(int ns, int w = 1) { return 60 * w + 4 - 4 * ns; }
+int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out, int w = 1);
 void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                  int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+                  int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
+                  int w = 1);
 // Wide chains (split scheme only): the chains with two columns per lane,
 // strips of 128 - 4 ns output columns; same segments order, U and V buffers
 // and results as launch_chain.

@@ -1865,13 +1865,14 @@ __global__ __launch_bounds__(256) void k_march2(const RveDev* __restrict__ rves,
 //
 // RowRing: the four-row LDS ring through which a stage hands its output rows
 // to the next (also the ring of the wave chains below).
-template <typename CT>
+// W waves side by side per stage (the chains): rows of 64 W entries.
+template <typename CT, int W = 1>
 struct RowRing {
-  double ux[4][64];
-  double uy[4][64];
-  CT vx[4][64];
-  CT vy[4][64];
-  unsigned m[4][64];
+  double ux[4][64 * W];
+  double uy[4][64 * W];
+  CT vx[4][64 * W];
+  CT vy[4][64 * W];
+  unsigned m[4][64 * W];
 };
 
 // Lockstep step q (0 <= q < nSteps) is row t = j0-4+q of stage A and row
@@ -2060,17 +2061,29 @@ __device__ __forceinline__ bool mask_clear(const RveDev& r, int ilo, int ihi, in
 // row j+1 (slot q-3) of its ring; a middle stage writes its row and that mask
 // row to slot q of the next ring.  Reads and the write of one step never share
 // a slot, and the barrier after each step orders a write before its reads.
-template <typename CT, typename VT, bool EX, bool EY, bool CLEAN, int NS>
+//
+// With W > 1 waves per stage, wave c of a stage takes columns s+60c .. s+60c+63
+// (neighbouring waves overlap by four columns) and the stage's rows span the
+// W waves' union of 60W+4 columns: a wave reads its 64 entries of the ring row
+// at position 60c+L, so its input halo comes from its neighbours' outputs,
+// and writes the entries it computes exactly (lanes 2..61, the union's outer
+// lanes 0, 1 and 62, 63 included), each position once.  Only the union's
+// outer edges lose two columns per stage: strips of 60W+4-4NS outputs.
+template <typename CT, typename VT, bool EX, bool EY, bool CLEAN, int NS, int W>
 __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, int s, int j0,
-                                          int j1, const CT* gt, int role, RowRing<CT>* ring,
-                                          int nSteps) {
+                                          int j1, const CT* gt, int role, int c,
+                                          RowRing<CT, W>* ring, int nSteps) {
   constexpr bool EDGE = EX || EY;
+  constexpr int UW = 60 * W + 4;  // columns of the union of a stage's waves
   using M = March<CT, VT>;
   using Ld = typename M::Ld;
   using Stage = MarchStage<CT, VT, EX, EY, CLEAN>;
   const int nX = r.nX, nY = r.nY, sx = nX + 1;
   const int L = threadIdx.x & 63;
-  const int i = s + L;
+  const int P = W == 1 ? L : 60 * c + L;  // position in the union (and the ring row)
+  const int i = s + P;
+  // the lanes whose outputs this wave hands on (each ring position once)
+  const bool wl = W == 1 || ((c == 0 || L >= 2) && (c == W - 1 || L < 62));
   const int jbA = j0 - 2 * NS, tEnd = j1 + 2 * NS - 3;
   Stage st;
   st.setup(r, gt, i);
@@ -2087,17 +2100,19 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   using I3 = std::integral_constant<int, 3>;
   if (role == 0) {
     // stage 0: rows jbA..tEnd (steps 0 .. tEnd-jbA) from HBM into ring 0
-    RowRing<CT>& out = ring[0];
+    RowRing<CT, W>& out = ring[0];
     const StageInput<CT, VT, EX, EY, CLEAN> in(r, cur, vpar, i, tEnd);
     in.start(st, jbA);
     const int nAct = tEnd - jbA + 1;
     auto stepA = [&](int q, const Ld& ld, auto sl) {
       const auto o = st.step(jbA + q, ld.ux, ld.uy, ld.m, (CT)ld.vx, (CT)ld.vy);
-      out.ux[sl][L] = o.ux;
-      out.uy[sl][L] = o.uy;
-      out.vx[sl][L] = o.vx;
-      out.vy[sl][L] = o.vy;
-      if (!CLEAN) out.m[sl][L] = ld.m;
+      if (wl) {
+        out.ux[sl][P] = o.ux;
+        out.uy[sl][P] = o.uy;
+        out.vx[sl][P] = o.vx;
+        out.vy[sl][P] = o.vy;
+        if (!CLEAN) out.m[sl][P] = ld.m;
+      }
       lds_barrier();
     };
     Ld P = in.fetch(jbA), Q;
@@ -2133,13 +2148,13 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
     // first step (rows in from the ring) and last step of this stage
     constexpr int qStart = 5 * k - 1;
     const int qLast = tEnd + k - jbA;
-    RowRing<CT>& in = ring[k - 1];
-    RowRing<CT>& out = ring[last ? k - 1 : k];  // not written by the last stage
+    RowRing<CT, W>& in = ring[k - 1];
+    RowRing<CT, W>& out = ring[last ? k - 1 : k];  // not written by the last stage
     double* __restrict__ Uxo = r.Ux[cur ^ 1];
     double* __restrict__ Uyo = r.Uy[cur ^ 1];
     VT* Vxo = static_cast<VT*>(vpar ? r.Vx : r.Vx2);
     VT* Vyo = static_cast<VT*>(vpar ? r.Vy : r.Vy2);
-    const bool outLane = L >= 2 * NS && L < 64 - 2 * NS && (!EX || i < nX);
+    const bool outLane = wl && P >= 2 * NS && P < UW - 2 * NS && (!EX || i < nX);
     const unsigned o8 = (unsigned)i * 8u, oV = (unsigned)i * (unsigned)sizeof(VT);
     // edge segments bound the V resources to their fields: a lane whose V
     // ComputeDisp leaves alone stores at kOob, which the hardware drops
@@ -2153,17 +2168,19 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
     auto stepK = [&](int q, auto ph) {
       constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;  // q-1, q-2, q-3
       const int j = jbA + q - lag;
-      const double uxn = in.ux[s2][L], uynn = in.uy[s1][L];
-      const unsigned mn = CLEAN ? 0u : in.m[s3][L];
-      const CT vxi = in.vx[s3][L], vyi = in.vy[s3][L];
+      const double uxn = in.ux[s2][P], uynn = in.uy[s1][P];
+      const unsigned mn = CLEAN ? 0u : in.m[s3][P];
+      const CT vxi = in.vx[s3][P], vyi = in.vy[s3][P];
       const double uxsK = st.uxs_c;  // Ux(i+1, j) of this stage's input, before the step
       const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
       if constexpr (!last) {
-        out.ux[ph][L] = o.ux;
-        out.uy[ph][L] = o.uy;
-        out.vx[ph][L] = o.vx;
-        out.vy[ph][L] = o.vy;
-        if (!CLEAN) out.m[ph][L] = mn;
+        if (wl) {
+          out.ux[ph][P] = o.ux;
+          out.uy[ph][P] = o.uy;
+          out.vx[ph][P] = o.vx;
+          out.vy[ph][P] = o.vy;
+          if (!CLEAN) out.m[ph][P] = mn;
+        }
       } else if (j >= j0 && outLane) {
         const unsigned ovx = EDGE && !o.updX ? kOob : oV, ovy = EDGE && !o.updY ? kOob : oV;
         bst(bVx, ovx, (unsigned)j * (unsigned)sx * (unsigned)sizeof(VT), (VT)o.vx);
@@ -2182,7 +2199,7 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
       // qStart-1), mask row jb (slot qStart-3)
       constexpr int ph = qStart & 3;
       constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;
-      st.start(in.ux[s2][L], in.uy[s1][L], CLEAN ? 0u : in.m[s3][L]);
+      st.start(in.ux[s2][P], in.uy[s1][P], CLEAN ? 0u : in.m[s3][P]);
       lds_barrier();
     }
     constexpr int P0 = (qStart + 1) & 3;
@@ -2216,17 +2233,20 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
   }
 }
 
-// One segment per workgroup of NS waves (stage k on wave k).  The split
-// scheme's register budget holds seven waves per SIMD.
-template <typename CT, typename VT, int NS>
-__global__ __launch_bounds__(64 * NS)
+// One segment per workgroup of NS x W waves (stage k on waves kW .. kW+W-1,
+// wave c of a stage on columns s+60c ..).  The split scheme's register
+// budget holds seven waves per SIMD.
+template <typename CT, typename VT, int NS, int W>
+__global__ __launch_bounds__(64 * NS * W)
 __attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? 7 : 1))) void k_chain(const RveDev* __restrict__ rves,
                                                    const int4* __restrict__ segs, int nsegs,
                                                    int parity, int vpar) {
-  __shared__ CT gt[NS][16];
-  __shared__ RowRing<CT> ring[NS - 1];
+  constexpr int UW = 60 * W + 4;
+  __shared__ CT gt[NS * W][16];
+  __shared__ RowRing<CT, W> ring[NS - 1];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int role = wave / W, c = wave % W;
   const int k = tile_of_block(blockIdx.x, gridDim.x);
   if (k >= nsegs) return;  // the whole workgroup
   const int4 sg = segs[k];
@@ -2237,23 +2257,33 @@ __attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? 7 : 1))) void k_chain(const
   const int cur = parity ^ r.cur;
   const int s = sg.y, j0 = sg.z, j1 = sg.w;
   const int nSteps = j1 - j0 + 5 * NS - 3;
-  // column edge (strips touching the left or right boundary) and row edge
-  // (segments whose rows reach the top or bottom), and whether every mask
-  // byte the segment reads (rows j0-2NS .. j1+2NS-2, columns s .. s+63) is 0
-  const bool ex = !(s >= 1 && s + 63 <= r.nX - 2);
+  // column edge (this wave's 64 columns touch the left or right boundary;
+  // the other waves of the stage take the interior path, which computes the
+  // same values there), row edge (the segment's rows reach the top or
+  // bottom), and whether every mask byte the segment reads (rows j0-2NS ..
+  // j1+2NS-2, columns s .. s+UW-1) is 0, decided for the whole workgroup: a
+  // mask-free stage hands no mask rows on
+  const int sw = s + 60 * c;
+  const bool ex = !(sw >= 1 && sw + 63 <= r.nX - 2);
+  const bool exAny = !(s >= 1 && s + UW - 1 <= r.nX - 2);
   const bool ey = !(j0 >= 2 * NS && j1 <= r.nY - (2 * NS - 1));
-  const bool clean = !ex && mask_clear(r, s, s + 63, j0 - 2 * NS, j1 + 2 * NS - 2);
+  const bool clean = !exAny && mask_clear(r, s, s + UW - 1, j0 - 2 * NS, j1 + 2 * NS - 2);
   const CT* g = gt[wave];
   if (ex)
-    chain_run<CT, VT, true, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    chain_run<CT, VT, true, true, false, NS, W>(r, cur, vpar, s, j0, j1, g, role, c, ring,
+                                                nSteps);
   else if (ey && clean)
-    chain_run<CT, VT, false, true, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    chain_run<CT, VT, false, true, true, NS, W>(r, cur, vpar, s, j0, j1, g, role, c, ring,
+                                                nSteps);
   else if (ey)
-    chain_run<CT, VT, false, true, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    chain_run<CT, VT, false, true, false, NS, W>(r, cur, vpar, s, j0, j1, g, role, c, ring,
+                                                 nSteps);
   else if (clean)
-    chain_run<CT, VT, false, false, true, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    chain_run<CT, VT, false, false, true, NS, W>(r, cur, vpar, s, j0, j1, g, role, c, ring,
+                                                 nSteps);
   else
-    chain_run<CT, VT, false, false, false, NS>(r, cur, vpar, s, j0, j1, g, wave, ring, nSteps);
+    chain_run<CT, VT, false, false, false, NS, W>(r, cur, vpar, s, j0, j1, g, role, c, ring,
+                                                  nSteps);
 }
 
 // ---------------------------------------------------------------------------
@@ -3094,9 +3124,9 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
   return n;
 }
 
-int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out) {
+int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out, int w) {
   if (rows < 1) rows = 1;
-  const int cols = chain_cols(ns_);
+  const int cols = chain_cols(ns_, w);
   const int ns = (nX + cols - 1) / cols;
   int n = 0;
   for (int j0 = 0; j0 < nY; j0 += rows)
@@ -3108,23 +3138,31 @@ int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out) {
 }
 
 void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
-                  int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+                  int vpar, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int w) {
   if (nsegs <= 0) return;
-  const dim3 grid(nsegs), block(64 * ns);
+  const dim3 grid(nsegs), block(64 * ns * w);
   auto go = [&](auto kern) {
     if (e0)
       hipExtLaunchKernelGGL(kern, grid, block, 0, s, e0, e1, 0, rves, segs, nsegs, parity, vpar);
     else
       hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
   };
-  if (ns == 3) {
-    if (mode == 2) go(k_chain<float, float, 3>);  // split: FP32 stress and velocity
-    else go(k_chain<double, double, 3>);
-  } else if (ns == 4) {
-    if (mode == 2) go(k_chain<float, float, 4>);
-    else go(k_chain<double, double, 4>);
+  const bool f32 = mode == 2;  // split: FP32 stress and velocity
+  if (w == 1 && ns == 3) {
+    if (f32) go(k_chain<float, float, 3, 1>);
+    else go(k_chain<double, double, 3, 1>);
+  } else if (w == 1 && ns == 4) {
+    if (f32) go(k_chain<float, float, 4, 1>);
+    else go(k_chain<double, double, 4, 1>);
+  } else if (w == 2 && ns == 3) {
+    if (f32) go(k_chain<float, float, 3, 2>);
+    else go(k_chain<double, double, 3, 2>);
+  } else if (w == 2 && ns == 4) {
+    if (f32) go(k_chain<float, float, 4, 2>);
+    else go(k_chain<double, double, 4, 2>);
   } else {
-    throw std::invalid_argument("chain kernels run 3 or 4 iterations per launch");
+    throw std::invalid_argument("chain kernels run 3 or 4 iterations per launch on 1 or 2 "
+                                "waves per iteration");
   }
   EP2D_HIP(hipGetLastError());
 }

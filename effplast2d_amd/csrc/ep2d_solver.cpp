@@ -274,16 +274,18 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   buildGatherTable();
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
     // automatic shape for one RVE, from one-RVE measurements: split from
-    // 1024^2 on chains of four waves (four iterations per launch; 16-row
-    // segments at 1024^2, 32 beyond: 10.2, 23.4 and 76.3 us per iteration
-    // at 1024^2, 2048^2 and 4096^2, against 13.9, 37.3 and 177.7 for the
-    // tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the tile
-    // kernel below those sizes.  Segments stay short enough to give the GPU
-    // waves.
+    // 1024^2 on four-stage chains with two waves per stage (four iterations
+    // per launch; 16-row segments at 1024^2, 32 at 2048^2, 48 beyond: 9.9,
+    // 22.1 and 69.3 us per iteration at 1024^2, 2048^2 and 4096^2, against
+    // 10.3, 23.9 and 77.8 on one wave per stage and 12.1, 33.6 and 170.6
+    // for the tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the
+    // tile kernel below those sizes.  Segments stay short enough to give the
+    // GPU waves.
     const int n = std::max(nX, nY);
     if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 16 : 32;
+      opt_.marchRows = n <= 1024 ? 16 : (n <= 2048 ? 32 : 48);
       opt_.itersPerLaunch = 4;
+      if (opt_.laneColumns != 2 && opt_.stageWaves == 0) opt_.stageWaves = 2;
     } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {
       opt_.marchRows = 26;
       opt_.itersPerLaunch = 2;
@@ -308,6 +310,10 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     if (opt_.laneColumns == 2 && (mode_ != EP2D_MODE_SPLIT || ipl_ < 3))
       throw ArgError("laneColumns 2 (wide chains) needs the split scheme and itersPerLaunch 3 or 4");
     wide_ = opt_.laneColumns == 2;
+    if (opt_.stageWaves < 0 || opt_.stageWaves > 2) throw ArgError("stageWaves must be 0, 1 or 2");
+    if (opt_.stageWaves == 2 && (ipl_ < 3 || wide_))
+      throw ArgError("stageWaves 2 needs itersPerLaunch 3 or 4 with one column per lane");
+    stageWaves_ = opt_.stageWaves == 2 ? 2 : 1;
     if (opt_.marchRows > 0) {
       // row-marching kernel: the parameter table and the segment list live
       // on the device (the kernel reads the current buffer as parity ^ cur)
@@ -326,7 +332,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
         auto segsOf = [&](int4* out) {
           if (ipl_ == 2) return march2_segments(nX, nY, marchRows_, 0, out);
           if (wide_) return wide_segments(ipl_, nX, nY, marchRows_, 0, out);
-          return chain_segments(ipl_, nX, nY, marchRows_, 0, out);
+          return chain_segments(ipl_, nX, nY, marchRows_, 0, out, stageWaves_);
         };
         nSegs2_ = segsOf(nullptr);
         std::vector<int4> s2((size_t)nSegs2_);
@@ -562,7 +568,7 @@ void Solver::launchMulti(int q) {
                 timed ? ev.first : nullptr, timed ? ev.second : nullptr);
   else
     launch_chain(ipl_, dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
-                 timed ? ev.first : nullptr, timed ? ev.second : nullptr);
+                 timed ? ev.first : nullptr, timed ? ev.second : nullptr, stageWaves_);
   cur_ ^= 1;
   stressStale_ = true;
   if (timed) {
@@ -635,7 +641,7 @@ void Solver::getTiming(double* avgMs, long* samples, const char** name) {
             : wide_ ? "k_wide<" + std::to_string(ipl_) + ">"
                     : std::string(mode_ == EP2D_MODE_SPLIT ? "k_chain<float, float, "
                                                            : "k_chain<double, double, ") +
-                          std::to_string(ipl_) + ">") +
+                          std::to_string(ipl_) + ", " + std::to_string(stageWaves_) + ">") +
            " rows=" + std::to_string(marchRows_) + " segments=" + std::to_string(nSegs2_) +
            " (per iteration)";
     else if (marchRows_ > 0)

@@ -170,6 +170,7 @@ class Solver {
   bool multiIter_ = false;          // more than one iteration per launch
   int ipl_ = 1;                     // iterations per launch: 1, 2 (pairs), 3 or 4 (chains)
   bool wide_ = false;               // chains on two columns per lane (split scheme)
+  int stageWaves_ = 1;              // chains: waves side by side per stage
   int nSegs2_ = 0;
   int4* dSegs2_ = nullptr;
   int vIdx_ = 0;                    // velocity buffer holding the current V

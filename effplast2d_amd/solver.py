@@ -50,7 +50,8 @@ class EffPlast2D:
     def __init__(self, pa, nX, nTimeSteps=1, nTasks=3, nIter=100_000, eIter=1e-10,
                  checkInterval=10_000, mode="fp64", device=0, shearLoadType=None,
                  verbose=False, logPath=None, tileRows=0, marchRows=0, itersPerLaunch=0,
-                 adaptiveCheck=False, stream=None, laneColumns=0, keepFp32Stress=False):
+                 adaptiveCheck=False, stream=None, laneColumns=0, keepFp32Stress=False,
+                 stageWaves=0):
         self.pa = np.ascontiguousarray(pa, dtype=np.float64)
         if self.pa.shape != (L.NPARS,):
             raise ValueError("pa must hold 11 values")
@@ -65,7 +66,8 @@ class EffPlast2D:
                                      itersPerLaunch=itersPerLaunch,
                                      adaptiveCheck=1 if adaptiveCheck else 0,
                                      laneColumns=laneColumns,
-                                     keepFp32Stress=1 if keepFp32Stress else 0)
+                                     keepFp32Stress=1 if keepFp32Stress else 0,
+                                     stageWaves=int(stageWaves))
         h = C.c_void_p()
         # stream: a hipStream_t (as an integer) the handle orders its work on
         check(lib().ep2d_create_on_stream(_dptr(self.pa), self.nX, self.nY, C.byref(self._opts),

@@ -122,6 +122,11 @@ typedef struct ep2d_options {
                            over eight checks (the FP32 stress no longer
                            resolves it) finishes with the FP64 arithmetic on
                            the same FP64 U and FP32 V; 1: never           */
+  int stageWaves;       /* with itersPerLaunch 3 or 4 and one column per lane:
+                           2 runs each iteration of a chain on two waves side
+                           by side (124-column strips, each stage's rows
+                           handed on across both), 1 on one wave; 0: chosen
+                           with the shape (same results)                  */
 } ep2d_options;
 
 typedef struct ep2d_moduli { double Kphi, Kd, G; } ep2d_moduli;  /* EffPlast2D.h:120-134 */
@@ -336,6 +341,10 @@ typedef struct ep2d_batch_options {
                             RVE that finishes a time step in the FP64
                             arithmetic leaves the batched launches for the
                             rest of that step                                */
+  int triple2Rows;       /* fixed three-iteration chains with two waves per
+                            iteration (ep2d_options.stageWaves), segment rows */
+  int quad2Rows;         /* fixed four-iteration chains with two waves per
+                            iteration, segment rows                          */
 } ep2d_batch_options;
 
 typedef struct ep2d_request {

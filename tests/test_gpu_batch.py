@@ -126,7 +126,7 @@ def test_tuned_kernel_equals_fixed():
     results."""
     _note("tuned vs fixed kernel shapes")
     tuned, st = _batch(REQS, "split")
-    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4, 5, 6, 7, 8, 9)
+    assert st["tunings"] >= 1 and st["kernel"] in (1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12)
     assert all(u > 0 for u in st["candUs"])
     for kw in ({"tileRows": 6}, {"tileRows": 30}, {"marchRows": 16}, {"marchRows": 128},
                {"persistRows": 6}, {"persistRows": 14}, {"march2Rows": 8}, {"march2Rows": 48},
@@ -189,19 +189,20 @@ def test_persistent_kernel_tiny_batch():
 @pytest.mark.parametrize("mode", ["fp64", "split"])
 @pytest.mark.parametrize("graphs", [True, False])
 @pytest.mark.parametrize("shape", ["march2Rows", "pairRows", "tripleRows", "quadRows",
-                                   "wide3Rows", "wide4Rows"])
+                                   "wide3Rows", "wide4Rows", "triple2Rows", "quad2Rows"])
 def test_march2_batch_equals_single(mode, graphs, shape):
     """Multi-iteration launches in the engine (two iterations per launch on one
-    wave or on a pair of waves per segment, three or four on a chain of waves
-    with one or (split scheme) two columns per lane; then one to four single
-    iterations per chunk; odd check intervals included) give the single-RVE
-    results."""
+    wave or on a pair of waves per segment, three or four on a chain of
+    stages with one or two waves per stage, or (split scheme) one wave with
+    two columns per lane; then one to four single iterations per chunk; odd
+    check intervals included) give the single-RVE results."""
     if shape.startswith("wide") and mode != "split":
         pytest.skip("the wide chains run the split scheme only")
     _note(f"{shape} batch {mode} graphs={graphs}")
     got, st = _batch(REQS, mode, useGraphs=graphs, timingEvery=1, **{shape: 32})
     assert st["kernel"] == {"march2Rows": 4, "pairRows": 5, "tripleRows": 6, "quadRows": 7,
-                            "wide3Rows": 8, "wide4Rows": 9}[shape]
+                            "wide3Rows": 8, "wide4Rows": 9, "triple2Rows": 11,
+                            "quad2Rows": 12}[shape]
     for t, rq in enumerate(REQS):
         _same(got[t]["result"], _single(rq, mode))
     nX, nP, phi, Y, nl, load = REQS[0]
@@ -245,8 +246,8 @@ def test_adaptive_check_schedule(mode):
 @pytest.mark.parametrize("mode", ["fp64", "split"])
 def test_chains_full_size_equal_tiles(mode):
     """1024^2 (interior and edge segments, the headline size): 3 x 300 plastic
-    iterations through the three- and four-stage chains give the tile
-    kernel's records and moduli bit for bit."""
+    iterations through the three- and four-stage chains, one or two waves per
+    stage, give the tile kernel's records and moduli bit for bit."""
     _note(f"chains 1024 {mode}")
     nX = 1024
     pa = make_pa(nX, nPores=1, porosity=0.01, Y=2.5e-3)
@@ -259,9 +260,11 @@ def test_chains_full_size_equal_tiles(mode):
             return b.poll()[0], b.stats()
 
     ref, _ = run(tileRows=14)
-    for shape in ({"tripleRows": 86}, {"tripleRows": 42}, {"quadRows": 64}, {"quadRows": 256}):
+    kinds = {"tripleRows": 6, "quadRows": 7, "triple2Rows": 11, "quad2Rows": 12}
+    for shape in ({"tripleRows": 86}, {"tripleRows": 42}, {"quadRows": 64}, {"quadRows": 256},
+                  {"triple2Rows": 64}, {"quad2Rows": 64}, {"quad2Rows": 200}):
         got, st = run(**shape)
-        assert st["kernel"] == (6 if "tripleRows" in shape else 7)
+        assert st["kernel"] == kinds[next(iter(shape))]
         _same(got["result"], ref["result"])
         assert got["result"]["totalIters"] == 900
 

@@ -76,7 +76,8 @@ def test_rve4096_fp64_matches_oracle(rve4096_oracle, kw):
 
 
 def test_rve4096_split_fields_close_to_oracle():
-    """Split at 4096^2 (automatic shape: four-wave chains, 32-row segments): after 30 plastic
+    """Split at 4096^2 (automatic shape: four-stage chains on two waves per stage,
+    48-row segments): after 30 plastic
     iterations from the same start (V rounded to FP32 for both), U within
     1e-6 relative and V and the stresses within 1e-4 relative L2 of FP64."""
     pa, init = _rve4096_start()

@@ -144,21 +144,24 @@ def test_march2_matches_oracle(rows, iters, nX, nPores):
             assert e <= 1e-12, f"{k}: rel err {e:.3e} ({np.count_nonzero(got != s[k])} differ)"
 
 
+@pytest.mark.parametrize("waves", [1, 2])
 @pytest.mark.parametrize("ipl", [3, 4])
 @pytest.mark.parametrize("rows,iters", [(1, 7), (5, 13), (16, 3), (64, 10)])
-@pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (8, 1)])
-def test_chain_matches_oracle(ipl, rows, iters, nX, nPores):
-    """Three or four iterations per launch on a chain of waves, FP64: every
-    remainder of the iteration count (the chain launches, then one to four
-    single iterations), short and long segments, bit for bit against the
-    oracle."""
+@pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (8, 1), (300, 3)])
+def test_chain_matches_oracle(ipl, rows, iters, nX, nPores, waves):
+    """Three or four iterations per launch on a chain of stages, one or two
+    waves per stage, FP64: every remainder of the iteration count (the chain
+    launches, then one to four single iterations), short and long segments,
+    strips that all touch a boundary (8^2, 61^2, 200^2) and interior ones
+    (300^2), bit for bit against the oracle."""
     nl = 3
-    _note(f"chain{ipl} fp64 rows={rows} iters={iters} nX={nX}")
+    _note(f"chain{ipl}x{waves} fp64 rows={rows} iters={iters} nX={nX}")
     pa = make_pa(nX, nPores=nPores, porosity=0.03)
     init = random_state(nX, 900 + rows + nX + ipl, pa=pa, boundary_v=True)
     s = oracle_state(nX, pa, nl, init)
     s.iterate(iters)
-    with EffPlast2D(pa, nX, nTasks=nl, mode="fp64", marchRows=rows, itersPerLaunch=ipl) as g:
+    with EffPlast2D(pa, nX, nTasks=nl, mode="fp64", marchRows=rows, itersPerLaunch=ipl,
+                    stageWaves=waves) as g:
         for k in STATE_FIELDS:
             g.set(k, init[k])
         g.iterate(iters)

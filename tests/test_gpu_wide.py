@@ -1,5 +1,6 @@
-"""GPU: the wide chains (two columns per lane, split scheme) against the
-chains of one column per lane and the tile kernel.
+"""GPU: the wide chains (two columns per lane, split scheme) and the chains
+with two waves per stage against the chains of one wave per stage and the
+tile kernel.
 
 Each wide stage computes every element with the split scheme's functions in
 the same order as the one-column chains, so the fields must agree bit for
@@ -69,15 +70,18 @@ def test_wide_equals_tiles(nl):
     _same_fields(wide, tiles)
 
 
-def test_wide_4096_sixteen_pores_equals_chain():
+@pytest.mark.parametrize("other", [{"laneColumns": 2}, {"stageWaves": 2}],
+                         ids=["wide", "two_waves"])
+def test_wide_4096_sixteen_pores_equals_chain(other):
     """4096^2 with 4 x 4 pores, porosity 0.05: interior segments with and
-    without pore cells, 30 plastic iterations."""
+    without pore cells, 30 plastic iterations; the wide chains and the chains
+    with two waves per stage against the chains of one wave per stage."""
     nX, iters = 4096, 30
-    _note("wide 4096 16 pores")
+    _note(f"4096 16 pores {other}")
     pa = make_pa(nX, nPores=4, porosity=0.05, Y=2.5e-3)
     init = random_state(nX, 4096, pa=pa, boundary_v=True)
     one = _fields(pa, nX, 3, init, iters, marchRows=56, itersPerLaunch=3, laneColumns=1)
-    two = _fields(pa, nX, 3, init, iters, marchRows=56, itersPerLaunch=3, laneColumns=2)
+    two = _fields(pa, nX, 3, init, iters, marchRows=56, itersPerLaunch=3, **other)
     _same_fields(two, one)
 
 
