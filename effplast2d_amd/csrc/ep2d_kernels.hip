@@ -1503,8 +1503,12 @@ struct MarchStage {
 
   // Ux row jb, Uy row jb+1, mask row jb (jb = first step)
   __device__ __forceinline__ void start(double ux, double uy1, unsigned m) {
+    startS(ux, lane_next(ux), uy1, m);
+  }
+  // the same with Ux(i+1, jb) given (read from a ring row instead of shifted)
+  __device__ __forceinline__ void startS(double ux, double uxs, double uy1, unsigned m) {
     ux_c = ux;
-    uxs_c = lane_next(ux_c);
+    uxs_c = uxs;
     uy_c = 0.0;
     uy_n = uy1;
     m_c = m;
@@ -1516,8 +1520,16 @@ struct MarchStage {
 
   __device__ __forceinline__ Out step(const int j, const double ux_n, const double uy_nn,
                                       const unsigned m_n, const CT vx_in, const CT vy_in) {
+    return stepS(j, ux_n, lane_next(ux_n), lane_next(uy_n), uy_nn, m_n, vx_in, vy_in);
+  }
+  // the step with the right neighbours uxs_n = Ux(i+1, j+1) and uys_n =
+  // Uy(i+1, j+1) given: a stage reading its rows from an LDS ring reads them
+  // at the next position instead of shifting lanes (the same values in every
+  // lane but the last, whose outputs no stage uses)
+  __device__ __forceinline__ Out stepS(const int j, const double ux_n, const double uxs_n,
+                                       const double uys_n, const double uy_nn,
+                                       const unsigned m_n, const CT vx_in, const CT vy_in) {
     // trial cell row j+1
-    const double uxs_n = lane_next(ux_n);
     CT tP_n, tXX_n, tYY_n;
     {
       const CT K = !CLEAN && (m_n & M_CELL_MAT) ? Kp : K0;
@@ -1544,7 +1556,6 @@ struct MarchStage {
     }
     const CT sXX_n = lane_next(tXX_n), sYY_n = lane_next(tYY_n);
     // trial vertex row j
-    const double uys_n = lane_next(uy_n);
     CT tXY_c;
     {
       const CT g = CLEAN ? g0 : gt[(m_c >> M_VCLASS_SHIFT) & 15];
@@ -2172,7 +2183,12 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
       const unsigned mn = CLEAN ? 0u : in.m[s3][P];
       const CT vxi = in.vx[s3][P], vyi = in.vy[s3][P];
       const double uxsK = st.uxs_c;  // Ux(i+1, j) of this stage's input, before the step
-      const auto o = st.step(j, uxn, uynn, mn, vxi, vyi);
+      // with two waves per stage the ring rows are 128 entries wide and the
+      // right neighbours are read at P+1 (slot q-2 holds Ux' and Uy' of row
+      // j+1); one wave per stage shifts lanes
+      const auto o = W > 1 ? st.stepS(j, uxn, in.ux[s2][P + 1], in.uy[s2][P + 1], uynn, mn,
+                                      vxi, vyi)
+                           : st.step(j, uxn, uynn, mn, vxi, vyi);
       if constexpr (!last) {
         if (wl) {
           out.ux[ph][P] = o.ux;
@@ -2199,7 +2215,10 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
       // qStart-1), mask row jb (slot qStart-3)
       constexpr int ph = qStart & 3;
       constexpr int s1 = (ph + 3) & 3, s2 = (ph + 2) & 3, s3 = (ph + 1) & 3;
-      st.start(in.ux[s2][P], in.uy[s1][P], CLEAN ? 0u : in.m[s3][P]);
+      if constexpr (W > 1)
+        st.startS(in.ux[s2][P], in.ux[s2][P + 1], in.uy[s1][P], CLEAN ? 0u : in.m[s3][P]);
+      else
+        st.start(in.ux[s2][P], in.uy[s1][P], CLEAN ? 0u : in.m[s3][P]);
       lds_barrier();
     }
     constexpr int P0 = (qStart + 1) & 3;
@@ -2235,10 +2254,11 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
 
 // One segment per workgroup of NS x W waves (stage k on waves kW .. kW+W-1,
 // wave c of a stage on columns s+60c ..).  The split scheme's register
-// budget holds seven waves per SIMD.
+// budget holds seven waves per SIMD (six with two waves per stage, where the
+// LDS of three workgroups of eight waves fills a CU).
 template <typename CT, typename VT, int NS, int W>
 __global__ __launch_bounds__(64 * NS * W)
-__attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? 7 : 1))) void k_chain(const RveDev* __restrict__ rves,
+__attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? (W == 1 ? 7 : 6) : 1))) void k_chain(const RveDev* __restrict__ rves,
                                                    const int4* __restrict__ segs, int nsegs,
                                                    int parity, int vpar) {
   constexpr int UW = 60 * W + 4;
