@@ -356,9 +356,10 @@ __device__ __forceinline__ float split_velocity(float v, float damp, float dTrho
                                                float ra, float b, float rb) {
   return v * damp + dTrho * (a * ra + b * rb);
 }
-// displacement, FP64: u + v dT
+// displacement, FP64: u + v dT with one rounding (a fused multiply-add; the
+// FP64 scheme keeps the reference's product and sum)
 __device__ __forceinline__ double split_disp(double u, float v, double dT) {
-  return u + (double)v * dT;
+  return __builtin_fma((double)v, dT, u);
 }
 
 // Workgroup barrier that waits for LDS traffic only: global loads issued
