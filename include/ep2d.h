@@ -388,7 +388,10 @@ typedef struct ep2d_batch_stats {
                                      two iterations per launch, 5 the same on
                                      pairs of waves (one iteration each), 6
                                      three iterations per launch on chains of
-                                     three waves, 7 four on chains of four     */
+                                     three waves, 7 four on chains of four, 8
+                                     and 9 the wide chains of three and four,
+                                     11 and 12 the chains of three and four
+                                     stages on two waves per stage            */
   int kernelRows;                 /* its tile height or segment rows          */
   int tunings;                    /* kernel tunings run                       */
   int nCand;                      /* kernel shapes the tuner compares        */
