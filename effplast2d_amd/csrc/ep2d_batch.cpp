@@ -55,6 +55,8 @@ Batch::Batch(const ep2d_batch_options& opt) : opt_(opt) {
   if (opt_.quadRows < 0) throw ArgError("quadRows must be >= 0");
   if (opt_.wide3Rows < 0) throw ArgError("wide3Rows must be >= 0");
   if (opt_.wide4Rows < 0) throw ArgError("wide4Rows must be >= 0");
+  if (opt_.triple2Rows < 0) throw ArgError("triple2Rows must be >= 0");
+  if (opt_.quad2Rows < 0) throw ArgError("quad2Rows must be >= 0");
   if ((opt_.wide3Rows != 0 || opt_.wide4Rows != 0) && opt_.mode != EP2D_MODE_SPLIT)
     throw ArgError("wide3Rows and wide4Rows (wide chains) need the split scheme");
   const int nfixed = (opt_.tileRows != 0) + (opt_.marchRows != 0) + (opt_.persistRows != 0) +

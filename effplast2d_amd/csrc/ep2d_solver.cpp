@@ -152,6 +152,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
   if (opt_.itersPerLaunch >= 2 && !multi_iter_fits(nX, nY))
     throw ArgError("itersPerLaunch 2 to 4 needs every field under 2 GiB (nX, nY <= 16382)");
+  if (opt_.laneColumns < 0 || opt_.laneColumns > 2) throw ArgError("laneColumns must be 0, 1 or 2");
+  if (opt_.stageWaves < 0 || opt_.stageWaves > 2) throw ArgError("stageWaves must be 0, 1 or 2");
   if (mode_ < EP2D_MODE_PLAIN || mode_ > EP2D_MODE_SPLIT) throw ArgError("unknown mode");
   if (opt_.nTasks < 1) throw ArgError("nTasks must be >= 1");
   if (opt_.nIter < 1) throw ArgError("nIter must be >= 1");
@@ -305,12 +307,9 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
       throw ArgError("itersPerLaunch 2 to 4 needs the row-marching kernel (marchRows > 0)");
     ipl_ = opt_.itersPerLaunch >= 2 ? opt_.itersPerLaunch : 1;
     multiIter_ = ipl_ > 1;
-    if (opt_.laneColumns < 0 || opt_.laneColumns > 2)
-      throw ArgError("laneColumns must be 0, 1 or 2");
     if (opt_.laneColumns == 2 && (mode_ != EP2D_MODE_SPLIT || ipl_ < 3))
       throw ArgError("laneColumns 2 (wide chains) needs the split scheme and itersPerLaunch 3 or 4");
     wide_ = opt_.laneColumns == 2;
-    if (opt_.stageWaves < 0 || opt_.stageWaves > 2) throw ArgError("stageWaves must be 0, 1 or 2");
     if (opt_.stageWaves == 2 && (ipl_ < 3 || wide_))
       throw ArgError("stageWaves 2 needs itersPerLaunch 3 or 4 with one column per lane");
     stageWaves_ = opt_.stageWaves == 2 ? 2 : 1;

@@ -153,6 +153,14 @@ def test_batch_options_and_errors():
     L.ep2d_batch_default_options(C.byref(o))
     o.tileRows = 7
     assert L.ep2d_batch_create(C.byref(o), C.byref(h)) == -1
+    L.ep2d_batch_default_options(C.byref(o))
+    o.quad2Rows = -1
+    assert L.ep2d_batch_create(C.byref(o), C.byref(h)) == -1
+    assert b"quad2Rows" in L.ep2d_last_error()
+    L.ep2d_batch_default_options(C.byref(o))
+    o.quadRows, o.triple2Rows = 64, 64
+    assert L.ep2d_batch_create(C.byref(o), C.byref(h)) == -1
+    assert b"at most one" in L.ep2d_last_error()
     if ep.device_count() == 0:
         L.ep2d_batch_default_options(C.byref(o))
         assert L.ep2d_batch_create(C.byref(o), C.byref(h)) in (-1, -2) and not h.value
@@ -215,6 +223,8 @@ def test_struct_layouts_match_the_header():
     (16383, {"marchRows": 16, "itersPerLaunch": 3}, "under 2 GiB"),
     (64, {"marchRows": 16, "itersPerLaunch": 5}, "itersPerLaunch must be"),
     (64, {"itersPerLaunch": 3}, "marchRows > 0"),
+    (64, {"marchRows": 16, "itersPerLaunch": 4, "stageWaves": 3}, "stageWaves must be"),
+    (64, {"marchRows": 16, "itersPerLaunch": 4, "laneColumns": 3}, "laneColumns must be"),
 ])
 def test_shape_and_size_limits_fail_before_the_device(nX, kw, msg):
     """Shape options and grid-size limits are checked before any device call:
