@@ -344,6 +344,29 @@ void Batch::enqueue(long n, bool dbl, long& l, long& d) {
   }
 }
 
+// The shapes of the closing launches (rem3Off_, rem2Off_).
+const Batch::Shape Batch::kClose3{Batch::kTriple, 42};
+const Batch::Shape Batch::kClose2{Batch::kPair, 26};
+
+// The 1..n single iterations that close a chunk of an n-iteration shape:
+// the last one is a single-iteration launch (it reads U^{n-1}); four of them
+// run as a three-iteration chain launch and the single, three as a wave-pair
+// launch and the single (the same results, in a third of the time).
+void Batch::closing(long singles, long& l, long& d) {
+  if (singles == 4 && rem3Cnt_ > 0) {
+    launchOne(kClose3, dTiles_ + rem3Off_, rem3Cnt_, true, l, d);
+    l++;
+    d++;
+    singles = 1;
+  } else if (singles == 3 && rem2Cnt_ > 0) {
+    launchOne(kClose2, dTiles_ + rem2Off_, rem2Cnt_, true, l, d);
+    l++;
+    d++;
+    singles = 1;
+  }
+  enqueue(singles, false, l, d);
+}
+
 // Graph of `len` launches of the current shape (multi-iteration launches for
 // a multi-iteration shape) from launch and double index 0 over the current
 // tables, captured once per length while the tables stay the same.
@@ -526,7 +549,10 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
     const Shape aux{kMarch, shape_.rows};
     const Shape esc{kEsc, 8};
     size_t need = (size_t)buildWork(shape_, nullptr) + (size_t)buildWork(esc, nullptr);
+    const bool close3 = ipl(shape_.kind) == 4, close2 = ipl(shape_.kind) >= 3;
     if (multiIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
+    if (close3) need += (size_t)buildWork(kClose3, nullptr);
+    if (close2) need += (size_t)buildWork(kClose2, nullptr);
     if (tuning)
       for (int k = 0; k < kNCand; k++) need += (size_t)buildWork(kCand[k], nullptr);
     ensureCapacity(active_.size(), need);
@@ -538,6 +564,12 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
       auxCnt_ = buildWork(aux, hTiles_ + off);
       off += auxCnt_;
     }
+    rem3Off_ = off;
+    rem3Cnt_ = close3 ? buildWork(kClose3, hTiles_ + off) : 0;
+    off += rem3Cnt_;
+    rem2Off_ = off;
+    rem2Cnt_ = close2 ? buildWork(kClose2, hTiles_ + off) : 0;
+    off += rem2Cnt_;
     escOff_ = off;
     escCnt_ = buildWork(esc, hTiles_ + off);
     off += escCnt_;
@@ -601,7 +633,7 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
       l++;
       if (two) d++;
     }
-    if (two) enqueue(singles, false, l, d);
+    if (two) closing(singles, l, d);
   } else {
     auto one = [&](bool dbl) {
       const bool t = timed && (st_.launches + l) % opt_.timingEvery == 0;
@@ -617,8 +649,7 @@ Batch::Seq Batch::runRest(long c, long it, long l, long d, double bytes, double 
       if (t) evPending_.push_back(ev);
     };
     for (long k = 0; k < main; k++) one(two);
-    if (two)
-      for (long k = 0; k < singles; k++) one(false);
+    if (two) closing(singles, l, d);   // not timed: only the main kernel is
   }
   st_.launches += l;
   st_.chunks++;

@@ -79,6 +79,7 @@ class Batch {
   // best, kRefRounds rounds of kRefLaunches back-to-back launches each
   static constexpr int kRefCands = 3, kRefRounds = 4, kRefLaunches = 4;
   static const Shape kCand[kNCand];
+  static const Shape kClose3, kClose2;  // launches that close a chunk (closing())
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
     long len = 0;
@@ -104,6 +105,7 @@ class Batch {
   void launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   void enqueue(long n, bool dbl, long& l, long& d);
+  void closing(long singles, long& l, long& d);
   void ensureCapacity(size_t nRves, size_t nTiles);
   void dropGraphs();
   hipGraphExec_t graphFor(long len);
@@ -136,6 +138,9 @@ class Batch {
   std::vector<int> curs_;         // current buffers in the uploaded RVE table
   int candOff_[kNCand] = {}, candCnt_[kNCand] = {};
   int auxOff_ = 0, auxCnt_ = 0;   // single-iteration segments of a two-iteration shape
+  // segments of the three- and two-iteration launches that close a chunk of
+  // a four- or three-iteration shape (closing())
+  int rem3Off_ = 0, rem3Cnt_ = 0, rem2Off_ = 0, rem2Cnt_ = 0;
   int escOff_ = 0, escCnt_ = 0;   // tiles of the RVEs in the FP64 arithmetic
 
   // kernel-shape tuning on real iterations

@@ -341,6 +341,24 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
                                 hipMemcpyHostToDevice, stream_));
         EP2D_HIP(hipStreamSynchronize(stream_));
       }
+      if (ipl_ >= 3) {
+        std::vector<int4> c3, c2((size_t)march2_segments(nX, nY, 26, 0, nullptr));
+        march2_segments(nX, nY, 26, 0, c2.data());
+        if (ipl_ == 4) {
+          c3.resize((size_t)chain_segments(3, nX, nY, 42, 0, nullptr, 1));
+          chain_segments(3, nX, nY, 42, 0, c3.data(), 1);
+        }
+        nSegsC3_ = (int)c3.size();
+        nSegsC2_ = (int)c2.size();
+        if (nSegsC3_ > 0) {
+          dSegsC3_ = (int4*)alloc(c3.size() * sizeof(int4));
+          EP2D_HIP(hipMemcpyAsync(dSegsC3_, c3.data(), c3.size() * sizeof(int4),
+                                  hipMemcpyHostToDevice, stream_));
+        }
+        dSegsC2_ = (int4*)alloc(c2.size() * sizeof(int4));
+        EP2D_HIP(hipMemcpyAsync(dSegsC2_, c2.data(), c2.size() * sizeof(int4),
+                                hipMemcpyHostToDevice, stream_));
+      }
       EP2D_HIP(hipStreamSynchronize(stream_));
     }
   }
@@ -594,11 +612,36 @@ void Solver::iterate(long n) {
   // place), so the last launch reads U^{n-1}, the state the stress fields are
   // recomputed from
   if (n <= 0) return;
-  const long singles = (n - 1) % ipl_ + 1;
-  const long d = (n - singles) / ipl_;
+  long singles = (n - 1) % ipl_ + 1;
+  long d = (n - singles) / ipl_;
   for (long q = 0; q < d; q++) launchMulti((int)(q & 1));
+  // four closing iterations run as a three-iteration launch and the single,
+  // three as a two-iteration launch and the single (the same results)
+  if (singles == 4 && nSegsC3_ > 0) {
+    launchClose(3, (int)(d & 1));
+    d++;
+    singles = 1;
+  } else if (singles == 3 && nSegsC2_ > 0) {
+    launchClose(2, (int)(d & 1));
+    d++;
+    singles = 1;
+  }
   if (d & 1) flipV();
   for (long q = 0; q < singles; q++) launchIteration();
+}
+
+// A three-iteration chain (three waves per segment) or a wave-pair launch
+// closing a run of multi-iteration launches; V is read from buffer
+// vIdx_ ^ q and written to the other, as in launchMulti.
+void Solver::launchClose(int ns, int q) {
+  if (ns == 3)
+    launch_chain(3, dRve_, dSegsC3_, nSegsC3_, mode_, cur_, vIdx_ ^ q, stream_, nullptr,
+                 nullptr, 1);
+  else
+    launch_march2(dRve_, dSegsC2_, nSegsC2_, mode_, cur_, vIdx_ ^ q, stream_, nullptr, nullptr,
+                  true);
+  cur_ ^= 1;
+  stressStale_ = true;
 }
 
 void Solver::resolveTiming(bool all) {

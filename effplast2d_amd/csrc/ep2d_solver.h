@@ -173,10 +173,17 @@ class Solver {
   int stageWaves_ = 1;              // chains: waves side by side per stage
   int nSegs2_ = 0;
   int4* dSegs2_ = nullptr;
+  // segments of the three- and two-iteration launches that close a run of a
+  // four- or three-iteration shape (chains of three waves, 42 rows; wave
+  // pairs, 26 rows)
+  int nSegsC3_ = 0, nSegsC2_ = 0;
+  int4* dSegsC3_ = nullptr;
+  int4* dSegsC2_ = nullptr;
   int vIdx_ = 0;                    // velocity buffer holding the current V
   void flipV();                     // the other velocity buffer becomes current
   std::pair<hipEvent_t, hipEvent_t> takeTimingEvents();
   void launchMulti(int q);         // ipl_ iterations, q-th such launch of a run
+  void launchClose(int ns, int q);  // ns = 3 or 2 iterations closing a run
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;

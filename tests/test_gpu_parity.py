@@ -146,12 +146,14 @@ def test_march2_matches_oracle(rows, iters, nX, nPores):
 
 @pytest.mark.parametrize("waves", [1, 2])
 @pytest.mark.parametrize("ipl", [3, 4])
-@pytest.mark.parametrize("rows,iters", [(1, 7), (5, 13), (16, 3), (64, 10)])
+@pytest.mark.parametrize("rows,iters", [(1, 7), (5, 13), (16, 3), (64, 10), (24, 8)])
 @pytest.mark.parametrize("nX,nPores", [(200, 2), (61, 1), (8, 1), (300, 3)])
 def test_chain_matches_oracle(ipl, rows, iters, nX, nPores, waves):
     """Three or four iterations per launch on a chain of stages, one or two
     waves per stage, FP64: every remainder of the iteration count (the chain
-    launches, then one to four single iterations), short and long segments,
+    launches, then one to four closing iterations, three of which run as a
+    wave-pair launch and four as a three-stage launch before the single
+    one), short and long segments,
     strips that all touch a boundary (8^2, 61^2, 200^2) and interior ones
     (300^2), bit for bit against the oracle."""
     nl = 3
