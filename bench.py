@@ -31,8 +31,10 @@ kernel time is the live HIP-event timing of the batched kernel's graph
 replays on the engine's stream (the sampled single-dispatch figure beside it).
 
 Other entry points:
-  --probe      a fixed 200-iteration run of one scheme plus a copy kernel of
-               known size (for rocprofv3 --pmc counter runs)
+  --probe      a fixed run of one scheme plus a copy kernel of known size
+               (for rocprofv3 --pmc counter runs): the batched kernel of
+               --probe-shape over --probe-batch RVEs, or one RVE with that
+               shape ("auto": the handle's own choice)
   --probe-shapes  the batched probe once per listed kernel shape (for a
                rocprofv3 kernel trace by grid size)
   --sweep      fused-kernel duration per grid size, scheme and tile height
@@ -472,14 +474,16 @@ def probe(mode, nx, iters, copy_mib, batch=0, shape="tiles8"):
     from effplast2d_amd._lib import check, lib
     pa, _ = make_pa(nx)
     kind = shape.rstrip("0123456789")
-    rows = int(shape[len(kind):])
+    rows = int(shape[len(kind):]) if shape != "auto" else 0
     if batch > 0:
         reqs = [dict(headline_request(pa, nx, k), nIter=iters, eIter=1e-300)
                 for k in range(batch)]
         run_batch(reqs, mode, 10 ** 9, timing_every=0, useGraphs=False,
                   **{SHAPE_OPTION[kind]: rows})
     else:
-        kw = {"tiles": {"tileRows": rows}, "march": {"marchRows": rows}}[kind]
+        # one RVE: a shape as single_shape names it, or "auto" (the handle's
+        # automatic choice)
+        kw = {} if shape == "auto" else single_shape(shape)
         with EffPlast2D(pa, nx, nTasks=3, mode=mode, **kw) as g:
             g.apply_affine(-3e-3, -3e-3, 0.0, 0.0)
             g.iterate(iters)
