@@ -104,7 +104,8 @@ typedef struct ep2d_options {
   int tileRows;         /* fused kernels: tile height 6, 8, 14 or 30 (0: tuned default) */
   int marchRows;        /* > 0: row-marching kernel, segments of this many rows  */
   int itersPerLaunch;   /* with marchRows: 2 iterations per launch on wave pairs,
-                           3 or 4 on chains of 3 or 4 waves (nX, nY <= 16382:
+                           3 or 4 on chains of 3 or 4 stages, one wave per
+                           stage unless stageWaves = 2 (nX, nY <= 16382:
                            fields under 2 GiB); 1: one; 0 (default) with
                            tileRows = marchRows = 0: the shape is chosen by grid
                            size and scheme (same results) */
@@ -324,7 +325,8 @@ typedef struct ep2d_batch_options {
                             waves, segment rows                                */
   int quadRows;          /* fixed four-iteration marching on chains of four
                             waves, segment rows                                */
-                         /* (the last four: requests up to nX = 16382)          */
+                         /* (the multi-iteration shapes, here and below:
+                            requests up to nX = 16382)                         */
                          /* all 0: kernel and shape tuned on the running batch  */
                          /* (multi-iteration shapes while every RVE has nX <= 16382) */
   int device;
