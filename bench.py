@@ -61,6 +61,7 @@ from effplast2d_amd.workloads import ITER_CAP_PER_N, mixed_requests, serving_str
 
 METRIC = ("converged RVE solves/s + cell-iter/s (% HBM peak), FP32-split vs FP64; p50/p99")
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12   # lane instructions per second, 10^12
 # algorithmic bytes per point per iteration: U and V read once and written once
 BYTES_PER_POINT = {"fp64": 64, "plain": 64, "split": 48}
 
@@ -864,6 +865,15 @@ def main():
     sq = profile_path("sq", head["kernel_shape"], a.mode, nx, a.steps)
     traffic = read_profile_value(pmc, "traffic_bytes_per_launch")
     valu_busy = read_profile_value(sq, "valu_busy_frac")
+    # VALU issue: lane instructions per point-iteration (the counter profile)
+    # times the launch's point-iterations over its duration, against the
+    # MI355X peak of one lane instruction per lane per cycle (256 CUs x 4
+    # SIMDs x 16 lanes x 2.4 GHz = 39.3 T/s; a fused multiply-add counts one)
+    valu_per_pt = read_profile_value(sq, "valu_lane_insts_per_point_iteration")
+    pt_per_launch = head["bytes_per_launch"] / BYTES_PER_POINT[a.mode] * \
+        head["iterations_per_launch"]
+    valu_t = (valu_per_pt * pt_per_launch / (head["kernel_avg_us"] * 1e-6) / 1e12
+              if valu_per_pt else None)
     out = {
         "metric": METRIC,
         "value": head["solves_per_s"],
@@ -909,7 +919,11 @@ def main():
                      "traffic_profile": os.path.relpath(pmc, ROOT) if traffic else None,
                      # fraction of SIMD cycles issuing VALU instructions
                      "valu_busy_frac": valu_busy,
-                     "valu_profile": os.path.relpath(sq, ROOT) if valu_busy else None},
+                     "valu_profile": os.path.relpath(sq, ROOT) if valu_busy else None,
+                     "valu_lane_insts_per_point_iteration": valu_per_pt,
+                     "valu_achieved_Tops": valu_t,
+                     "valu_peak_Tops": VALU_PEAK_TOPS,
+                     "valu_frac": valu_t / VALU_PEAK_TOPS if valu_t else None},
         "cell_iter_per_s": head["cell_iter_per_s"],
         "p50_ms": head["p50_ms"],
         "p99_ms": head["p99_ms"],
