@@ -2259,7 +2259,7 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
 // LDS of three workgroups of eight waves fills a CU).
 template <typename CT, typename VT, int NS, int W>
 __global__ __launch_bounds__(64 * NS * W)
-__attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? (W == 1 ? 7 : 6) : 1))) void k_chain(const RveDev* __restrict__ rves,
+__attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? (W == 1 ? 7 : 6) : (W == 1 ? 5 : 1)))) void k_chain(const RveDev* __restrict__ rves,
                                                    const int4* __restrict__ segs, int nsegs,
                                                    int parity, int vpar) {
   constexpr int UW = 60 * W + 4;
