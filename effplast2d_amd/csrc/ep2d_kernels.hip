@@ -2256,7 +2256,9 @@ __device__ __forceinline__ void chain_run(const RveDev& r, int cur, int vpar, in
 // One segment per workgroup of NS x W waves (stage k on waves kW .. kW+W-1,
 // wave c of a stage on columns s+60c ..).  The split scheme's register
 // budget holds seven waves per SIMD (six with two waves per stage, where the
-// LDS of three workgroups of eight waves fills a CU).
+// LDS of three workgroups of eight waves fills a CU); FP64 chains of one
+// wave per stage are held to five (a few registers spilled, 2-4% faster than
+// four), with two waves per stage they run four.
 template <typename CT, typename VT, int NS, int W>
 __global__ __launch_bounds__(64 * NS * W)
 __attribute__((amdgpu_waves_per_eu(sizeof(CT) == 4 ? (W == 1 ? 7 : 6) : (W == 1 ? 5 : 1)))) void k_chain(const RveDev* __restrict__ rves,
