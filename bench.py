@@ -202,7 +202,10 @@ def batch_stats_summary(st, mode):
             "tuning_us": cand_us(st),
             "cell_iterations": st["cellIterations"], "bytes_algorithmic": st["bytesAlgorithmic"],
             # time steps the split scheme finished in the FP64 arithmetic
-            "fp64_finishes": st["fp64Finishes"]}
+            "fp64_finishes": st["fp64Finishes"],
+            # share of the cell-iterations advanced in the FP64 arithmetic
+            "fp64_cell_iter_frac": (st["fp64CellIterations"] / st["cellIterations"]
+                                    if st["cellIterations"] else 0.0)}
 
 
 def run_batch(reqs, mode, check, timing_every=1, **opts):

@@ -136,7 +136,7 @@ class BatchStats(C.Structure):
                 ("kernel", C.c_int), ("kernelRows", C.c_int), ("tunings", C.c_int),
                 ("nCand", C.c_int), ("candKind", C.c_int * MAX_CAND),
                 ("candRows", C.c_int * MAX_CAND), ("candUs", C.c_double * MAX_CAND),
-                ("fp64Finishes", C.c_longlong)]
+                ("fp64Finishes", C.c_longlong), ("fp64CellIterations", C.c_longlong)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

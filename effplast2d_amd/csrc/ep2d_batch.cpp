@@ -256,13 +256,14 @@ void Batch::dropGraphs() {
 // (one iteration), 56 (two), 52 (three) or 48 (four) output columns by runs
 // of `rows` rows, entries (RVE index, s, j0, j1).  The RVE's current buffers are in its
 // table entry, so the list stays valid while the RVEs iterate.
-int Batch::buildWork(Shape sh, int4* out) const {
+int Batch::buildWork(Shape sh, int4* out, bool esc) const {
   int t = 0;
   for (size_t k = 0; k < active_.size(); k++) {
     const Solver& s = *active_[k].s;
     // the RVEs finishing a time step in the FP64 arithmetic are in the
-    // escalated list only (tiles of 8 rows), the others in every other list
-    if (s.escalated() != (sh.kind == kEsc)) continue;
+    // escalated lists only (esc, or the tiles of 8 rows of kEsc), the others
+    // in every other list
+    if (s.escalated() != (esc || sh.kind == kEsc)) continue;
     if (sh.kind == kMarch) {
       t += march_segments(s.nX(), s.nY(), sh.rows, (int)k, out ? out + t : nullptr);
       continue;
@@ -344,9 +345,45 @@ void Batch::enqueue(long n, bool dbl, long& l, long& d) {
   }
 }
 
-// The shapes of the closing launches (rem3Off_, rem2Off_).
+// The shapes of the closing launches (rem3Off_, rem2Off_), and the chain
+// shape of the RVEs finishing a time step in the FP64 arithmetic.
 const Batch::Shape Batch::kClose3{Batch::kTriple, 42};
 const Batch::Shape Batch::kClose2{Batch::kPair, 26};
+const Batch::Shape Batch::kEscMain{Batch::kQuad2, 64};
+
+// c iterations of the RVEs finishing a time step in the FP64 arithmetic (FP64
+// stress and velocity arithmetic on the split state: FP64 U, FP32 V), with
+// the U and V parities of a chunk of their own.  On the chain kernels (the
+// four-stage chains on two waves, closed like every chunk, EffPlast2D.cu:428)
+// when they all fit them, else c launches of the tile kernel (U flips each
+// launch, V in place).
+Batch::Seq Batch::runEsc(long c) {
+  if (!escChains_) {
+    for (long k = 0; k < c; k++)
+      launch_fused_batch(dRves_, dTiles_ + escOff_, escCnt_, kModeSplitStress64, 8,
+                         (int)(k & 1), stream_);
+    return Seq{c, 0};
+  }
+  const int m = kModeSplitStress64;
+  long packs = 0, singles = 0, l = 0, d = 0;
+  split_multi(c, 4, packs, singles);
+  for (long k = 0; k < packs; k++, l++, d++)
+    launch_chain(4, dRves_, dTiles_ + escMainOff_, escMainCnt_, m, (int)(l & 1), (int)(d & 1),
+                 stream_, nullptr, nullptr, 2);
+  if (singles == 4) {
+    launch_chain(3, dRves_, dTiles_ + escC3Off_, escC3Cnt_, m, (int)(l & 1), (int)(d & 1), stream_,
+                 nullptr, nullptr, 1);
+    l++, d++, singles = 1;
+  } else if (singles == 3) {
+    launch_march2(dRves_, dTiles_ + escC2Off_, escC2Cnt_, m, (int)(l & 1), (int)(d & 1), stream_,
+                  nullptr, nullptr, true);
+    l++, d++, singles = 1;
+  }
+  for (long k = 0; k < singles; k++, l++)
+    launch_march(dRves_, dTiles_ + escAuxOff_, escAuxCnt_, m, (int)(l & 1), (int)(d & 1),
+                 stream_);
+  return Seq{l, d};
+}
 
 // The 1..n single iterations that close a chunk of an n-iteration shape:
 // the last one is a single-iteration launch (it reads U^{n-1}); four of them
@@ -549,6 +586,9 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
     const Shape aux{kMarch, shape_.rows};
     const Shape esc{kEsc, 8};
     size_t need = (size_t)buildWork(shape_, nullptr) + (size_t)buildWork(esc, nullptr);
+    need += (size_t)buildWork(kEscMain, nullptr, true) + (size_t)buildWork(kClose3, nullptr, true) +
+            (size_t)buildWork(kClose2, nullptr, true) +
+            (size_t)buildWork(Shape{kMarch, kEscMain.rows}, nullptr, true);
     const bool close3 = ipl(shape_.kind) == 4, close2 = ipl(shape_.kind) >= 3;
     if (multiIter(shape_.kind)) need += (size_t)buildWork(aux, nullptr);
     if (close3) need += (size_t)buildWork(kClose3, nullptr);
@@ -573,6 +613,29 @@ void Batch::prepareWork(const std::vector<long long>& ids, bool tuning) {
     escOff_ = off;
     escCnt_ = buildWork(esc, hTiles_ + off);
     off += escCnt_;
+    // the same RVEs on the chain kernels in the FP64 arithmetic, when all of
+    // them fit the multi-iteration kernels
+    escChains_ = false;
+    escMainCnt_ = escC3Cnt_ = escC2Cnt_ = escAuxCnt_ = 0;
+    if (escCnt_ > 0) {
+      escChains_ = true;
+      for (const auto& sl : active_)
+        if (sl.s->escalated() && !multi_iter_fits(sl.s->nX(), sl.s->nY())) escChains_ = false;
+    }
+    if (escChains_) {
+      escMainOff_ = off;
+      escMainCnt_ = buildWork(kEscMain, hTiles_ + off, true);
+      off += escMainCnt_;
+      escC3Off_ = off;
+      escC3Cnt_ = buildWork(kClose3, hTiles_ + off, true);
+      off += escC3Cnt_;
+      escC2Off_ = off;
+      escC2Cnt_ = buildWork(kClose2, hTiles_ + off, true);
+      off += escC2Cnt_;
+      escAuxOff_ = off;
+      escAuxCnt_ = buildWork(Shape{kMarch, kEscMain.rows}, hTiles_ + off, true);
+      off += escAuxCnt_;
+    }
     if (tuning)
       for (int k = 0; k < kNCand; k++) {
         candOff_[k] = off;
@@ -716,16 +779,13 @@ void Batch::step(int* nActive, int* nQueued) {
     for (auto& sl : active_) c = std::min(c, sl.s->tsChunk());
     c = std::max<long>(c, 1);
     const Seq q = launchChunk(c);
-    // the RVEs finishing a time step in the FP64 arithmetic: c one-iteration
-    // launches of the tile kernel over their own list (U flips each launch,
-    // V is updated in place)
-    for (long k = 0; k < c && escCnt_ > 0; k++)
-      launch_fused_batch(dRves_, dTiles_ + escOff_, escCnt_, kModeSplitStress64, 8,
-                         (int)(k & 1), stream_);
+    // the RVEs finishing a time step in the FP64 arithmetic, on their own lists
+    const Seq qe = escCnt_ > 0 ? runEsc(c) : Seq{0, 0};
     for (auto& sl : active_) {
       if (sl.s->escalated()) {
-        sl.s->noteBatchIterations(c, c, 0);
+        sl.s->noteBatchIterations(c, qe.launches, qe.doubles);
         st_.cellIterations += (long long)c * sl.s->nX() * sl.s->nY();
+        st_.fp64CellIterations += (long long)c * sl.s->nX() * sl.s->nY();
       } else {
         sl.s->noteBatchIterations(c, q.launches, q.doubles);
       }

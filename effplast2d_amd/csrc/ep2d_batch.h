@@ -80,6 +80,7 @@ class Batch {
   static constexpr int kRefCands = 3, kRefRounds = 4, kRefLaunches = 4;
   static const Shape kCand[kNCand];
   static const Shape kClose3, kClose2;  // launches that close a chunk (closing())
+  static const Shape kEscMain;          // chains of the FP64 finish (runEsc())
   static_assert(kNCand <= EP2D_MAX_CAND, "candidate table exceeds the stats record");
   struct GraphEntry {
     long len = 0;
@@ -101,7 +102,10 @@ class Batch {
   void prepareWork(const std::vector<long long>& ids, bool tuning);
   static int candIndex(Shape sh);
   Seq runRest(long c, long it, long l, long d, double bytes, double cells, bool tuning);
-  int buildWork(Shape sh, int4* out) const;  // work list of all admitted RVEs
+  // work list of the admitted RVEs for a shape: those advancing in their own
+  // scheme, or (esc) those finishing a time step in the FP64 arithmetic
+  int buildWork(Shape sh, int4* out, bool esc = false) const;
+  Seq runEsc(long c);  // c iterations of the RVEs in the FP64 arithmetic
   void launchOne(Shape sh, const int4* work, int n, bool dbl, long l, long d,
                  hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
   void enqueue(long n, bool dbl, long& l, long& d);
@@ -142,6 +146,12 @@ class Batch {
   // a four- or three-iteration shape (closing())
   int rem3Off_ = 0, rem3Cnt_ = 0, rem2Off_ = 0, rem2Cnt_ = 0;
   int escOff_ = 0, escCnt_ = 0;   // tiles of the RVEs in the FP64 arithmetic
+  // their chain segments (four stages on two waves), closing three- and two-
+  // iteration segments and single-iteration segments, when every one of
+  // them fits the multi-iteration kernels (escChains_)
+  int escMainOff_ = 0, escMainCnt_ = 0, escC3Off_ = 0, escC3Cnt_ = 0;
+  int escC2Off_ = 0, escC2Cnt_ = 0, escAuxOff_ = 0, escAuxCnt_ = 0;
+  bool escChains_ = false;
 
   // kernel-shape tuning on real iterations
   bool tuneDue_ = true;

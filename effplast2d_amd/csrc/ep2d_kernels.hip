@@ -1641,10 +1641,12 @@ struct MarchStage {
     o.updX = (!EX || (i > 0 && i < nX)) && (!EY || (j > 0 && j < nY - 1));
     o.updY = (!EX || (i > 0 && i < nX - 1)) && (!EY || (j > 0 && j < nY));
     if constexpr (sizeof(CT) == 8) {
+      // U advances with the velocity before its rounding to the storage type
+      // (as the tile and march kernels do); the next stage gets the stored value
       if (o.updX)
-        vx = (CT)(VT)(vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p)));
+        vx = vx * vdampX + dTrho * (dxs(-cP_c + rP + cXX_c - rXX) + dys(rXY_c - rXY_p));
       if (o.updY)
-        vy = (CT)(VT)(vy * vdampY + dTrho * (dys(-cP_c + cP_p + cYY_c - cYY_p) + dxs(cXY_p - rXY_p)));
+        vy = vy * vdampY + dTrho * (dys(-cP_c + cP_p + cYY_c - cYY_p) + dxs(cXY_p - rXY_p));
     } else {
       if (o.updX)
         vx = (CT)(VT)split_velocity(vx, vdampX, dTrho, -cP_c + rP + cXX_c - rXX, rdXc,
@@ -1654,8 +1656,8 @@ struct MarchStage {
                                     cXY_p - rXY_p, rdXc);
     }
     const bool in = (!EX || inX) && (!EY || (j >= 0 && j < nY));
-    o.vx = vx;
-    o.vy = vy;
+    o.vx = (CT)(VT)vx;
+    o.vy = (CT)(VT)vy;
     if constexpr (sizeof(CT) == 8) {
       o.ux = in ? ux_c + (double)vx * dT : ux_c;
       o.uy = in ? uy_c + (double)vy * dT : uy_c;
@@ -3131,6 +3133,7 @@ void launch_march(const RveDev* rves, const int4* segs, int nsegs, int mode, int
       hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
   };
   if (mode == 2) go(k_march<float, float>);  // split: FP32 stress and velocity
+  else if (mode == kModeSplitStress64) go(k_march<double, float>);  // split state, FP64 stress
   else go(k_march<double, double>);
   EP2D_HIP(hipGetLastError());
 }
@@ -3171,7 +3174,18 @@ void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int m
       hipLaunchKernelGGL(kern, grid, block, 0, s, rves, segs, nsegs, parity, vpar);
   };
   const bool f32 = mode == 2;  // split: FP32 stress and velocity
-  if (w == 1 && ns == 3) {
+  // the split state (FP32 V) advanced with the FP64 arithmetic: the
+  // three-iteration chains of three waves and the four-iteration chains of
+  // two waves per stage (the shapes the engine's FP64 finish runs)
+  const bool s64 = mode == kModeSplitStress64;
+  if (s64 && w == 1 && ns == 3) {
+    go(k_chain<double, float, 3, 1>);
+  } else if (s64 && w == 2 && ns == 4) {
+    go(k_chain<double, float, 4, 2>);
+  } else if (s64) {
+    throw std::invalid_argument("the FP64 finish runs chains of three stages on one wave or "
+                                "four stages on two waves");
+  } else if (w == 1 && ns == 3) {
     if (f32) go(k_chain<float, float, 3, 1>);
     else go(k_chain<double, double, 3, 1>);
   } else if (w == 1 && ns == 4) {
@@ -3239,6 +3253,7 @@ void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, in
   };
   if (pairs) {
     if (mode == 2) go(k_march2p<float, float>);
+    else if (mode == kModeSplitStress64) go(k_march2p<double, float>);
     else go(k_march2p<double, double>);
   } else if (mode == 2) {
     go(k_march2<float, float>);  // split: FP32 stress and velocity

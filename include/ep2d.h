@@ -403,6 +403,8 @@ typedef struct ep2d_batch_stats {
                                      candidate k (0 if not timed)             */
   long long fp64Finishes;         /* time steps finished in the FP64 arithmetic
                                      (ep2d_options.keepFp32Stress)            */
+  long long fp64CellIterations;   /* cells x iterations advanced in the FP64
+                                     arithmetic (part of cellIterations)      */
 } ep2d_batch_stats;
 
 typedef struct ep2d_batch ep2d_batch;

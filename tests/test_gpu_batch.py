@@ -431,7 +431,9 @@ def test_split_stall_finishes_in_fp64_arithmetic():
     FP32 stress noise holds the split residual near 3e-7, above eIter =
     1e-8.  With the FP64 finish (the default) every time step reaches eIter
     and the moduli stay within 1e-4 of the FP64 solve; with the FP32 stress
-    kept, the steps run to the iteration cap."""
+    kept, the steps run to the iteration cap.  In the batched engine, beside
+    a request that does not stall, the FP64 finish runs on the chain kernels
+    and gives the single handle's bits."""
     rq = _sampled()[4]
     assert abs(rq["initLoad"] * rq["loadType"][0]) < abs(rq["initLoad"] * rq["loadType"][1]) / 30
     _note("split stall")
@@ -452,3 +454,18 @@ def test_split_stall_finishes_in_fp64_arithmetic():
         e = abs(res_s["num"][k] - res64["num"][k]) / abs(res64["num"][k])
         _note(f"  {k}: rel {e:.2e}")
         assert e <= 1e-4, k
+    # in the engine, beside a request that does not stall, the stalled steps
+    # finish on the chain kernels in the FP64 arithmetic (the single handle
+    # runs the tile kernel there): the same bits
+    other = _sampled()[0]
+    with Batch(mode="split", checkInterval=K) as b:
+        for q in (rq, other):
+            b.submit(q["pa"], q["nX"], nTasks=q["nTasks"], nIter=4 * q["nIter"],
+                     eIter=q["eIter"], initLoad=q["initLoad"], loadType=q["loadType"],
+                     tag=q["tag"])
+        b.run()
+        got = {r["tag"]: r["result"] for r in b.poll()}
+        st = b.stats()
+    assert st["fp64Finishes"] >= 1 and st["fp64CellIterations"] > 0
+    _same(got[rq["tag"]], res_s)
+
