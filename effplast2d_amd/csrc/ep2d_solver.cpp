@@ -578,13 +578,13 @@ void Solver::launchMulti(int q) {
   std::pair<hipEvent_t, hipEvent_t> ev{};
   if (timed) ev = takeTimingEvents();
   if (ipl_ == 2)
-    launch_march2(dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
+    launch_march2(dRve_, dSegs2_, nSegs2_, iterMode(), cur_, vIdx_ ^ (q & 1), stream_,
                   timed ? ev.first : nullptr, timed ? ev.second : nullptr, true);
   else if (wide_)
     launch_wide(ipl_, dRve_, dSegs2_, nSegs2_, cur_, vIdx_ ^ (q & 1), stream_,
                 timed ? ev.first : nullptr, timed ? ev.second : nullptr);
   else
-    launch_chain(ipl_, dRve_, dSegs2_, nSegs2_, mode_, cur_, vIdx_ ^ (q & 1), stream_,
+    launch_chain(ipl_, dRve_, dSegs2_, nSegs2_, iterMode(), cur_, vIdx_ ^ (q & 1), stream_,
                  timed ? ev.first : nullptr, timed ? ev.second : nullptr, stageWaves_);
   cur_ ^= 1;
   stressStale_ = true;
@@ -602,9 +602,20 @@ void Solver::flipV() {
   std::swap(r_.Vy, r_.Vy2);
 }
 
+// The FP64 finish of a split load step runs on the handle's multi-iteration
+// shape when that shape has an FP64-arithmetic variant (wave pairs, three
+// stages on one wave, four stages on two waves), else on tiles of 8 rows;
+// both give the same bits.
+bool Solver::multi64() const {
+  return multiIter_ && !wide_ &&
+         (ipl_ == 2 || (ipl_ == 3 && stageWaves_ == 1) || (ipl_ == 4 && stageWaves_ == 2));
+}
+
+int Solver::iterMode() const { return stress64_ ? kModeSplitStress64 : mode_; }
+
 void Solver::iterate(long n) {
   EP2D_HIP(hipSetDevice(device_));
-  if (!multiIter_ || stress64_) {
+  if (!multiIter_ || (stress64_ && !multi64())) {
     for (long k = 0; k < n; k++) launchIteration();
     return;
   }
@@ -635,11 +646,11 @@ void Solver::iterate(long n) {
 // vIdx_ ^ q and written to the other, as in launchMulti.
 void Solver::launchClose(int ns, int q) {
   if (ns == 3)
-    launch_chain(3, dRve_, dSegsC3_, nSegsC3_, mode_, cur_, vIdx_ ^ q, stream_, nullptr,
+    launch_chain(3, dRve_, dSegsC3_, nSegsC3_, iterMode(), cur_, vIdx_ ^ q, stream_, nullptr,
                  nullptr, 1);
   else
-    launch_march2(dRve_, dSegsC2_, nSegsC2_, mode_, cur_, vIdx_ ^ q, stream_, nullptr, nullptr,
-                  true);
+    launch_march2(dRve_, dSegsC2_, nSegsC2_, iterMode(), cur_, vIdx_ ^ q, stream_, nullptr,
+                  nullptr, true);
   cur_ ^= 1;
   stressStale_ = true;
 }

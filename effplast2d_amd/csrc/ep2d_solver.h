@@ -184,6 +184,8 @@ class Solver {
   std::pair<hipEvent_t, hipEvent_t> takeTimingEvents();
   void launchMulti(int q);         // ipl_ iterations, q-th such launch of a run
   void launchClose(int ns, int q);  // ns = 3 or 2 iterations closing a run
+  bool multi64() const;            // the FP64 finish runs on the multi-iteration shape
+  int iterMode() const;            // kernel arithmetic of the next iterations
   int cur_ = 0;
   bool stressStale_ = false;
   unsigned long long* dMax_ = nullptr;

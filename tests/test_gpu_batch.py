@@ -468,4 +468,11 @@ def test_split_stall_finishes_in_fp64_arithmetic():
         st = b.stats()
     assert st["fp64Finishes"] >= 1 and st["fp64CellIterations"] > 0
     _same(got[rq["tag"]], res_s)
+    # a single handle on a multi-iteration shape finishes on that shape
+    for shape in (dict(marchRows=16, itersPerLaunch=4, stageWaves=2),
+                  dict(marchRows=16, itersPerLaunch=3),
+                  dict(marchRows=14, itersPerLaunch=2)):
+        with EffPlast2D(rq["pa"], rq["nX"], nTasks=rq["nTasks"], nIter=4 * rq["nIter"],
+                        eIter=rq["eIter"], checkInterval=K, mode="split", **shape) as g:
+            _same(g.ComputeEffModuli(rq["initLoad"], rq["loadType"]), res_s)
 
