@@ -807,9 +807,9 @@ def main():
         summarize(a.summarize, a.summary_out, a.pmc_command)
         return 0
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+        print(f"bench.py: the solver runs on one GPU only (asked for --gpus {a.gpus}); "
               "a multi-GPU run would shard independent RVEs across devices and is not "
-              "part of this build", file=sys.stderr)
+              "implemented", file=sys.stderr)
         return 2
     if a.probe_shapes:
         probe_shapes(a.probe_mode, a.nx, a.probe_iters, a.probe_batch or 8,
