@@ -723,27 +723,46 @@ __global__ __launch_bounds__(64 * NW) void k_fused(RveDev r, int cur, int ntx, i
 // and a count of non-finite values.  Wave64 shuffle reduction, then one
 // 64-bit atomicMax per block on the bit pattern (monotone for values >= 0).
 // NaN is skipped as in FindMaxAbs, and counted.
-template <typename VT>
+template <typename VT, bool VEC> struct Vec16 { using T = VT; static constexpr int N = 1; };
+template <> struct Vec16<float, true> { using T = float4; static constexpr int N = 4; };
+template <> struct Vec16<double, true> { using T = double2; static constexpr int N = 2; };
+
+__device__ __forceinline__ void maxabs_acc(double a, double& m, unsigned long long& bad) {
+  const double v = fabs(a);
+  if (v > m) m = v;                               // NaN never replaces the max
+  if (!(v <= 1.7976931348623157e308)) bad++;      // NaN and infinity
+}
+
+template <typename VT, int N, typename V>
+__device__ __forceinline__ void maxabs_vec(const V& v, double& m, unsigned long long& bad) {
+  const VT* e = reinterpret_cast<const VT*>(&v);
+#pragma unroll
+  for (int q = 0; q < N; q++) maxabs_acc((double)e[q], m, bad);
+}
+
+// Both arrays are read in one loop, so each thread has loads of both in
+// flight: as 16-byte vectors when both start 16-byte aligned (VEC), then the
+// few elements past the last whole vector.
+template <typename VT, bool VEC>
 __global__ __launch_bounds__(256) void k_maxabs(const VT* __restrict__ Vx, size_t nx,
                                                 const VT* __restrict__ Vy, size_t ny,
                                                 unsigned long long* out) {
+  using V = typename Vec16<VT, VEC>::T;
+  constexpr int N = Vec16<VT, VEC>::N;
   double mx = 0.0, my = 0.0;
   unsigned long long bad = 0;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  // a few blocks loop over many elements (one atomic per block); the
-  // unrolled loops keep several loads in flight per thread
-#pragma unroll 4
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nx; k += stride) {
-    const double v = fabs((double)Vx[k]);
-    if (v > mx) mx = v;
-    if (!(v <= 1.7976931348623157e308)) bad++;
+  const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t nxv = nx / N, nyv = ny / N, nv = nxv > nyv ? nxv : nyv;
+  const V* __restrict__ X = reinterpret_cast<const V*>(Vx);
+  const V* __restrict__ Y = reinterpret_cast<const V*>(Vy);
+#pragma unroll 2
+  for (size_t k = t0; k < nv; k += stride) {
+    if (k < nxv) maxabs_vec<VT, N>(X[k], mx, bad);
+    if (k < nyv) maxabs_vec<VT, N>(Y[k], my, bad);
   }
-#pragma unroll 4
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < ny; k += stride) {
-    const double v = fabs((double)Vy[k]);
-    if (v > my) my = v;
-    if (!(v <= 1.7976931348623157e308)) bad++;
-  }
+  for (size_t k = nxv * N + t0; k < nx; k += stride) maxabs_acc((double)Vx[k], mx, bad);
+  for (size_t k = nyv * N + t0; k < ny; k += stride) maxabs_acc((double)Vy[k], my, bad);
   for (int off = 32; off > 0; off >>= 1) {
     mx = fmax(mx, __shfl_xor(mx, off, 64));
     my = fmax(my, __shfl_xor(my, off, 64));
@@ -3274,15 +3293,24 @@ const char* fused_kernel_name(int mode) {
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s) {
   const size_t nx = (size_t)(r.nX + 1) * r.nY, ny = (size_t)r.nX * (r.nY + 1);
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, out, 3);
-  // about 16 elements of each array per thread, at most 1024 blocks: the
-  // blocks' atomics on the three result words serialise, so fewer is faster
+  // 16 elements of each array per thread, at most 1024 blocks: the blocks'
+  // atomics on the three result words serialise, so fewer is faster
   const int blocks = (int)std::max<size_t>(1, std::min<size_t>((std::max(nx, ny) + 4095) / 4096, 1024));
-  if (f32)
-    hipLaunchKernelGGL((k_maxabs<float>), dim3(blocks), dim3(256), 0, s,
-                       static_cast<const float*>(r.Vx), nx, static_cast<const float*>(r.Vy), ny, out);
-  else
-    hipLaunchKernelGGL((k_maxabs<double>), dim3(blocks), dim3(256), 0, s,
-                       static_cast<const double*>(r.Vx), nx, static_cast<const double*>(r.Vy), ny, out);
+  const bool vec = ((reinterpret_cast<uintptr_t>(r.Vx) | reinterpret_cast<uintptr_t>(r.Vy)) & 15u) == 0;
+  auto go = [&](auto kern, auto* vx, auto* vy) {
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), 0, s, vx, nx, vy, ny, out);
+  };
+  if (f32) {
+    const float* vx = static_cast<const float*>(r.Vx);
+    const float* vy = static_cast<const float*>(r.Vy);
+    if (vec) go(k_maxabs<float, true>, vx, vy);
+    else go(k_maxabs<float, false>, vx, vy);
+  } else {
+    const double* vx = static_cast<const double*>(r.Vx);
+    const double* vy = static_cast<const double*>(r.Vy);
+    if (vec) go(k_maxabs<double, true>, vx, vy);
+    else go(k_maxabs<double, false>, vx, vy);
+  }
   EP2D_HIP(hipGetLastError());
 }
 

@@ -395,3 +395,52 @@ def test_full_size_split_pairs_equal_tiles():
         for k in STATE_FIELDS + STRESS_FIELDS:
             assert np.array_equal(o[k], out[0][k]), \
                 f"{k}: {np.count_nonzero(o[k] != out[0][k])} differ"
+
+
+@pytest.mark.parametrize("f32", [True, False])
+def test_residual_vector_and_element_reads_agree(f32):
+    """The residual kernel reads velocity fields that start 16-byte aligned
+    as 16-byte vectors and others element by element: on the same values
+    (fields of (nX+1) nX elements, not a whole number of vectors, two NaNs
+    planted) both give the same residual bits and non-finite count, and the
+    largest value, put in the last element past the vectors, is seen."""
+    import ctypes as C
+    import torch
+    from effplast2d_amd._lib import check, lib
+    dt = torch.float32 if f32 else torch.float64
+    stream = torch.cuda.current_stream().cuda_stream
+    e = (C.c_double * 3)(-2e-3, -1e-3, 5e-4)
+
+    def residual(vx, vy, nX):
+        err = torch.zeros(1, dtype=torch.float64, device="cuda")
+        bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+        check(lib().ep2d_residual_noalloc(
+            C.c_void_p(vx.data_ptr()), C.c_void_p(vy.data_ptr()), nX, nX, 1 if f32 else 0,
+            C.c_double(0.02), C.c_double(0.02), C.c_double(0.007), e,
+            C.c_void_p(err.data_ptr()), C.c_void_p(bad.data_ptr()), C.c_void_p(stream)))
+        torch.cuda.current_stream().synchronize()
+        return float(err.item()), int(bad.item())
+
+    for nX in (97, 1023):
+        n = (nX + 1) * nX
+        _note(f"residual reads f32={f32} nX={nX}")
+        gen = torch.Generator().manual_seed(nX)
+        vals = torch.randn(2, n, generator=gen, dtype=torch.float64)
+        vals[0, n - 1] = 40.0             # the largest |Vx|, past the last whole vector
+        vals[1, 0] = -30.0                # the largest |Vy|, in the first vector
+        vals[0, 7] = float("nan")
+        vals[1, n // 2] = float("nan")
+        buf = torch.zeros(2, (n + 11) // 4 * 4, dtype=dt, device="cuda")
+        assert buf.data_ptr() % 16 == 0 and buf[1].data_ptr() % 16 == 0
+        out = {}
+        for off in (0, 1, 3):             # 16-byte aligned, then element-aligned only
+            buf.zero_()
+            buf[:, off:off + n] = vals.to(dt).cuda()
+            out[off] = residual(buf[0, off:off + n], buf[1, off:off + n], nX)
+        assert out[0] == out[1] == out[3], out
+        assert out[0][1] == 2
+        vals[0, n - 1] = 0.0
+        buf.zero_()
+        buf[:, :n] = vals.to(dt).cuda()
+        smaller = residual(buf[0, :n], buf[1, :n], nX)
+        assert smaller[0] < out[0][0], (smaller, out[0])
