@@ -107,6 +107,10 @@ Batch::~Batch() {
   if (dTiles_) (void)hipFree(dTiles_);
   if (hRves_) (void)hipHostFree(hRves_);
   if (hTiles_) (void)hipHostFree(hTiles_);
+  if (dJobs_) (void)hipFree(dJobs_);
+  if (dRes_) (void)hipFree(dRes_);
+  if (hJobs_) (void)hipHostFree(hJobs_);
+  if (hRes_) (void)hipHostFree(hRes_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -240,6 +244,28 @@ void Batch::ensureCapacity(size_t nRves, size_t nTiles) {
     EP2D_HIP(hipHostMalloc((void**)&hTiles_, cap * sizeof(int4), hipHostMallocDefault));
     capTiles_ = cap;
   }
+}
+
+// Room for n residual jobs (called between chunks, after the stream was
+// synchronised, so no copy or kernel still uses the old buffers).
+void Batch::ensureJobs(size_t n) {
+  if (n <= capJobs_) return;
+  const size_t cap = std::max<size_t>(n, 2 * capJobs_);
+  if (dJobs_) EP2D_HIP(hipFree(dJobs_));
+  if (dRes_) EP2D_HIP(hipFree(dRes_));
+  if (hJobs_) EP2D_HIP(hipHostFree(hJobs_));
+  if (hRes_) EP2D_HIP(hipHostFree(hRes_));
+  dJobs_ = nullptr;
+  dRes_ = nullptr;
+  hJobs_ = nullptr;
+  hRes_ = nullptr;
+  capJobs_ = 0;
+  EP2D_HIP(hipMalloc((void**)&dJobs_, cap * sizeof(MaxabsJob)));
+  EP2D_HIP(hipMalloc((void**)&dRes_, 3 * cap * sizeof(unsigned long long)));
+  EP2D_HIP(hipHostMalloc((void**)&hJobs_, cap * sizeof(MaxabsJob), hipHostMallocDefault));
+  EP2D_HIP(hipHostMalloc((void**)&hRes_, 3 * cap * sizeof(unsigned long long),
+                         hipHostMallocDefault));
+  capJobs_ = cap;
 }
 
 void Batch::dropGraphs() {
@@ -781,7 +807,11 @@ void Batch::step(int* nActive, int* nQueued) {
     const Seq q = launchChunk(c);
     // the RVEs finishing a time step in the FP64 arithmetic, on their own lists
     const Seq qe = escCnt_ > 0 ? runEsc(c) : Seq{0, 0};
-    for (auto& sl : active_) {
+    // the RVEs at a residual check: job at[k] of one reduction launch
+    std::vector<int> at(active_.size(), -1);
+    int nj = 0;
+    for (size_t k = 0; k < active_.size(); k++) {
+      Slot& sl = active_[k];
       if (sl.s->escalated()) {
         sl.s->noteBatchIterations(c, qe.launches, qe.doubles);
         st_.cellIterations += (long long)c * sl.s->nX() * sl.s->nY();
@@ -789,7 +819,22 @@ void Batch::step(int* nActive, int* nQueued) {
       } else {
         sl.s->noteBatchIterations(c, q.launches, q.doubles);
       }
-      if (sl.s->tsAtCheck()) sl.s->tsCheckLaunch();
+      if (sl.s->tsAtCheck()) at[k] = nj++;
+    }
+    if (nj > 0) {
+      ensureJobs((size_t)nj);
+      const bool f32 = active_.front().s->splitVelocity();
+      for (size_t k = 0; k < active_.size(); k++) {
+        if (at[k] < 0) continue;
+        if (active_[k].s->splitVelocity() != f32)
+          throw std::logic_error("batch: velocity storage differs between RVEs");
+        hJobs_[at[k]] = active_[k].s->residualJob();
+      }
+      EP2D_HIP(hipMemcpyAsync(dJobs_, hJobs_, (size_t)nj * sizeof(MaxabsJob),
+                              hipMemcpyHostToDevice, stream_));
+      launch_maxabs_multi(hJobs_, dJobs_, nj, f32, dRes_, stream_);
+      EP2D_HIP(hipMemcpyAsync(hRes_, dRes_, 3 * (size_t)nj * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, stream_));
     }
     EP2D_HIP(hipStreamSynchronize(stream_));
     resolveEvents();
@@ -797,7 +842,7 @@ void Batch::step(int* nActive, int* nQueued) {
     for (size_t k = 0; k < active_.size(); k++) {
       Slot& sl = active_[k];
       try {
-        if (sl.s->tsAtCheck()) sl.s->tsCheckFinish();
+        if (at[k] >= 0) sl.s->tsCheckFinish(hRes_ + 3 * (size_t)at[k]);
         if (sl.s->tsOver()) {
           sl.s->tsFinish();
           if (sl.s->jobDone()) {

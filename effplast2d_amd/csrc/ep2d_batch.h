@@ -133,6 +133,14 @@ class Batch {
   RveDev* dRves_ = nullptr;
   int4* dTiles_ = nullptr;
   RveDev* hRves_ = nullptr;
+  // the residual checks of one chunk, reduced in one launch: job table
+  // (pinned host and device copies) and three result words per job
+  MaxabsJob* hJobs_ = nullptr;
+  MaxabsJob* dJobs_ = nullptr;
+  unsigned long long* hRes_ = nullptr;
+  unsigned long long* dRes_ = nullptr;
+  size_t capJobs_ = 0;
+  void ensureJobs(size_t n);
   int4* hTiles_ = nullptr;
   size_t capRves_ = 0, capTiles_ = 0;
   int nTiles_ = 0;

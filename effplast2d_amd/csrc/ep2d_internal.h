@@ -182,6 +182,17 @@ void launch_fused_persist(const RveDev* rves, const int4* tiles, int ntiles, int
                         hipEvent_t e1 = nullptr);
 // residual: out[0] = bits of max|Vx|, out[1] = bits of max|Vy|, out[2] = nonfinite count
 void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream_t s);
+// One residual reduction job: an RVE's current velocity fields.
+struct MaxabsJob {
+  const void* vx;
+  const void* vy;
+  unsigned long long nx, ny;  // element counts
+};
+// launch_maxabs for n RVEs in one launch (after zeroing the 3 n result
+// words): hJobs is the host copy of the device table dJobs (for the launch
+// shape); RVE k's three words land at out[3k].
+void launch_maxabs_multi(const MaxabsJob* hJobs, const MaxabsJob* dJobs, int n, bool f32,
+                         unsigned long long* out, hipStream_t s);
 // volume averages: partial[nblocks*NAVG] then result[NAVG]
 int avg_partial_blocks(int nX, int nY);
 void launch_averages(const RveDev& r, const StressDev& f, const double* Ux,

@@ -739,19 +739,20 @@ __device__ __forceinline__ void maxabs_vec(const V& v, double& m, unsigned long 
   for (int q = 0; q < N; q++) maxabs_acc((double)e[q], m, bad);
 }
 
-// Both arrays are read in one loop, so each thread has loads of both in
-// flight: as 16-byte vectors when both start 16-byte aligned (VEC), then the
-// few elements past the last whole vector.
+// One block's share of the reduction: blocks b of nb (256 threads) stride
+// over both arrays in one loop, so each thread has loads of both in flight:
+// as 16-byte vectors when both start 16-byte aligned (VEC), then the few
+// elements past the last whole vector.
 template <typename VT, bool VEC>
-__global__ __launch_bounds__(256) void k_maxabs(const VT* __restrict__ Vx, size_t nx,
-                                                const VT* __restrict__ Vy, size_t ny,
-                                                unsigned long long* out) {
+__device__ __forceinline__ void maxabs_block(const VT* __restrict__ Vx, size_t nx,
+                                             const VT* __restrict__ Vy, size_t ny,
+                                             unsigned long long* out, unsigned b, unsigned nb) {
   using V = typename Vec16<VT, VEC>::T;
   constexpr int N = Vec16<VT, VEC>::N;
   double mx = 0.0, my = 0.0;
   unsigned long long bad = 0;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)nb * blockDim.x;
+  const size_t t0 = b * (size_t)blockDim.x + threadIdx.x;
   const size_t nxv = nx / N, nyv = ny / N, nv = nxv > nyv ? nxv : nyv;
   const V* __restrict__ X = reinterpret_cast<const V*>(Vx);
   const V* __restrict__ Y = reinterpret_cast<const V*>(Vy);
@@ -788,8 +789,26 @@ __global__ __launch_bounds__(256) void k_maxabs(const VT* __restrict__ Vx, size_
   }
 }
 
+template <typename VT, bool VEC>
+__global__ __launch_bounds__(256) void k_maxabs(const VT* __restrict__ Vx, size_t nx,
+                                                const VT* __restrict__ Vy, size_t ny,
+                                                unsigned long long* out) {
+  maxabs_block<VT, VEC>(Vx, nx, Vy, ny, out, blockIdx.x, gridDim.x);
+}
+
+// The residuals of many RVEs in one launch: blocks (b, k) reduce job k's
+// velocity fields into out[3k .. 3k+2] (gridDim.x blocks per job).
+template <typename VT, bool VEC>
+__global__ __launch_bounds__(256) void k_maxabs_multi(const MaxabsJob* __restrict__ jobs,
+                                                      unsigned long long* out) {
+  const MaxabsJob jb = jobs[blockIdx.y];
+  maxabs_block<VT, VEC>(static_cast<const VT*>(jb.vx), (size_t)jb.nx,
+                        static_cast<const VT*>(jb.vy), (size_t)jb.ny, out + 3 * (size_t)blockIdx.y,
+                        blockIdx.x, gridDim.x);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n) {
-  if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = 0ull;
 }
 
 // ---------------------------------------------------------------------------
@@ -3309,6 +3328,36 @@ void launch_maxabs(const RveDev& r, bool f32, unsigned long long* out, hipStream
     const double* vy = static_cast<const double*>(r.Vy);
     if (vec) go(k_maxabs<double, true>, vx, vy);
     else go(k_maxabs<double, false>, vx, vy);
+  }
+  EP2D_HIP(hipGetLastError());
+}
+
+void launch_maxabs_multi(const MaxabsJob* hJobs, const MaxabsJob* dJobs, int n, bool f32,
+                         unsigned long long* out, hipStream_t s) {
+  if (n <= 0) return;
+  size_t most = 0;
+  bool vec = true;
+  for (int k = 0; k < n; k++) {
+    most = std::max<size_t>(most, std::max(hJobs[k].nx, hJobs[k].ny));
+    vec = vec && ((reinterpret_cast<uintptr_t>(hJobs[k].vx) |
+                   reinterpret_cast<uintptr_t>(hJobs[k].vy)) & 15u) == 0;
+  }
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(256), 0, s, out, 3 * n);
+  // as launch_maxabs for the largest job; the blocks past a smaller job's
+  // elements add nothing
+  const unsigned bx = (unsigned)std::max<size_t>(1, std::min<size_t>((most + 4095) / 4096, 1024));
+  // at most 65535 jobs per launch (the grid's y extent)
+  for (int j0 = 0; j0 < n; j0 += 65535) {
+    const dim3 grid(bx, (unsigned)std::min(n - j0, 65535));
+    const MaxabsJob* jb = dJobs + j0;
+    unsigned long long* o = out + 3 * (size_t)j0;
+    if (f32) {
+      if (vec) hipLaunchKernelGGL((k_maxabs_multi<float, true>), grid, dim3(256), 0, s, jb, o);
+      else hipLaunchKernelGGL((k_maxabs_multi<float, false>), grid, dim3(256), 0, s, jb, o);
+    } else {
+      if (vec) hipLaunchKernelGGL((k_maxabs_multi<double, true>), grid, dim3(256), 0, s, jb, o);
+      else hipLaunchKernelGGL((k_maxabs_multi<double, false>), grid, dim3(256), 0, s, jb, o);
+    }
   }
   EP2D_HIP(hipGetLastError());
 }

@@ -1025,7 +1025,19 @@ void Solver::tsCheckLaunch() {
                           stream_));
 }
 
-bool Solver::tsCheckFinish() {
+// The velocity fields tsCheckLaunch reduces, for an engine that reduces many
+// RVEs in one launch.
+MaxabsJob Solver::residualJob() const {
+  MaxabsJob j;
+  j.vx = r_.Vx;
+  j.vy = r_.Vy;
+  j.nx = (unsigned long long)(nX_ + 1) * nY_;
+  j.ny = (unsigned long long)nX_ * (nY_ + 1);
+  return j;
+}
+
+bool Solver::tsCheckFinish(const unsigned long long* words) {
+  if (words) std::memcpy(hMax_, words, 3 * sizeof(unsigned long long));
   double mx, my;
   std::memcpy(&mx, &hMax_[0], 8);
   std::memcpy(&my, &hMax_[1], 8);

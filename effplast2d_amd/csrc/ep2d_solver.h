@@ -79,7 +79,11 @@ class Solver {
   long tsDoneIters() const { return tsDone_; }
   bool tsAtCheck() const { return tsDone_ > 0 && tsDone_ == nextCheck_; }
   void tsCheckLaunch();           // residual reduction + async copy of its result
-  bool tsCheckFinish();           // after the stream is synchronised
+  // after the stream is synchronised; words: the three result words of a
+  // reduction the caller launched over residualJob() instead
+  bool tsCheckFinish(const unsigned long long* words = nullptr);
+  MaxabsJob residualJob() const;  // the current velocity fields the check reduces
+  bool splitVelocity() const { return mode_ == EP2D_MODE_SPLIT; }
   void tsFinish();                // averages, boundary, hole area, analytic moduli
   void jobFinish(ep2d_result* out);
   void runJob(ep2d_result* out);
