@@ -277,15 +277,16 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
     // automatic shape for one RVE, from one-RVE measurements: split from
     // 1024^2 on four-stage chains with two waves per stage (four iterations
-    // per launch; 16-row segments at 1024^2, 32 at 2048^2, 48 beyond: 9.9,
-    // 22.1 and 69.3 us per iteration at 1024^2, 2048^2 and 4096^2, against
+    // per launch; 14-row segments at 1024^2, 32 at 2048^2, 48 beyond: 8.0,
+    // 19.2 and 64 us per iteration at 1024^2, 2048^2 and 4096^2, against
     // 10.3, 23.9 and 77.8 on one wave per stage and 12.1, 33.6 and 170.6
     // for the tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the
-    // tile kernel below those sizes.  Segments stay short enough to give the
-    // GPU waves.
+    // tile kernel below those sizes.  At 1024^2 the 740 workgroups of
+    // 14-row segments are one round of the GPU (three per CU): 16 rows
+    // leave SIMDs idle (8.6 us), 12 rows need a second round (9.8 us).
     const int n = std::max(nX, nY);
     if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 16 : (n <= 2048 ? 32 : 48);
+      opt_.marchRows = n <= 1024 ? 14 : (n <= 2048 ? 32 : 48);
       opt_.itersPerLaunch = 4;
       if (opt_.laneColumns != 2 && opt_.stageWaves == 0) opt_.stageWaves = 2;
     } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {

@@ -15,6 +15,11 @@ import sys
 
 import numpy as np
 import pytest
+# torch before the library: torch's HIP libraries name the runtime without
+# its version, so once the library has loaded /opt/rocm's runtime, a later
+# torch import loads a second runtime and finds no GPU; imported first, the
+# library binds to torch's runtime and the process has one
+import torch
 
 from effplast2d_amd import EffPlast2D
 from oracle import ora
@@ -405,7 +410,6 @@ def test_residual_vector_and_element_reads_agree(f32):
     planted) both give the same residual bits and non-finite count, and the
     largest value, put in the last element past the vectors, is seen."""
     import ctypes as C
-    import torch
     from effplast2d_amd._lib import check, lib
     dt = torch.float32 if f32 else torch.float64
     stream = torch.cuda.current_stream().cuda_stream

@@ -12,6 +12,7 @@ import sys
 
 import numpy as np
 import pytest
+# torch before the library (one HIP runtime in the process)
 import torch
 
 from effplast2d_amd import EffPlast2D
