@@ -160,6 +160,10 @@ void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, in
 // launch_march2.
 constexpr int chain_cols(int ns, int w = 1) { return 60 * w + 4 - 4 * ns; }
 int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out, int w = 1);
+// Column-edge segments first within each XCD's share of a chain work list
+// of n segments (ns stages of w waves; dims[rve] = (nX, nY) of the RVE a
+// segment's .x names), the others in their order.
+void chain_edges_first(int4* segs, int n, int ns, int w, const int2* dims);
 void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                   int w = 1);

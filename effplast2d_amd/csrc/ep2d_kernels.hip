@@ -3187,6 +3187,26 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
   return n;
 }
 
+// The chain kernel gives XCD x the contiguous index range [x q + min(x, r),
+// ...) of an n-entry list (q = n / 8, r = n % 8, tile_of_block), and an XCD
+// starts its workgroups in index order.  Segments on the column-edge path
+// take about a third longer than interior ones (20 x 1024^2, four stages on
+// two waves: 123 against 90 us); started first, they leave workgroups of
+// more even length for the end of the launch (3% shorter launches there;
+// moving the row-edge segments forward as well gains less).  Segments write
+// disjoint outputs, so the order changes no result.
+void chain_edges_first(int4* segs, int n, int ns, int w, const int2* dims) {
+  (void)ns;
+  const int q = n >> 3, r = n & 7, uw = 60 * w + 4;
+  for (int x = 0; x < 8; x++) {
+    const int lo = x * q + std::min(x, r), cnt = q + (x < r ? 1 : 0);
+    std::stable_partition(segs + lo, segs + lo + cnt, [&](const int4& a) {
+      const int2 d = dims[a.x];
+      return !(a.y >= 1 && a.y + uw - 1 <= d.x - 2);  // the column-edge path
+    });
+  }
+}
+
 int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out, int w) {
   if (rows < 1) rows = 1;
   const int cols = chain_cols(ns_, w);
