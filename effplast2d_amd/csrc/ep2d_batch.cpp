@@ -316,11 +316,13 @@ int Batch::buildWork(Shape sh, int4* out, bool esc) const {
         t++;
       }
   }
-  if (out && chain(sh.kind)) {
+  if (chain(sh.kind)) {
     std::vector<int2> dims(active_.size());
     for (size_t k = 0; k < active_.size(); k++)
       dims[k] = make_int2(active_[k].s->nX(), active_[k].s->nY());
-    chain_edges_first(out, t, ipl(sh.kind), stageWaves(sh.kind), dims.data());
+    const int res = chain_resident(ipl(sh.kind), stageWaves(sh.kind),
+                                   opt_.mode == EP2D_MODE_SPLIT && !esc);
+    t = chain_work_order(out, t, stageWaves(sh.kind), dims.data(), res, out);
   }
   return t;
 }

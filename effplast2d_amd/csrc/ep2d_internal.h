@@ -160,10 +160,15 @@ void launch_march2(const RveDev* rves, const int4* segs, int nsegs, int mode, in
 // launch_march2.
 constexpr int chain_cols(int ns, int w = 1) { return 60 * w + 4 - 4 * ns; }
 int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out, int w = 1);
-// Column-edge segments first within each XCD's share of a chain work list
-// of n segments (ns stages of w waves; dims[rve] = (nX, nY) of the RVE a
-// segment's .x names), the others in their order.
-void chain_edges_first(int4* segs, int n, int ns, int w, const int2* dims);
+// The launch order of a chain work list of n segments (stages of w waves;
+// dims[rve] = (nX, nY) of the RVE a segment's .x names): column-edge
+// segments first in each XCD's share, the last twelfth of each share cut
+// into half-height segments unless the device holds all n at once
+// (resident workgroups).  Writes the ordered list to out (which may be in)
+// and returns its length; with out null, returns the length only.
+int chain_work_order(const int4* in, int n, int w, const int2* dims, int resident, int4* out);
+// workgroups of k_chain<., ., ns, w> (FP32 or FP64 stress) the device holds at once
+int chain_resident(int ns, int w, bool f32);
 void launch_chain(int ns, const RveDev* rves, const int4* segs, int nsegs, int mode, int parity,
                   int vpar, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr,
                   int w = 1);

@@ -3187,24 +3187,63 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
   return n;
 }
 
-// The chain kernel gives XCD x the contiguous index range [x q + min(x, r),
-// ...) of an n-entry list (q = n / 8, r = n % 8, tile_of_block), and an XCD
-// starts its workgroups in index order.  Segments on the column-edge path
-// take about a third longer than interior ones (20 x 1024^2, four stages on
-// two waves: 123 against 90 us); started first, they leave workgroups of
-// more even length for the end of the launch (3% shorter launches there;
-// moving the row-edge segments forward as well gains less).  Segments write
-// disjoint outputs, so the order changes no result.
-void chain_edges_first(int4* segs, int n, int ns, int w, const int2* dims) {
-  (void)ns;
-  const int q = n >> 3, r = n & 7, uw = 60 * w + 4;
+// Order of a chain work list.  The chain kernel gives XCD x the contiguous
+// index range [x q + min(x, r), ...) of an n-entry list (q = n / 8,
+// r = n % 8, tile_of_block), and an XCD starts its workgroups in index
+// order.  Two measures shorten the end of a launch, where few workgroups
+// are left (20 x 1024^2, four stages on two waves):
+//  * column-edge segments first in each XCD's share: they run about a third
+//    longer than interior ones (123 against 90 us), and started first they
+//    leave workgroups of more even length for the end (3% shorter launches;
+//    moving the row-edge segments forward as well gains less);
+//  * the last twelfth of each share cut into half-height segments, so the
+//    last workgroups to start end sooner (2% more; a sixteenth and an
+//    eighth gain less).
+// Each XCD's share grows by the number of segments cut, so the shares stay
+// where the kernel looks for them.  Segments write disjoint outputs (a
+// one-row segment is listed twice instead of cut: the same outputs twice),
+// so the order and the cuts change no result.  A list that the GPU holds at
+// once (n <= resident workgroups) has no last round and is not cut.
+// Returns the entries of the ordered list; out may be the input (or null,
+// for the count alone).
+int chain_work_order(const int4* in, int n, int w, const int2* dims, int resident,
+                     int4* out) {
+  const int q = n >> 3, r = n & 7, sp = n > resident ? q / 12 : 0, q2 = q + sp;
+  const int uw = 60 * w + 4;
+  if (!out) return n + 8 * sp;
+  const std::vector<int4> tmp(in, in + n);
   for (int x = 0; x < 8; x++) {
     const int lo = x * q + std::min(x, r), cnt = q + (x < r ? 1 : 0);
-    std::stable_partition(segs + lo, segs + lo + cnt, [&](const int4& a) {
+    std::vector<int4> share(tmp.begin() + lo, tmp.begin() + lo + cnt);
+    std::stable_partition(share.begin(), share.end(), [&](const int4& a) {
       const int2 d = dims[a.x];
       return !(a.y >= 1 && a.y + uw - 1 <= d.x - 2);  // the column-edge path
     });
+    int4* o = out + x * q2 + std::min(x, r);
+    for (int k = 0; k < cnt - sp; k++) *o++ = share[(size_t)k];
+    for (int k = cnt - sp; k < cnt; k++) {
+      const int4 a = share[(size_t)k];
+      const int m = a.z + (a.w - a.z) / 2;
+      *o++ = m > a.z ? make_int4(a.x, a.y, a.z, m) : a;
+      *o++ = m > a.z ? make_int4(a.x, a.y, m, a.w) : a;
+    }
   }
+  return n + 8 * sp;
+}
+
+// Workgroups of k_chain<., ., ns, w> the device holds at once.
+int chain_resident(int ns, int w, bool f32) {
+  int nb = 0, dev = 0, cus = 0;
+  auto occ = [&](auto kern) {
+    EP2D_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)kern, 64 * ns * w, 0));
+  };
+  if (w == 2 && ns == 4) f32 ? occ(k_chain<float, float, 4, 2>) : occ(k_chain<double, double, 4, 2>);
+  else if (w == 2) f32 ? occ(k_chain<float, float, 3, 2>) : occ(k_chain<double, double, 3, 2>);
+  else if (ns == 4) f32 ? occ(k_chain<float, float, 4, 1>) : occ(k_chain<double, double, 4, 1>);
+  else f32 ? occ(k_chain<float, float, 3, 1>) : occ(k_chain<double, double, 3, 1>);
+  EP2D_HIP(hipGetDevice(&dev));
+  EP2D_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return std::max(nb, 1) * std::max(cus, 1);
 }
 
 int chain_segments(int ns_, int nX, int nY, int rows, int rve, int4* out, int w) {

@@ -277,8 +277,8 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
     // automatic shape for one RVE, from one-RVE measurements: split from
     // 1024^2 on four-stage chains with two waves per stage (four iterations
-    // per launch; 14-row segments at 1024^2, 48 at 2048^2, 64 beyond: 8.0,
-    // 17.8 and 60.3 us per iteration at 1024^2, 2048^2 and 4096^2, against
+    // per launch; 14-row segments at 1024^2, 48 at 2048^2, 96 beyond: 7.4,
+    // 16.4 and 57.6 us per iteration at 1024^2, 2048^2 and 4096^2, against
     // 10.3, 23.9 and 77.8 on one wave per stage and 12.1, 33.6 and 170.6
     // for the tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the
     // tile kernel below those sizes.  At 1024^2 the 740 workgroups of
@@ -286,7 +286,7 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
     // leave SIMDs idle (8.6 us), 12 rows need a second round (9.8 us).
     const int n = std::max(nX, nY);
     if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 14 : (n <= 2048 ? 48 : 64);
+      opt_.marchRows = n <= 1024 ? 14 : (n <= 2048 ? 48 : 96);
       opt_.itersPerLaunch = 4;
       if (opt_.laneColumns != 2 && opt_.stageWaves == 0) opt_.stageWaves = 2;
     } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {
@@ -334,11 +334,14 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
           if (wide_) return wide_segments(ipl_, nX, nY, marchRows_, 0, out);
           return chain_segments(ipl_, nX, nY, marchRows_, 0, out, stageWaves_);
         };
-        nSegs2_ = segsOf(nullptr);
+        const int raw = segsOf(nullptr);
+        const int2 dims = make_int2(nX, nY);
+        const bool ordered = ipl_ >= 3 && !wide_;
+        const int res = ordered ? chain_resident(ipl_, stageWaves_, mode_ == EP2D_MODE_SPLIT) : 0;
+        nSegs2_ = ordered ? chain_work_order(nullptr, raw, stageWaves_, &dims, res, nullptr) : raw;
         std::vector<int4> s2((size_t)nSegs2_);
         segsOf(s2.data());
-        const int2 dims = make_int2(nX, nY);
-        if (ipl_ >= 3 && !wide_) chain_edges_first(s2.data(), nSegs2_, ipl_, stageWaves_, &dims);
+        if (ordered) chain_work_order(s2.data(), raw, stageWaves_, &dims, res, s2.data());
         dSegs2_ = (int4*)alloc(s2.size() * sizeof(int4));
         EP2D_HIP(hipMemcpyAsync(dSegs2_, s2.data(), s2.size() * sizeof(int4),
                                 hipMemcpyHostToDevice, stream_));
@@ -348,10 +351,12 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
         std::vector<int4> c3, c2((size_t)march2_segments(nX, nY, 26, 0, nullptr));
         march2_segments(nX, nY, 26, 0, c2.data());
         if (ipl_ == 4) {
-          c3.resize((size_t)chain_segments(3, nX, nY, 42, 0, nullptr, 1));
-          chain_segments(3, nX, nY, 42, 0, c3.data(), 1);
+          const int raw = chain_segments(3, nX, nY, 42, 0, nullptr, 1);
           const int2 dims = make_int2(nX, nY);
-          chain_edges_first(c3.data(), (int)c3.size(), 3, 1, &dims);
+          const int res = chain_resident(3, 1, mode_ == EP2D_MODE_SPLIT);
+          c3.resize((size_t)chain_work_order(nullptr, raw, 1, &dims, res, nullptr));
+          chain_segments(3, nX, nY, 42, 0, c3.data(), 1);
+          chain_work_order(c3.data(), raw, 1, &dims, res, c3.data());
         }
         nSegsC3_ = (int)c3.size();
         nSegsC2_ = (int)c2.size();
