@@ -163,7 +163,7 @@ int chain_segments(int ns, int nX, int nY, int rows, int rve, int4* out, int w =
 // The launch order of a chain work list of n segments (stages of w waves;
 // dims[rve] = (nX, nY) of the RVE a segment's .x names): column-edge
 // segments first in each XCD's share, the last twelfth of each share cut
-// into half-height segments unless the device holds all n at once
+// into halves and quarters of their height unless the device holds all n at once
 // (resident workgroups).  Writes the ordered list to out (which may be in)
 // and returns its length; with out null, returns the length only.
 int chain_work_order(const int4* in, int n, int w, const int2* dims, int resident, int4* out);

@@ -3196,21 +3196,23 @@ int march2_segments(int nX, int nY, int rows, int rve, int4* out) {
 //    longer than interior ones (123 against 90 us), and started first they
 //    leave workgroups of more even length for the end (3% shorter launches;
 //    moving the row-edge segments forward as well gains less);
-//  * the last twelfth of each share cut into half-height segments, so the
-//    last workgroups to start end sooner (2% more; a sixteenth and an
-//    eighth gain less).
-// Each XCD's share grows by the number of segments cut, so the shares stay
-// where the kernel looks for them.  Segments write disjoint outputs (a
-// one-row segment is listed twice instead of cut: the same outputs twice),
-// so the order and the cuts change no result.  A list that the GPU holds at
+//  * the last twelfth of each share cut into shorter segments, so the last
+//    workgroups to start end sooner: halves, and quarters for the last
+//    forty-eighth (2.6% and 0.8% more; cutting a sixteenth or an eighth
+//    into halves gains less).
+// Each XCD's share grows by the pieces added, so the shares stay where the
+// kernel looks for them.  Segments write disjoint outputs (a piece with no
+// rows is replaced by the whole segment: the same outputs twice), so the
+// order and the cuts change no result.  A list that the GPU holds at
 // once (n <= resident workgroups) has no last round and is not cut.
 // Returns the entries of the ordered list; out may be the input (or null,
 // for the count alone).
 int chain_work_order(const int4* in, int n, int w, const int2* dims, int resident,
                      int4* out) {
-  const int q = n >> 3, r = n & 7, sp = n > resident ? q / 12 : 0, q2 = q + sp;
+  const int q = n >> 3, r = n & 7, sp = n > resident ? q / 12 : 0;
+  const int s4 = n > resident ? q / 48 : 0, s2 = sp - s4, q2 = q + s2 + 3 * s4;
   const int uw = 60 * w + 4;
-  if (!out) return n + 8 * sp;
+  if (!out) return n + 8 * (s2 + 3 * s4);
   const std::vector<int4> tmp(in, in + n);
   for (int x = 0; x < 8; x++) {
     const int lo = x * q + std::min(x, r), cnt = q + (x < r ? 1 : 0);
@@ -3223,12 +3225,14 @@ int chain_work_order(const int4* in, int n, int w, const int2* dims, int residen
     for (int k = 0; k < cnt - sp; k++) *o++ = share[(size_t)k];
     for (int k = cnt - sp; k < cnt; k++) {
       const int4 a = share[(size_t)k];
-      const int m = a.z + (a.w - a.z) / 2;
-      *o++ = m > a.z ? make_int4(a.x, a.y, a.z, m) : a;
-      *o++ = m > a.z ? make_int4(a.x, a.y, m, a.w) : a;
+      const int parts = k < cnt - s4 ? 2 : 4;
+      for (int p = 0; p < parts; p++) {
+        const int b0 = a.z + (a.w - a.z) * p / parts, b1 = a.z + (a.w - a.z) * (p + 1) / parts;
+        *o++ = b1 > b0 ? make_int4(a.x, a.y, b0, b1) : a;
+      }
     }
   }
-  return n + 8 * sp;
+  return n + 8 * (s2 + 3 * s4);
 }
 
 // Workgroups of k_chain<., ., ns, w> the device holds at once.
