@@ -413,6 +413,33 @@ def _run_alone(rq, mode="split"):
         return g.ComputeEffModuli(rq["initLoad"], rq["loadType"])
 
 
+@pytest.mark.parametrize("mode", ["split", "fp64"])
+def test_cut_work_lists_equal_single(mode):
+    """A batch whose chain work list spans more than one round of the GPU:
+    24 RVEs of 161^2 on 8-row segments of the two-wave four-stage chains
+    (42 segments each, 1008 in all; each XCD's share of 126 is three RVEs).
+    The list is reordered and the end of each share cut into half- and
+    quarter-height segments; the last band of each RVE has one row, so its
+    segments are listed several times instead of cut.  Every result equals
+    the request's solve alone (tile kernel) bit for bit."""
+    _note(f"cut work lists {mode}")
+    reqs = []
+    for t in range(24):
+        pa = make_pa(161, nPores=1 + t % 3, porosity=0.01 + 0.002 * (t % 5), Y=2.5e-3 + 1e-4 * t)
+        reqs.append(dict(pa=pa, nX=161, nTasks=3, nIter=600, eIter=1e-300,
+                         initLoad=-2e-3 - 1e-4 * (t % 4), loadType=LT, tag=t))
+    with Batch(mode=mode, checkInterval=K, quad2Rows=8) as b:
+        for rq in reqs:
+            b.submit(**rq)
+        b.run()
+        got = {r["tag"]: r for r in b.poll()}
+        st = b.stats()
+    assert st["kernel"] == 12 and st["kernelRows"] == 8
+    for rq in reqs:
+        assert got[rq["tag"]]["status"] == 0
+        _same(got[rq["tag"]]["result"], _run_alone(rq, mode))
+
+
 def test_admission_estimate_is_what_an_rve_holds():
     """The bytes counted at admission are the bytes an admitted RVE holds."""
     reqs = mixed_requests(3, 5, sizes=(64, 96, 160))
