@@ -535,10 +535,24 @@ Batch::Seq Batch::launchChunk(long c) {
     // a warm-up round, then the timed rounds, candidates interleaved; a
     // candidate's two launches of a round run back to back between one pair
     // of events (so a multi-iteration candidate leaves V back in its buffer
-    // before the next candidate)
-    for (long round = 0; round < rounds; round++)
+    // before the next candidate).  After the first timed round the
+    // candidates more than a quarter slower than the fastest drop out: the
+    // later rounds only rank the close ones (the tuning chunk's iterations
+    // on slow shapes cost time)
+    bool keep[kNCand];
+    for (int k = 0; k < kNCand; k++) keep[k] = allowed(k);
+    for (long round = 0; round < rounds; round++) {
+      if (round == 2) {
+        EP2D_HIP(hipStreamSynchronize(stream_));
+        resolveEvents();
+        double best = 0.0;
+        for (int k = 0; k < kNCand; k++)
+          if (keep[k] && tuneN_[k] && (best == 0.0 || st_.candUs[k] < best)) best = st_.candUs[k];
+        for (int k = 0; k < kNCand; k++)
+          if (keep[k] && tuneN_[k] && st_.candUs[k] > 1.25 * best) keep[k] = false;
+      }
       for (int k = 0; k < kNCand; k++) {
-        if (!allowed(k)) continue;
+        if (!keep[k]) continue;
         const Shape sh = kCand[k];
         const int4* w = dTiles_ + candOff_[k];
         const bool dbl = multiIter(sh.kind);
@@ -560,6 +574,7 @@ Batch::Seq Batch::launchChunk(long c) {
           evPending_.push_back(ev);
         }
       }
+    }
     for (int k = 0; k < kNCand; k++) tuneMs_[k] = 0, tuneN_[k] = 0;
     tunedCells_ = cells;
     tuneDue_ = false;
