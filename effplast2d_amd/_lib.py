@@ -202,6 +202,8 @@ SIGNATURES = {
     "ep2d_set_timing": ([C.c_void_p, C.c_long], C.c_int),
     "ep2d_get_timing": ([C.c_void_p, _dp, C.POINTER(C.c_long), C.POINTER(C.c_char_p)], C.c_int),
     "ep2d_diag_copy": ([C.c_int, C.c_size_t, C.c_int, _dp], C.c_int),
+    "ep2d_diag_chain_order": ([C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                               C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     "ep2d_an_static_fields": ([C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),
                                C.POINTER(AnStatic)], C.c_int),
     "ep2d_save_an_static": ([C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_double),

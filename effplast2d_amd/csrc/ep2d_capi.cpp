@@ -4,6 +4,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ep2d_batch.h"
 
@@ -340,6 +341,34 @@ int ep2d_set_timing(ep2d_handle* h, long sampleEvery) {
 
 int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples, const char** kernelName) {
   return guarded([&] { S(h)->getTiming(avgKernelMs, samples, kernelName); });
+}
+
+int ep2d_diag_chain_order(const int* segs, int n, int stageWaves, const int* dims, int nRves,
+                          int resident, int* out, int outCapacity, int* nOut) {
+  return guarded([&] {
+    if (!segs || !dims || !nOut || n < 0 || nRves < 1 || (stageWaves != 1 && stageWaves != 2))
+      throw ep2d::ArgError("ep2d_diag_chain_order: bad argument");
+    std::vector<int4> in((size_t)n);
+    for (int k = 0; k < n; k++) {
+      in[(size_t)k] = make_int4(segs[4 * k], segs[4 * k + 1], segs[4 * k + 2], segs[4 * k + 3]);
+      if (in[(size_t)k].x < 0 || in[(size_t)k].x >= nRves)
+        throw ep2d::ArgError("ep2d_diag_chain_order: segment names no RVE");
+    }
+    std::vector<int2> d((size_t)nRves);
+    for (int k = 0; k < nRves; k++) d[(size_t)k] = make_int2(dims[2 * k], dims[2 * k + 1]);
+    const int m = ep2d::chain_work_order(in.data(), n, stageWaves, d.data(), resident, nullptr);
+    *nOut = m;
+    if (!out) return;
+    if (outCapacity < m) throw ep2d::ArgError("ep2d_diag_chain_order: out holds fewer entries");
+    std::vector<int4> o((size_t)m);
+    ep2d::chain_work_order(in.data(), n, stageWaves, d.data(), resident, o.data());
+    for (int k = 0; k < m; k++) {
+      out[4 * k] = o[(size_t)k].x;
+      out[4 * k + 1] = o[(size_t)k].y;
+      out[4 * k + 2] = o[(size_t)k].z;
+      out[4 * k + 3] = o[(size_t)k].w;
+    }
+  });
 }
 
 int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs) {

@@ -277,6 +277,17 @@ int ep2d_get_timing(ep2d_handle* h, double* avgKernelMs, long* samples,
  * times on `device` and reports the mean kernel time.  Used to calibrate HBM
  * traffic counters and as the achievable-bandwidth reference. */
 int ep2d_diag_copy(int device, size_t bytes, int reps, double* avgMs);
+/* Diagnostic, no counterpart in the reference (host only, no device call):
+ * the launch order the engine gives a work list of the chain kernels of
+ * stageWaves waves per stage.  segs: n entries of four ints (RVE index,
+ * first column, first row, end row), dims: nRves pairs (nX, nY), resident:
+ * workgroups the device holds at once.  Column-edge segments move to the
+ * front of each XCD's share and, when n > resident, the end of each share is
+ * cut into half- and quarter-height pieces.  *nOut receives the length of
+ * the ordered list, written to out when out is not null (outCapacity
+ * entries of four ints).  For tests of the order's coverage. */
+int ep2d_diag_chain_order(const int* segs, int n, int stageWaves, const int* dims, int nRves,
+                          int resident, int* out, int outCapacity, int* nOut);
 /* Analytic static solution of the single-pore problem beside the numerical
  * fields of the current state (SaveAnStatic2D, EffPlast2D.cu:1060-1292; the
  * reference calls it after ComputeEffModuli with nTasks > 1 and after

@@ -235,3 +235,82 @@ def test_shape_and_size_limits_fail_before_the_device(nX, kw, msg):
     with pytest.raises(ep.Ep2dError, match=msg) as ei:
         ep.EffPlast2D(pa, nX, mode="split", **kw)
     assert "EP2D_ERR_ARG" in str(ei.value)
+
+
+def _chain_list(sizes, ns, w, rows):
+    """Work list of the chain kernels as the engine builds it: per RVE,
+    bands of `rows` rows, strips of 60w+4-4ns output columns starting at
+    column -2ns."""
+    cols = 60 * w + 4 - 4 * ns
+    segs = []
+    for rve, (nx, ny) in enumerate(sizes):
+        for j0 in range(0, ny, rows):
+            for k in range(-(-nx // cols)):
+                segs.append((rve, cols * k - 2 * ns, j0, min(j0 + rows, ny)))
+    return segs
+
+
+def _shares(n):
+    q, r = n // 8, n % 8
+    return [(x * q + min(x, r), q + (1 if x < r else 0)) for x in range(8)]
+
+
+def _covered(segs):
+    return {(a, s, j) for a, s, z, e in segs for j in range(z, e)}
+
+
+@pytest.mark.parametrize("resident", [10 ** 6, 768])
+@pytest.mark.parametrize("rows", [8, 96])
+def test_chain_launch_order_covers_the_same_work(rows, resident):
+    """The launch order of a chain work list (ep2d_diag_chain_order, the
+    engine's own function, host only): with every XCD's share of the list
+    taken as the kernel takes it (contiguous, n/8 entries and one more for
+    the first n%8), each share covers exactly the rows of exactly the strips
+    its share of the input list covered; every piece lies inside one input
+    segment; column-edge segments come first in each share; a list of at
+    most `resident` entries keeps its length, a longer one grows by the
+    pieces cut from the end of each share (one-row segments listed again
+    instead of cut)."""
+    ns, w = 4, 2
+    uw = 60 * w + 4
+    sizes = [(161, 161)] * 5 + [(300, 257), (1024, 1024), (96, 130)]
+    segs = _chain_list(sizes, ns, w, rows)
+    n = len(segs)
+    flat = np.array(segs, dtype=np.int32).ravel()
+    dims = np.array(sizes, dtype=np.int32).ravel()
+    m = C.c_int()
+    L = _lib.lib()
+    assert L.ep2d_diag_chain_order(flat.ctypes.data, n, w, dims.ctypes.data, len(sizes),
+                                   resident, None, 0, C.byref(m)) == 0
+    out = np.zeros(4 * m.value, dtype=np.int32)
+    got = C.c_int()
+    assert L.ep2d_diag_chain_order(flat.ctypes.data, n, w, dims.ctypes.data, len(sizes),
+                                   resident, out.ctypes.data, m.value, C.byref(got)) == 0
+    assert got.value == m.value
+    res = [tuple(int(v) for v in out[4 * k:4 * k + 4]) for k in range(m.value)]
+    if n <= resident:
+        assert m.value == n
+    else:
+        assert m.value > n and (m.value - n) % 8 == 0
+    by_key = {}
+    for a, s, z, e in segs:
+        by_key.setdefault((a, s), []).append((z, e))
+    for a, s, z, e in res:
+        assert z < e
+        assert any(z0 <= z and e <= e0 for z0, e0 in by_key[(a, s)]), (a, s, z, e)
+    for (lo, cnt), (lo2, cnt2) in zip(_shares(n), _shares(m.value)):
+        src, dst = segs[lo:lo + cnt], res[lo2:lo2 + cnt2]
+        assert _covered(dst) == _covered(src)
+        edge = [not (s >= 1 and s + uw - 1 <= sizes[a][0] - 2) for a, s, z, e in dst]
+        assert edge == sorted(edge, reverse=True)
+
+
+def test_chain_launch_order_rejects_bad_arguments():
+    L = _lib.lib()
+    segs = np.array([0, -8, 0, 8, 3, -8, 0, 8], dtype=np.int32)
+    dims = np.array([64, 64], dtype=np.int32)
+    m = C.c_int()
+    assert L.ep2d_diag_chain_order(segs.ctypes.data, 2, 2, dims.ctypes.data, 1, 0, None, 0,
+                                   C.byref(m)) != 0          # RVE 3 of 1
+    assert L.ep2d_diag_chain_order(segs.ctypes.data, 1, 3, dims.ctypes.data, 1, 0, None, 0,
+                                   C.byref(m)) != 0          # three waves per stage
