@@ -275,23 +275,30 @@ void Solver::init(const double pa[EP2D_NPARS], hipStream_t stream) {
   for (double* p : {f_.tauXY, f_.J2XY}) launch_fill(p, nV, 0.0, stream_);
   buildGatherTable();
   if (fused && opt_.tileRows == 0 && opt_.marchRows == 0 && opt_.itersPerLaunch == 0) {
-    // automatic shape for one RVE, from one-RVE measurements: split from
-    // 1024^2 on four-stage chains with two waves per stage (four iterations
-    // per launch; 14-row segments at 1024^2, 48 at 2048^2, 96 beyond: 7.4,
-    // 16.4 and 57.6 us per iteration at 1024^2, 2048^2 and 4096^2, against
-    // 10.3, 23.9 and 77.8 on one wave per stage and 12.1, 33.6 and 170.6
-    // for the tile kernel), FP64 from 2048^2 on wave pairs (26 rows); the
-    // tile kernel below those sizes.  At 1024^2 the 740 workgroups of
-    // 14-row segments are one round of the GPU (three per CU): 16 rows
-    // leave SIMDs idle (8.6 us), 12 rows need a second round (9.8 us).
+    // automatic shape for one RVE, from one-RVE measurements: both schemes
+    // from 1024^2 on four-stage chains with two waves per stage (four
+    // iterations per launch, in the launch order of chain_work_order): at
+    // 1024^2 the shortest segments whose workgroups the device holds in one
+    // round (14 rows for split, 21 for FP64 on 256 CUs: 7.4 and 15.6 us per
+    // iteration; 16 split rows take 8.6, 20 FP64 rows 17.3 and the FP64
+    // tile kernel 16.7), 48 rows at 2048^2 and 96 beyond (split 16.4 and
+    // 57.6 us, FP64 43.0 and 124.3 against 47.6 and 154.7 on wave pairs);
+    // the tile kernel below 1024^2.
     const int n = std::max(nX, nY);
-    if (mode_ == EP2D_MODE_SPLIT && n >= 1024 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = n <= 1024 ? 14 : (n <= 2048 ? 48 : 96);
+    const bool chains = (mode_ == EP2D_MODE_SPLIT || mode_ == EP2D_MODE_FP64) && n >= 1024 &&
+                        multi_iter_fits(nX, nY) &&
+                        !(mode_ == EP2D_MODE_FP64 && opt_.laneColumns == 2);
+    if (chains) {
+      int rows = n <= 2048 ? 48 : 96;
+      if (n <= 1024) {
+        const int cols = chain_cols(4, 2);
+        const int strips = (nX + cols - 1) / cols;
+        const int bands = std::max(1, chain_resident(4, 2, mode_ == EP2D_MODE_SPLIT) / strips);
+        rows = std::max(8, (nY + bands - 1) / bands);
+      }
+      opt_.marchRows = rows;
       opt_.itersPerLaunch = 4;
       if (opt_.laneColumns != 2 && opt_.stageWaves == 0) opt_.stageWaves = 2;
-    } else if (mode_ == EP2D_MODE_FP64 && n >= 2048 && multi_iter_fits(nX, nY)) {
-      opt_.marchRows = 26;
-      opt_.itersPerLaunch = 2;
     }
   }
   if (fused) {
